@@ -1,0 +1,233 @@
+"""ctypes binding of libfpgm_ci.so, the C ABI declared in include/fpgm_ci.h.
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfpgm_ci.so")
+_LIB = None
+
+FPGM_OK = 0
+FPGM_EINVAL = 1
+FPGM_ENOMEM = 2
+FPGM_EDEVICE = 3
+FPGM_ETOOBIG = 4
+FPGM_ETIMEOUT = 5
+
+FPGM_MODE_BATCHED = 0
+FPGM_MODE_PLAIN = 1
+FPGM_F_EXACT = 1
+FPGM_F_FP32ONLY = 2
+
+EXPORTS = [
+    "fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ctx_destroy", "fpgm_last_error", "fpgm_status_str",
+    "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
+    "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats",
+]
+
+
+class FpgmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"fpgm status {status}: {msg}")
+        self.status = status
+
+
+class Test(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("d", C.c_int32), ("z_off", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("g2", C.c_double), ("p_value", C.c_double), ("df", C.c_int32), ("indep", C.c_uint8),
+                ("in_band", C.c_uint8), ("exact", C.c_uint8), ("pad", C.c_uint8)]
+
+
+class PcOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("n_threads", C.c_int32), ("group_size", C.c_int32),
+                ("verbose", C.c_int32), ("round_tests", C.c_int64), ("max_rounds", C.c_int64),
+                ("flags", C.c_uint32), ("band", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("num_ci_test", C.c_int64), ("tests_per_level", C.c_int64 * 64),
+                ("edges_after_level", C.c_int64 * 64), ("executed_per_level", C.c_int64 * 64),
+                ("in_band_per_level", C.c_int64 * 64), ("rounds_per_level", C.c_int64 * 64),
+                ("seconds_per_level", C.c_double * 64), ("n_levels", C.c_int32), ("pad", C.c_int32),
+                ("n_sepsets", C.c_int64), ("sep_z_total", C.c_int64), ("ci_seconds", C.c_double),
+                ("max_rel_err_g2_fp32", C.c_double)]
+
+
+class Sepset(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("level", C.c_int32), ("d", C.c_int32),
+                ("z_off", C.c_int64)]
+
+
+def load(path: str = LIB_PATH):
+    """Loads the library (no GPU call is made)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run `python -m fastpgm_amd.build`")
+    L = C.CDLL(path)
+    vp, i32, i64, f64, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint32
+    L.fpgm_ctx_create.argtypes = [vp, i32, i64, i64, vp, C.POINTER(vp)]
+    L.fpgm_ctx_create_device.argtypes = [vp, i32, i64, i64, vp, C.POINTER(vp)]
+    L.fpgm_ctx_destroy.argtypes = [vp]
+    L.fpgm_ctx_destroy.restype = None
+    L.fpgm_last_error.argtypes = [vp]
+    L.fpgm_last_error.restype = C.c_char_p
+    L.fpgm_status_str.argtypes = [i32]
+    L.fpgm_status_str.restype = C.c_char_p
+    L.fpgm_ci_test.argtypes = [vp, i32, i32, vp, i32, f64, C.POINTER(Result)]
+    L.fpgm_ci_batch.argtypes = [vp, vp, i64, vp, i64, f64, u32, vp]
+    L.fpgm_ci_counts.argtypes = [vp, i32, i32, vp, i32, vp, i64, C.POINTER(i64)]
+    L.fpgm_pc_skeleton.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, C.POINTER(Stats)]
+    L.fpgm_sepset_count.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    L.fpgm_sepset_get.argtypes = [vp, vp, i64, vp, i64]
+    L.fpgm_pchisq.argtypes = [f64, i32]
+    L.fpgm_pchisq.restype = f64
+    L.fpgm_chisq_crit.argtypes = [i32, f64]
+    L.fpgm_chisq_crit.restype = f64
+    L.fpgm_last_kernel_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
+    for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
+                 "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats"):
+        getattr(L, name).restype = i32
+    _LIB = L
+    return L
+
+
+class Context:
+    """Owns an fpgm_ctx: the uint8 columns resident on the GPU."""
+
+    def __init__(self, cols: np.ndarray | None = None, n_rows: int = 0, card=None, *, device_ptr: int = 0,
+                 n_vars: int = 0, stride: int = 0):
+        L = load()
+        self._L = L
+        self.card = np.ascontiguousarray(np.asarray(card, dtype=np.int32))
+        h = C.c_void_p()
+        if device_ptr:
+            st = L.fpgm_ctx_create_device(C.c_void_p(device_ptr), int(n_vars), int(n_rows), int(stride),
+                                          self.card.ctypes.data, C.byref(h))
+            self.n_vars = int(n_vars)
+        else:
+            cols = np.ascontiguousarray(cols, dtype=np.uint8)
+            st = L.fpgm_ctx_create(cols.ctypes.data, cols.shape[0], int(n_rows), cols.shape[1],
+                                   self.card.ctypes.data, C.byref(h))
+            self.n_vars = int(cols.shape[0])
+        if st != FPGM_OK:
+            raise FpgmError(st, L.fpgm_status_str(st).decode())
+        self.h = h
+        self.n_rows = int(n_rows)
+
+    @classmethod
+    def of(cls, sample):
+        return cls(sample.cols, sample.n_rows, sample.card)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.fpgm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != FPGM_OK:
+            raise FpgmError(st, self._L.fpgm_last_error(self.h).decode())
+
+    def ci_test(self, x, y, z=(), alpha=0.05):
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.int32))
+        r = Result()
+        self._check(self._L.fpgm_ci_test(self.h, int(x), int(y), z.ctypes.data if len(z) else None, len(z),
+                                         float(alpha), C.byref(r)))
+        return {"g2": r.g2, "p_value": r.p_value, "df": r.df, "indep": bool(r.indep), "in_band": bool(r.in_band),
+                "exact": bool(r.exact)}
+
+    def ci_batch(self, tests, alpha=0.05, flags=0):
+        """tests: sequence of (x, y, z-tuple)."""
+        n = len(tests)
+        arr = (Test * max(n, 1))()
+        pool = []
+        for i, (x, y, z) in enumerate(tests):
+            arr[i].x, arr[i].y, arr[i].d, arr[i].z_off = int(x), int(y), len(z), len(pool)
+            pool.extend(int(v) for v in z)
+        zp = np.ascontiguousarray(np.asarray(pool if pool else [0], dtype=np.int32))
+        out = (Result * max(n, 1))()
+        self._check(self._L.fpgm_ci_batch(self.h, C.cast(arr, C.c_void_p), n, zp.ctypes.data, len(pool),
+                                          float(alpha), int(flags), C.cast(out, C.c_void_p)))
+        a = np.ctypeslib.as_array(out)[:n]
+        return {"g2": a["g2"].copy(), "p_value": a["p_value"].copy(), "df": a["df"].copy(),
+                "indep": a["indep"].astype(bool), "in_band": a["in_band"].astype(bool),
+                "exact": a["exact"].astype(bool)}
+
+    def counts(self, x, y, z=()):
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.int32))
+        cells = int(np.prod([self.card[x], self.card[y]] + [self.card[v] for v in z]))
+        out = np.zeros(max(cells, 1), dtype=np.int32)
+        nc = C.c_int64()
+        self._check(self._L.fpgm_ci_counts(self.h, int(x), int(y), z.ctypes.data if len(z) else None, len(z),
+                                           out.ctypes.data, len(out), C.byref(nc)))
+        dimz = int(np.prod([self.card[v] for v in z])) if len(z) else 1
+        return out[:nc.value].reshape(dimz, int(self.card[x]), int(self.card[y]))
+
+    def pc_skeleton(self, alpha, depth=1000, mode=FPGM_MODE_BATCHED, flags=0, round_tests=0, verbose=0,
+                    band=0.0, max_rounds=0):
+        n = self.n_vars
+        adj = np.zeros((n, n), dtype=np.uint8)
+        o = PcOpts(mode=mode, n_threads=1, group_size=1, verbose=verbose, round_tests=round_tests,
+                   max_rounds=max_rounds, flags=flags, band=band)
+        st = Stats()
+        self._check(self._L.fpgm_pc_skeleton(self.h, float(alpha), int(depth), C.byref(o), adj.ctypes.data,
+                                             C.byref(st)))
+        nr, nz = C.c_int64(), C.c_int64()
+        self._check(self._L.fpgm_sepset_count(self.h, C.byref(nr), C.byref(nz)))
+        recs = (Sepset * max(1, nr.value))()
+        zs = np.zeros(max(1, nz.value), dtype=np.int32)
+        self._check(self._L.fpgm_sepset_get(self.h, C.cast(recs, C.c_void_p), nr.value, zs.ctypes.data,
+                                            nz.value))
+        seps = {}
+        for k in range(nr.value):
+            r = recs[k]
+            seps[(r.x, r.y)] = (r.level, tuple(int(v) for v in zs[r.z_off:r.z_off + r.d]))
+        L = st.n_levels
+        return {
+            "adj": adj,
+            "sepsets": seps,
+            "num_ci_test": int(st.num_ci_test),
+            "tests_per_level": [int(v) for v in st.tests_per_level[:L]],
+            "edges_after_level": [int(v) for v in st.edges_after_level[:L]],
+            "executed_per_level": [int(v) for v in st.executed_per_level[:L]],
+            "in_band_per_level": [int(v) for v in st.in_band_per_level[:L]],
+            "rounds_per_level": [int(v) for v in st.rounds_per_level[:L]],
+            "seconds_per_level": [float(v) for v in st.seconds_per_level[:L]],
+            "ci_seconds": float(st.ci_seconds),
+            "max_rel_err_g2_fp32": float(st.max_rel_err_g2_fp32),
+        }
+
+    def kernel_stats(self):
+        ms, b, n = C.c_double(), C.c_double(), C.c_int64()
+        self._check(self._L.fpgm_last_kernel_stats(self.h, C.byref(ms), C.byref(b), C.byref(n)))
+        return {"kernel_ms": ms.value, "alg_bytes": b.value, "launches": n.value}
+
+
+def pchisq(x, df):
+    return load().fpgm_pchisq(float(x), int(df))
+
+
+def chisq_crit(df, alpha):
+    return load().fpgm_chisq_crit(int(df), float(alpha))

@@ -1,0 +1,65 @@
+// Device-side data structures and launch wrappers of the CI counting kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fpgm {
+
+constexpr int kWG = 256;                  // threads per workgroup (4 wave64)
+constexpr int kRowsPerThread = 16;        // one 16-byte load per column per step
+constexpr int kRowsPerStep = kWG * kRowsPerThread;  // 4096 rows per workgroup step
+constexpr int kLdsCells = 8192;           // LDS table capacity in int32 cells (32 KiB)
+constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
+constexpr int kCritTab = 4096;            // critical values tabulated for df = 1..kCritTab
+
+// One test as the device sees it (32 bytes).
+struct DevTest {
+    int32_t x, y, d, z_off;  // z_off into the device z pool
+    int32_t cx, cy;
+    int32_t dimz;            // product of the Z cardinalities
+    int32_t n_chunks;        // row chunks of this test (work items)
+    int64_t ws_off;          // int32 offset of the test's workspace, -1 when none
+    int32_t item0;           // first work item of the test
+    int32_t flags;           // kTF* below
+};
+
+constexpr int32_t kTFLdsTable = 1;   // table counted in LDS
+constexpr int32_t kTFLdsMarg = 2;    // single chunk and table+marginals fit in LDS
+constexpr int32_t kTFExact = 4;      // FP64 statistic and p-value regardless of the band
+constexpr int32_t kTFKeep = 8;       // leave the counted table in the workspace (parity reads)
+
+// Result of one test on the device (32 bytes).
+struct DevResult {
+    float g2_32;      // FP32 compensated G^2
+    float abs_sum;    // sum of |terms|, feeds the FP32 error bound
+    int32_t df;
+    uint8_t indep, in_band, exact, pad;
+    double g2_64;     // FP64 G^2 (exact/in-band tests)
+    double p64;       // FP64 p-value (exact/in-band tests), NaN otherwise
+};
+
+struct LaunchArgs {
+    const uint8_t *cols;
+    int64_t stride;
+    int64_t n_rows;
+    const DevTest *tests;
+    const int32_t *zpool;
+    const int32_t *card;
+    const int32_t *item_test;
+    int64_t n_items;
+    int64_t rows_per_chunk;
+    int32_t *ws;
+    uint32_t *done;
+    const double *crit;      // crit[df-1] for df = 1..kCritTab at this alpha
+    double alpha;
+    float band;
+    uint32_t flags;          // FPGM_F_* of the call
+    DevResult *res;
+};
+
+hipError_t launch_count_stat(const LaunchArgs &a, int grid, hipStream_t s);
+hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
+                               const int32_t *card, uint32_t *bad, hipStream_t s);
+
+}  // namespace fpgm

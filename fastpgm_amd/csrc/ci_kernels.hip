@@ -1,0 +1,391 @@
+// Fused count + G^2 kernels of the PC-stable CI stage, written for gfx950 (CDNA4).
+//
+// One work item = (test, row chunk).  A 256-thread workgroup streams the chunk's
+// uint8 columns with 16-byte loads (16 rows per thread per step, 4096 rows per
+// workgroup step), forms the contingency-table index of every row
+//     key = ((z_1 * c(Z_2) + z_2) ... ) * c(X) + x) * c(Y) + y
+// (Counts3D / Counts2D layout, CellTable.cpp:254-291,430-441), and increments an
+// LDS histogram, one copy per wave when the table is small enough.  A test whose
+// rows span several chunks merges the chunk histograms into a global table with
+// integer atomics; the workgroup that retires the last chunk computes the
+// statistic.  Counts are exact int32 throughout.
+//
+// Statistic (IndependenceTest.cpp:94-154): marginals, adjusted df, then G^2 in FP32
+// with Neumaier-compensated summation of terms 2 n log(n n_z / (n_x n_y)), each
+// term's ratio formed from exact int64 products.  When the FP32 value lies within
+// max(band * crit, error bound) of the chi-square critical value (or df exceeds the
+// tabulated range, or the caller asked for exact results) the workgroup recomputes
+// G^2 in FP64 with the reference's expression and evaluates the FP64 p-value.
+#include "ci_kernels.h"
+#include "chisq.h"
+#include "fpgm_ci.h"
+
+namespace fpgm {
+
+namespace {
+
+__device__ __forceinline__ uint32_t byte_of(const uint4 &v, int b)
+{
+    const uint32_t w = (b < 4) ? v.x : (b < 8) ? v.y : (b < 12) ? v.z : v.w;
+    return __builtin_amdgcn_ubfe(w, (b & 3) * 8, 8);
+}
+
+__device__ __forceinline__ void nadd(float &s, float &c, float v)
+{
+    const float t = s + v;
+    if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ void wave_nsum(float &s, float &c)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const float s2 = __shfl_xor(s, o, 64);
+        const float c2 = __shfl_xor(c, o, 64);
+        nadd(s, c, s2);
+        c += c2;
+    }
+}
+
+struct Shared {
+    int32_t zcol[kMaxZLds];
+    int32_t zcard[kMaxZLds];
+    int32_t red_i[kWG / 64];
+    float red_f[3 * (kWG / 64)];
+    double red_d[kWG / 64];
+    int32_t flag;
+    int32_t df;
+    int32_t need64;
+    float g2_32;
+    float abs_sum;
+};
+
+// FP32 term 2 n log(n nk / (ni nj)); the ratio comes from exact int64 products, and
+// log1p is used when the ratio is near 1 so that small log values keep their
+// relative accuracy.
+__device__ __forceinline__ float term32(int32_t n, int32_t ni, int32_t nj, int32_t nk)
+{
+    const int64_t num = (int64_t)n * nk;
+    const int64_t den = (int64_t)ni * nj;
+    const int64_t diff = num - den;
+    const int64_t ad = diff < 0 ? -diff : diff;
+    float L;
+    if (2 * ad <= den) L = log1pf((float)diff / (float)den);
+    else L = logf((float)num / (float)den);
+    return 2.0f * (float)n * L;
+}
+
+// Rows [r_begin, r_end) of test T into histogram h (LDS or global).
+template <bool kGlobal>
+__device__ __forceinline__ void count_rows(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+                                           int64_t r_begin, int64_t r_end, int32_t *h)
+{
+    const uint8_t *colx = a.cols + (int64_t)T.x * a.stride;
+    const uint8_t *coly = a.cols + (int64_t)T.y * a.stride;
+    const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
+    const int d = T.d;
+    for (int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread; r < r_end; r += kRowsPerStep) {
+        uint32_t key[kRowsPerThread];
+#pragma unroll
+        for (int b = 0; b < kRowsPerThread; ++b) key[b] = 0;
+        for (int j = 0; j < d; ++j) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(a.cols + (int64_t)sh.zcol[j] * a.stride + r);
+            const uint32_t c = (uint32_t)sh.zcard[j];
+#pragma unroll
+            for (int b = 0; b < kRowsPerThread; ++b)
+                key[b] = kGlobal ? key[b] * c + byte_of(v, b) : __umul24(key[b], c) + byte_of(v, b);
+        }
+        const uint4 vx = *reinterpret_cast<const uint4 *>(colx + r);
+        const uint4 vy = *reinterpret_cast<const uint4 *>(coly + r);
+#pragma unroll
+        for (int b = 0; b < kRowsPerThread; ++b) {
+            if (kGlobal) key[b] = (key[b] * cx + byte_of(vx, b)) * cy + byte_of(vy, b);
+            else key[b] = __umul24(__umul24(key[b], cx) + byte_of(vx, b), cy) + byte_of(vy, b);
+        }
+        const int64_t nv = r_end - r;
+        if (nv >= kRowsPerThread) {
+#pragma unroll
+            for (int b = 0; b < kRowsPerThread; ++b) atomicAdd(&h[key[b]], 1);
+        } else {
+#pragma unroll
+            for (int b = 0; b < kRowsPerThread; ++b)
+                if (b < nv) atomicAdd(&h[key[b]], 1);
+        }
+    }
+}
+
+// Statistic of test t from its full table (LDS or global) with marginal scratch.
+__device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int32_t *tab,
+                         int32_t *marg, Shared &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cx = T.cx, cy = T.cy;
+    const int64_t dimz = T.dimz, cxy = (int64_t)cx * cy, cells = dimz * cxy;
+    int32_t *ni = marg;
+    int32_t *nj = marg + dimz * cx;
+    int32_t *nk = nj + dimz * cy;
+
+    // marginals n_{x+z}, n_{+yz} (CellTable.cpp:242-250)
+    for (int64_t idx = tid; idx < dimz * cx; idx += kWG) {
+        const int64_t z = idx / cx, i = idx - z * cx;
+        const int32_t *row = tab + z * cxy + i * cy;
+        int32_t s = 0;
+        for (int j = 0; j < cy; ++j) s += row[j];
+        ni[idx] = s;
+    }
+    for (int64_t idx = tid; idx < dimz * cy; idx += kWG) {
+        const int64_t z = idx / cy, j = idx - z * cy;
+        const int32_t *col = tab + z * cxy + j;
+        int32_t s = 0;
+        for (int i = 0; i < cx; ++i) s += col[(int64_t)i * cy];
+        nj[idx] = s;
+    }
+    __syncthreads();
+    // n_{++z} and the adjusted degrees of freedom (IndependenceTest.cpp:96-110)
+    int df_loc = 0;
+    for (int64_t z = tid; z < dimz; z += kWG) {
+        int32_t s = 0;
+        int alx = 0, aly = 0;
+        for (int i = 0; i < cx; ++i) {
+            const int32_t v = ni[z * cx + i];
+            s += v;
+            alx += (v > 0);
+        }
+        for (int j = 0; j < cy; ++j) aly += (nj[z * cy + j] > 0);
+        alx = alx >= 1 ? alx : 1;
+        aly = aly >= 1 ? aly : 1;
+        df_loc += (alx - 1) * (aly - 1);
+        nk[z] = s;
+    }
+    df_loc = wave_sum_i(df_loc);
+    if (lane == 0) sh.red_i[wave] = df_loc;
+    __syncthreads();
+    // FP32 G^2, compensated
+    float s = 0.f, c = 0.f, as = 0.f;
+    for (int64_t cell = tid; cell < cells; cell += kWG) {
+        const int32_t n = tab[cell];
+        if (n == 0) continue;
+        const int64_t z = cell / cxy, rem = cell - z * cxy;
+        const int64_t i = rem / cy, j = rem - i * cy;
+        const float tm = term32(n, ni[z * cx + i], nj[z * cy + j], nk[z]);
+        nadd(s, c, tm);
+        as += fabsf(tm);
+    }
+    wave_nsum(s, c);
+    as = wave_sum_f(as);
+    if (lane == 0) {
+        sh.red_f[3 * wave] = s;
+        sh.red_f[3 * wave + 1] = c;
+        sh.red_f[3 * wave + 2] = as;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int df = 0;
+        float S = 0.f, Cc = 0.f, AS = 0.f;
+        for (int w = 0; w < kWG / 64; ++w) {
+            df += sh.red_i[w];
+            nadd(S, Cc, sh.red_f[3 * w]);
+            Cc += sh.red_f[3 * w + 1];
+            AS += sh.red_f[3 * w + 2];
+        }
+        const float g2 = S + Cc;
+        sh.df = df;
+        sh.g2_32 = g2;
+        sh.abs_sum = AS;
+        int need64 = 0;
+        uint8_t indep = 0, in_band = 0;
+        const bool exact = (T.flags & kTFExact) || (a.flags & FPGM_F_EXACT);
+        if (df == 0) {
+            indep = 1;  // IndependenceTest.cpp:140-142
+            need64 = exact;
+        } else {
+            const double crit = (df <= kCritTab) ? a.crit[df - 1] : NAN;
+            const double bound = 4e-6 * (double)AS + 1e-30;
+            const double width = fmax((double)a.band * crit, bound);
+            in_band = (isnan(crit) || fabs((double)g2 - crit) <= width) ? 1 : 0;
+            if ((a.flags & FPGM_F_FP32ONLY) && !isnan(crit)) {
+                indep = (double)g2 < crit;
+                need64 = exact;
+            } else if (in_band || exact) {
+                need64 = 1;
+            } else {
+                indep = (double)g2 < crit;  // p > alpha  <=>  G^2 < critical value
+            }
+        }
+        sh.need64 = need64;
+        DevResult &R = a.res[t];
+        R.g2_32 = g2;
+        R.abs_sum = AS;
+        R.df = df;
+        R.indep = indep;
+        R.in_band = in_band;
+        R.exact = 0;
+        R.g2_64 = NAN;
+        R.p64 = (df == 0) ? 1.0 : NAN;
+    }
+    __syncthreads();
+    if (sh.need64) {
+        // FP64 G^2 with the reference's expression (IndependenceTest.cpp:134-135)
+        double g = 0.0;
+        for (int64_t cell = tid; cell < cells; cell += kWG) {
+            const int32_t n = tab[cell];
+            if (n == 0) continue;
+            const int64_t z = cell / cxy, rem = cell - z * cxy;
+            const int64_t i = rem / cy, j = rem - i * cy;
+            const double expected = (double)nj[z * cy + j] * (double)ni[z * cx + i] / (double)nk[z];
+            g += 2.0 * (double)n * log((double)n / expected);
+        }
+        g = wave_sum_d(g);
+        if (lane == 0) sh.red_d[wave] = g;
+        __syncthreads();
+        if (tid == 0) {
+            double G = 0.0;
+            for (int w = 0; w < kWG / 64; ++w) G += sh.red_d[w];
+            DevResult &R = a.res[t];
+            R.g2_64 = G;
+            R.exact = 1;
+            if (sh.df == 0) {
+                R.p64 = 1.0;
+                R.indep = 1;
+            } else {
+                const double p = 1.0 - pchisq(G, sh.df);  // IndependenceTest.cpp:146
+                R.p64 = p;
+                R.indep = (p > a.alpha) ? 1 : 0;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
+{
+    __shared__ int32_t lds[kLdsCells];
+    __shared__ Shared sh;
+    const int tid = threadIdx.x, wave = tid >> 6;
+
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so give
+    // each XCD a contiguous slab of work items (neighbouring tests share columns).
+    const int64_t nx = 8;
+    const int64_t xcd = blockIdx.x % nx, local = blockIdx.x / nx, per = gridDim.x / nx;
+    const int64_t slab = (a.n_items + nx - 1) / nx;
+    const int64_t lo = xcd * slab;
+    const int64_t hi = (lo + slab < a.n_items) ? lo + slab : a.n_items;
+
+    for (int64_t item = lo + local; item < hi; item += per) {
+        const int t = a.item_test[item];
+        const DevTest T = a.tests[t];
+        const int64_t chunk = item - T.item0;
+        const int64_t r_begin = chunk * a.rows_per_chunk;
+        const int64_t r_end = (r_begin + a.rows_per_chunk < a.n_rows) ? r_begin + a.rows_per_chunk : a.n_rows;
+        const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
+        for (int j = tid; j < T.d; j += kWG) {
+            const int32_t zc = a.zpool[T.z_off + j];
+            sh.zcol[j] = zc;
+            sh.zcard[j] = a.card[zc];
+        }
+        __syncthreads();
+        int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
+        if (T.flags & kTFLdsTable) {
+            const int c32 = (int)cells;
+            const int copies = (c32 <= kLdsCells / 4) ? 4 : (c32 <= kLdsCells / 2) ? 2 : 1;
+            for (int i = tid; i < copies * c32; i += kWG) lds[i] = 0;
+            __syncthreads();
+            count_rows<false>(a, T, sh, r_begin, r_end, lds + (wave % copies) * c32);
+            __syncthreads();
+            if (copies > 1) {
+                for (int i = tid; i < c32; i += kWG) {
+                    int32_t s = lds[i];
+                    for (int k = 1; k < copies; ++k) s += lds[k * c32 + i];
+                    lds[i] = s;
+                }
+                __syncthreads();
+            }
+            if (T.flags & kTFLdsMarg) {
+                finalize(a, T, t, lds, lds + c32, sh);
+                continue;
+            }
+            for (int i = tid; i < c32; i += kWG) {
+                const int32_t v = lds[i];
+                if (v) atomicAdd(&g[i], v);
+            }
+        } else {
+            count_rows<true>(a, T, sh, r_begin, r_end, g);
+        }
+        // retire the chunk; the workgroup retiring the last chunk owns the statistic
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t prev = atomicAdd(&a.done[t], 1u);
+            sh.flag = (prev + 1 == (uint32_t)T.n_chunks) ? 1 : 0;
+        }
+        __syncthreads();
+        if (sh.flag) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            finalize(a, T, t, g, g + cells, sh);
+            if (!(T.flags & kTFKeep)) {
+                for (int64_t i = tid; i < cells; i += kWG) g[i] = 0;
+            }
+            if (tid == 0) a.done[t] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows,
+                                                     int n_vars, const int32_t *card, uint32_t *bad)
+{
+    const int v = blockIdx.y;
+    if (v >= n_vars) return;
+    const uint32_t c = (uint32_t)card[v];
+    const uint8_t *col = cols + (int64_t)v * stride;
+    uint32_t flag = 0;
+    for (int64_t r = ((int64_t)blockIdx.x * kWG + threadIdx.x) * 16; r < n_rows;
+         r += (int64_t)gridDim.x * kWG * 16) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(col + r);
+        const int64_t nv = n_rows - r;
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            if (b < nv && byte_of(q, b) >= c) flag = 1;
+    }
+    if (flag) atomicOr(bad, 1u);
+}
+
+}  // namespace
+
+hipError_t launch_count_stat(const LaunchArgs &a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_count_stat, dim3(grid), dim3(kWG), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
+                               const int32_t *card, uint32_t *bad, hipStream_t s)
+{
+    if (n_vars <= 0 || n_rows <= 0) return hipSuccess;
+    int64_t steps = (n_rows + (int64_t)kWG * 16 - 1) / ((int64_t)kWG * 16);
+    const int gx = (int)(steps < 64 ? steps : 64);
+    hipLaunchKernelGGL(k_check_values, dim3(gx, n_vars), dim3(kWG), 0, s, cols, stride, n_rows, n_vars, card,
+                       bad);
+    return hipGetLastError();
+}
+
+}  // namespace fpgm

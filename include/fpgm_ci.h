@@ -1,0 +1,172 @@
+/*
+ * fpgm_ci.h -- C ABI of the MI355X (gfx950) PC-stable conditional-independence stage.
+ *
+ * A drop-in for the CI stage of FastPGM's PC-stable skeleton search.  Every entry
+ * point below names the reference interface it replaces (paths relative to the
+ * FastPGM source tree).  Plain C types only; no allocation crosses the ABI: the
+ * caller owns every host buffer, the context owns every device buffer.
+ *
+ * Data: n_vars uint8 columns, column v at cols + v*row_stride, n_rows valid bytes
+ * each (identity value coding, value < card[v] <= 255).  This is the reference's
+ * Dataset::dataset_columns (Dataset.h:32) narrowed from int32 to uint8.
+ *
+ * Errors are returned as fpgm_status codes; nothing exits or throws across the ABI
+ * (the reference calls exit(1): IndependenceTest.cpp:13-17, ChoiceGenerator.cpp:15-18).
+ * Calls on one context are serialised by the context; distinct contexts may be used
+ * from different threads.
+ */
+#ifndef FPGM_CI_H
+#define FPGM_CI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t fpgm_status;
+#define FPGM_OK 0
+#define FPGM_EINVAL 1   /* bad argument (alpha outside [0,1], bad index, value >= card ...) */
+#define FPGM_ENOMEM 2   /* device or host allocation failed                                */
+#define FPGM_EDEVICE 3  /* HIP runtime error or no gfx950 device                           */
+#define FPGM_ETOOBIG 4  /* a contingency table exceeds the dense-table limit               */
+#define FPGM_ETIMEOUT 5 /* a level exceeded the caller's round limit                        */
+
+typedef struct fpgm_ctx fpgm_ctx;
+
+/* One CI test X _||_ Y | Z.  z points into the caller's array of d node ids, in the
+ * order the reference passes them (ascending node id for PC-stable). */
+typedef struct {
+    int32_t x;
+    int32_t y;
+    int32_t d;
+    int32_t z_off; /* offset of Z in the z pool passed with the batch */
+} fpgm_test;
+
+/* IndependenceTest::Result (IndependenceTest.h:38-49) plus the statistic. */
+typedef struct {
+    double g2;        /* G^2: FP64 when exact/rechecked, else the FP32 value widened */
+    double p_value;   /* 1 - pchisq(g2, df); NaN when the FP32 path decided alone    */
+    int32_t df;       /* adjusted degrees of freedom (IndependenceTest.cpp:94-110)    */
+    uint8_t indep;    /* p_value > alpha, or df == 0                                  */
+    uint8_t in_band;  /* FP32 statistic fell in the recheck band, FP64 decided       */
+    uint8_t exact;    /* g2/p_value are FP64                                          */
+    uint8_t pad;
+} fpgm_result;
+
+/* Options of the whole-stage entry point. */
+typedef struct {
+    int32_t mode;          /* FPGM_MODE_BATCHED (default) or FPGM_MODE_PLAIN             */
+    int32_t n_threads;     /* accepted for call-shape parity; results never depend on it */
+    int32_t group_size;    /* accepted for call-shape parity; results never depend on it */
+    int32_t verbose;       /* 0 silent, 1 per-level lines on stderr                       */
+    int64_t round_tests;   /* batched: target tests per speculative round (0 = auto)      */
+    int64_t max_rounds;    /* batched: per-level round cap, 0 = none (FPGM_ETIMEOUT)      */
+    uint32_t flags;        /* FPGM_F_* below                                              */
+    float band;            /* relative half-width of the FP64 recheck band (0 = 0.02)     */
+} fpgm_pc_opts;
+
+#define FPGM_MODE_BATCHED 0
+#define FPGM_MODE_PLAIN 1
+
+#define FPGM_F_EXACT 1u     /* compute FP64 G^2 and p-value for every test              */
+#define FPGM_F_FP32ONLY 2u  /* diagnostics: decide every test from FP32 alone           */
+
+/* Per-level statistics of the last fpgm_pc_skeleton call. */
+typedef struct {
+    int64_t num_ci_test;            /* reference-equivalent tests (group size 1)          */
+    int64_t tests_per_level[64];
+    int64_t edges_after_level[64];
+    int64_t executed_per_level[64]; /* tests the device ran, speculative ones included    */
+    int64_t in_band_per_level[64];  /* tests decided by the FP64 recheck                  */
+    int64_t rounds_per_level[64];
+    double seconds_per_level[64];
+    int32_t n_levels;
+    int32_t pad;
+    int64_t n_sepsets;
+    int64_t sep_z_total;
+    double ci_seconds;              /* the "pc-stable step 1" region, PCStable.cpp:109-232 */
+    double max_rel_err_g2_fp32;     /* over rechecked tests: |g2_32 - g2_64| / g2_64       */
+} fpgm_stats;
+
+/* Sepset record: PCStable::sepset entry (PCStable.h:35), key (x,y) with x<y. */
+typedef struct {
+    int32_t x;
+    int32_t y;
+    int32_t level;
+    int32_t d;
+    int64_t z_off; /* into the z array returned by fpgm_sepset_get */
+} fpgm_sepset;
+
+/*
+ * Context.  Replaces handing a Dataset* to IndependenceTest / PCStable
+ * (IndependenceTest.cpp:12-21, PCStable.cpp:25): uploads the columns once.
+ * fpgm_ctx_create copies host columns to the device; fpgm_ctx_create_device
+ * borrows columns already resident in device memory (the caller keeps them alive).
+ */
+fpgm_status fpgm_ctx_create(const uint8_t *cols, int32_t n_vars, int64_t n_rows,
+                            int64_t row_stride, const int32_t *card, fpgm_ctx **out);
+fpgm_status fpgm_ctx_create_device(const void *d_cols, int32_t n_vars, int64_t n_rows,
+                                   int64_t row_stride, const int32_t *card, fpgm_ctx **out);
+void fpgm_ctx_destroy(fpgm_ctx *ctx);
+const char *fpgm_last_error(const fpgm_ctx *ctx);
+const char *fpgm_status_str(fpgm_status s);
+
+/*
+ * One CI test, one launch, host-synchronous: replaces
+ * IndependenceTest::IndependenceResult(x, y, z, "g square", timer, 1)
+ * (IndependenceTest.cpp:35-58 -> ComputeGSquareXY :295-364 / ComputeGSquareXYZ :65-155).
+ * Exact FP64 G^2 and p-value.
+ */
+fpgm_status fpgm_ci_test(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, int32_t d,
+                         double alpha, fpgm_result *out);
+
+/*
+ * n tests in one batched launch: replaces a sequence of IndependenceResult calls and
+ * the grouped ComputeGSquareXYZGroup (IndependenceTest.cpp:163-287).  flags: FPGM_F_*.
+ */
+fpgm_status fpgm_ci_batch(fpgm_ctx *ctx, const fpgm_test *tests, int64_t n, const int32_t *z_pool,
+                          int64_t z_pool_len, double alpha, uint32_t flags, fpgm_result *out);
+
+/*
+ * Raw contingency table of one test, for parity checks: the dense
+ * n[z][x][y] int32 table of Counts3D / Counts2D (CellTable.cpp:23-91, :174-205,
+ * :226-291, :430-455), z the mixed-radix index with the first Z most significant.
+ * cap = capacity of table in int32 entries; *cells receives the table size.
+ */
+fpgm_status fpgm_ci_counts(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, int32_t d,
+                           int32_t *table, int64_t cap, int64_t *cells);
+
+/*
+ * The whole CI stage on the device: replaces step 1 of
+ * PCStable::StructLearnByPCStable (PCStable.cpp:109-232: complete graph, level 0,
+ * SearchAtDepth per level :265-388).  Results equal the reference's with group size 1:
+ * adj_out (n_vars*n_vars uint8, symmetric) is the skeleton handed to orientation,
+ * sepsets are read back with fpgm_sepset_count / fpgm_sepset_get, stats (may be NULL)
+ * receives num_ci_test and per-level figures.  depth as in PCStable(net, alpha, depth):
+ * levels d = 1 .. depth-1 after level 0.
+ */
+fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
+                             uint8_t *adj_out, fpgm_stats *stats);
+
+/* Sizes of the sepset map of the last fpgm_pc_skeleton call (PCStable::sepset). */
+fpgm_status fpgm_sepset_count(const fpgm_ctx *ctx, int64_t *n_records, int64_t *n_z);
+/* Records in ascending (x,y) order; z receives the sets (ascending node ids). */
+fpgm_status fpgm_sepset_get(const fpgm_ctx *ctx, fpgm_sepset *records, int64_t cap_records,
+                            int32_t *z, int64_t cap_z);
+
+/* chi-square helpers of the library's own FP64 CDF (the reference's stats::pchisq,
+ * called at IndependenceTest.cpp:146,268,355), exposed for checks. */
+double fpgm_pchisq(double x, int32_t df);
+double fpgm_chisq_crit(int32_t df, double alpha);
+
+/* Timing of the last batched launch sequence, for the roofline figure: the summed
+ * duration (HIP events on the context stream) and algorithmic bytes of the counting
+ * kernel launches of the last fpgm_pc_skeleton / fpgm_ci_batch call. */
+fpgm_status fpgm_last_kernel_stats(const fpgm_ctx *ctx, double *kernel_ms, double *alg_bytes,
+                                   int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

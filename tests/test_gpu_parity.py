@@ -1,0 +1,163 @@
+"""HIP path against the oracle, through the C ABI (needs an MI355X)."""
+import numpy as np
+import pytest
+
+from fastpgm_amd import _lib, configs, datagen
+from oracle import oracle as O
+
+from helpers import alarm_sample, alarm_struct
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(sample):
+    return _lib.Context.of(sample)
+
+
+def _rand_tests(rng, n_vars, n, dmax):
+    out = []
+    for _ in range(n):
+        d = int(rng.integers(0, dmax + 1))
+        vs = rng.choice(n_vars, size=d + 2, replace=False)
+        x, y = int(min(vs[0], vs[1])), int(max(vs[0], vs[1]))
+        out.append((x, y, tuple(sorted(int(v) for v in vs[2:]))))
+    return out
+
+
+@pytest.mark.parametrize("n_rows", [1, 15, 4099, 70001])
+def test_counts_bit_exact(n_rows):
+    s = datagen.random_sample(20 + n_rows, 10, n_rows, 1, 5)
+    D = O.Data.of(s)
+    rng = np.random.default_rng(n_rows)
+    with _ctx(s) as ctx:
+        for (x, y, z) in _rand_tests(rng, s.n_vars, 25, 4):
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z)), (x, y, z)
+
+
+def test_counts_large_tables_global_path():
+    # card 8: d = 3 gives 8^3 * 64 = 32768 cells (beyond the LDS table)
+    s = datagen.random_sample(5, 8, 200_003, 8, 8)
+    D = O.Data.of(s)
+    with _ctx(s) as ctx:
+        for z in [(2, 3, 4), (1, 5, 6), (0, 2, 7)]:
+            x, y = [v for v in range(8) if v not in z][:2]
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
+
+
+@pytest.mark.parametrize("flags", [_lib.FPGM_F_EXACT, 0])
+def test_ci_batch_matches_oracle(flags):
+    w = configs.get(1)
+    s = w.sample(60_000)
+    D = O.Data.of(s)
+    rng = np.random.default_rng(4)
+    tests = _rand_tests(rng, s.n_vars, 600, 3)
+    with _ctx(s) as ctx:
+        r = ctx.ci_batch(tests, alpha=0.05, flags=flags)
+    arr = np.array([[x, y, len(z)] + list(z) + [0] * (3 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 3, 0.05, n_threads=8)
+    assert np.array_equal(r["df"], o["df"])
+    assert np.array_equal(r["indep"], o["indep"])
+    if flags & _lib.FPGM_F_EXACT:
+        assert r["exact"].all()
+        np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+        np.testing.assert_allclose(r["p_value"], o["p_value"], rtol=1e-9, atol=1e-12)
+    else:
+        # FP32 statistic within its error bound of the FP64 one
+        np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-3, atol=1e-2)
+
+
+def test_ci_test_single_launch_matches_oracle():
+    s = alarm_sample()
+    D = O.Data.of(s)
+    rng = np.random.default_rng(8)
+    with _ctx(s) as ctx:
+        for (x, y, z) in _rand_tests(rng, s.n_vars, 20, 3):
+            r = ctx.ci_test(x, y, z, alpha=0.05)
+            o = O.ci_test(D, x, y, z, alpha=0.05)
+            assert r["df"] == o["df"] and r["indep"] == o["indep"]
+            assert r["g2"] == pytest.approx(o["g2"], rel=1e-11, abs=1e-9)
+            assert r["p_value"] == pytest.approx(o["p_value"], rel=1e-9, abs=1e-12)
+
+
+def _assert_same(g, o):
+    assert np.array_equal(g["adj"], o["adj"])
+    assert g["sepsets"] == o["sepsets"]
+    assert g["tests_per_level"] == o["tests_per_level"]
+    assert g["edges_after_level"] == o["edges_after_level"]
+    assert g["num_ci_test"] == o["num_ci_test"]
+
+
+@pytest.mark.parametrize("mode", [_lib.FPGM_MODE_BATCHED, _lib.FPGM_MODE_PLAIN])
+def test_skeleton_alarm(mode):
+    s = alarm_sample()
+    o = O.pc_skeleton(O.Data.of(s), 0.05, 1000, n_threads=8)
+    with _ctx(s) as ctx:
+        g = ctx.pc_skeleton(0.05, 1000, mode=mode)
+    _assert_same(g, o)
+    got = sorted([i, j] for i in range(s.n_vars) for j in range(i + 1, s.n_vars) if g["adj"][i, j])
+    assert got == alarm_struct()["adjacencies"]
+
+
+@pytest.mark.parametrize("key", [0, 1])
+def test_skeleton_workloads(key):
+    w = configs.get(key)
+    s = w.sample()
+    o = O.pc_skeleton(O.Data.of(s), w.alpha, w.depth, n_threads=8)
+    with _ctx(s) as ctx:
+        g = ctx.pc_skeleton(w.alpha, w.depth)
+    _assert_same(g, o)
+
+
+def test_skeleton_round_size_does_not_change_results():
+    s = configs.get(0).sample()
+    with _ctx(s) as ctx:
+        a = ctx.pc_skeleton(0.05, 1000, round_tests=1)
+        b = ctx.pc_skeleton(0.05, 1000, round_tests=1 << 20)
+        c = ctx.pc_skeleton(0.05, 1000, flags=_lib.FPGM_F_EXACT)
+    _assert_same(a, b)
+    _assert_same(a, c)
+    assert sum(c["in_band_per_level"]) <= sum(c["executed_per_level"])
+
+
+def test_skeleton_depth_and_alpha():
+    w = configs.get(2)
+    s = w.sample(20_000)
+    D = O.Data.of(s)
+    with _ctx(s) as ctx:
+        for alpha, depth in ((0.01, 5), (0.05, 2), (0.2, 3)):
+            o = O.pc_skeleton(D, alpha, depth, n_threads=8)
+            g = ctx.pc_skeleton(alpha, depth)
+            _assert_same(g, o)
+
+
+def test_edge_cases():
+    # no rows: every level-0 table is empty, df = 0, all edges removed
+    s = datagen.random_sample(1, 5, 0, 2, 3)
+    with _ctx(s) as ctx:
+        g = ctx.pc_skeleton(0.05, 1000)
+    assert g["adj"].sum() == 0 and len(g["sepsets"]) == 10
+    # a constant column (cardinality 1)
+    rows = np.random.default_rng(0).integers(0, 2, size=(500, 4))
+    rows[:, 2] = 0
+    s = datagen.sample_from_rows(rows, [2, 2, 1, 2])
+    D = O.Data.of(s)
+    with _ctx(s) as ctx:
+        assert np.array_equal(ctx.counts(0, 1, [2]), O.counts(D, 0, 1, [2]))
+        _assert_same(ctx.pc_skeleton(0.05, 1000), O.pc_skeleton(D, 0.05, 1000))
+
+
+def test_errors_are_status_codes():
+    s = datagen.random_sample(2, 4, 100, 2, 3)
+    bad = s.cols.copy()
+    bad[1, 7] = 9  # value >= cardinality
+    with pytest.raises(_lib.FpgmError) as e:
+        _lib.Context(bad, s.n_rows, s.card)
+    assert e.value.status == _lib.FPGM_EINVAL
+    with _ctx(s) as ctx:
+        with pytest.raises(_lib.FpgmError) as e:
+            ctx.ci_test(0, 1, (), alpha=1.5)
+        assert e.value.status == _lib.FPGM_EINVAL
+        with pytest.raises(_lib.FpgmError):
+            ctx.ci_test(0, 0, ())
+        with pytest.raises(_lib.FpgmError):
+            ctx.pc_skeleton(-0.1)
