@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfpgm_ci.so")
+GEN_LIB = os.path.join(HERE, "libfpgm_gen.so")
 
 SOURCES = ["ci_kernels.hip", "fpgm_ci.hip"]
 HEADERS = ["ci_kernels.h", "chisq.h"]
@@ -36,7 +37,20 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+def build_gen(verbose: bool = False) -> str:
+    """Host-only synthetic-data sampler (csrc/gen.c)."""
+    tmp = GEN_LIB + ".tmp"
+    cmd = [os.environ.get("CC", "gcc"), "-O3", "-fopenmp", "-fPIC", "-shared", "-o", tmp,
+           os.path.join(CSRC, "gen.c")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, GEN_LIB)
+    return GEN_LIB
+
+
 def build(verbose: bool = False) -> str:
+    build_gen(verbose)
     objs = []
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
     bdir = os.path.join(HERE, "_build")

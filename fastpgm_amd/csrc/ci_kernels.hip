@@ -342,7 +342,10 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             finalize(a, T, t, g, g + cells, sh);
             if (!(T.flags & kTFKeep)) {
-                for (int64_t i = tid; i < cells; i += kWG) g[i] = 0;
+                // the workspace must read zero wherever the next launch places a table,
+                // so the marginal scratch behind the table is cleared as well
+                const int64_t used = cells + (int64_t)T.dimz * (T.cx + T.cy + 1);
+                for (int64_t i = tid; i < used; i += kWG) g[i] = 0;
             }
             if (tid == 0) a.done[t] = 0;
         }

@@ -596,7 +596,10 @@ fpgm_status fpgm_ci_counts(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z
     if (st != FPGM_OK) return st;
     HIP_TRY(hipMemcpyAsync(table, ctx->d_ws.p + off, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, ctx->stream),
             "download table");
-    HIP_TRY(hipMemsetAsync(ctx->d_ws.p + off, 0, sizeof(int32_t) * nc, ctx->stream), "clear table");
+    int64_t dimz = 1;
+    for (int j = 0; j < d; ++j) dimz *= ctx->card[z[j]];
+    const int64_t used = nc + dimz * (ctx->card[x] + ctx->card[y] + 1);
+    HIP_TRY(hipMemsetAsync(ctx->d_ws.p + off, 0, sizeof(int32_t) * used, ctx->stream), "clear table");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
     return FPGM_OK;
 }

@@ -131,31 +131,65 @@ def make_network(seed: int, n: int, n_edges: int, max_in: int, card_law, conc: f
     return Network(n_vars=n, card=card, parents=parents, order=order, cpts=cpts)
 
 
+_GEN = None
+
+
+def _gen_lib():
+    """Native sampler (csrc/gen.c); built on first use when absent."""
+    global _GEN
+    if _GEN is None:
+        import ctypes as C
+        import os
+        from . import build as B
+        if not os.path.exists(B.GEN_LIB):
+            B.build_gen()
+        L = C.CDLL(B.GEN_LIB)
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.fpgm_gen_forward.restype = None
+        L.fpgm_gen_forward.argtypes = [i32, i64, i64, vp, vp, vp, vp, vp, vp, C.c_uint64, vp]
+        L.fpgm_gen_gather.restype = None
+        L.fpgm_gen_gather.argtypes = [i32, i64, vp, i64, vp, vp, i64]
+        _GEN = L
+    return _GEN
+
+
 def forward_sample(net: Network, n_rows: int, seed: int, stream: int = 2) -> Sample:
-    """Ancestral sampling of ``n_rows`` rows, one variable at a time over all rows."""
+    """Ancestral sampling of ``n_rows`` rows.  Row r's draw for variable v is a
+    counter-based uniform of (seed stream, v, r), so the sample is independent of
+    the thread count and of how rows are split."""
     ss = np.random.SeedSequence(seed)
-    rng = np.random.Generator(np.random.PCG64(ss.spawn(4)[stream]))
+    row_seed = int(ss.spawn(4)[stream].generate_state(1, dtype=np.uint64)[0])
     stride = padded_stride(n_rows)
     cols = np.zeros((net.n_vars, stride), dtype=np.uint8)
-    for v in net.order:
-        ps = net.parents[v]
-        cfg = np.zeros(n_rows, dtype=np.int64)
-        for p in ps:
-            cfg = cfg * int(net.card[p]) + cols[p, :n_rows]
-        cum = np.cumsum(net.cpts[v], axis=1)
-        u = rng.random(n_rows)
-        state = (u[:, None] >= cum[cfg]).sum(axis=1)
-        np.minimum(state, int(net.card[v]) - 1, out=state)
-        cols[v, :n_rows] = state.astype(np.uint8)
+    order = np.ascontiguousarray(net.order, dtype=np.int32)
+    card = np.ascontiguousarray(net.card, dtype=np.int32)
+    par_off = np.zeros(net.n_vars + 1, dtype=np.int64)
+    for v in range(net.n_vars):
+        par_off[v + 1] = par_off[v] + len(net.parents[v])
+    par = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.int32) for p in net.parents])
+                               if par_off[-1] else np.zeros(1, np.int32), dtype=np.int32)
+    cums = [np.cumsum(c, axis=1).ravel() for c in net.cpts]
+    cpt_off = np.zeros(net.n_vars, dtype=np.int64)
+    acc = 0
+    for v in range(net.n_vars):
+        cpt_off[v] = acc
+        acc += len(cums[v])
+    cum = np.ascontiguousarray(np.concatenate(cums), dtype=np.float64)
+    if n_rows > 0:
+        _gen_lib().fpgm_gen_forward(net.n_vars, n_rows, stride, order.ctypes.data, card.ctypes.data,
+                                    par_off.ctypes.data, par.ctypes.data, cpt_off.ctypes.data, cum.ctypes.data,
+                                    row_seed, cols.ctypes.data)
     return Sample(cols=cols, n_rows=n_rows, card=net.card.astype(np.int32).copy())
 
 
 def bootstrap(base: Sample, n_rows: int, rng: np.random.Generator) -> Sample:
     """Rows drawn with replacement from ``base``."""
-    idx = rng.integers(0, base.n_rows, size=n_rows)
+    idx = np.ascontiguousarray(rng.integers(0, base.n_rows, size=n_rows), dtype=np.int64)
     stride = padded_stride(n_rows)
     cols = np.zeros((base.n_vars, stride), dtype=np.uint8)
-    cols[:, :n_rows] = base.cols[:, idx]
+    if n_rows > 0:
+        _gen_lib().fpgm_gen_gather(base.n_vars, n_rows, base.cols.ctypes.data, base.stride, idx.ctypes.data,
+                                   cols.ctypes.data, stride)
     return Sample(cols=cols, n_rows=n_rows, card=base.card.copy())
 
 
