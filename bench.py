@@ -1,0 +1,189 @@
+"""PC-stable CI-stage benchmark on one MI355X.
+
+One step = the whole CI stage (level 0 through the last level, PCStable.cpp:109-232)
+of PC-stable on a seeded synthetic workload whose uint8 columns are already resident
+in HBM.  The metric is reference-equivalent CI tests per second (num_ci_test with group
+size 1, speculative tests excluded) over the timed steps.
+
+    python bench.py [--steps K] [--warmup W] [--workload 3]
+
+Prints one JSON line.  Beside the metric it reports the plain one-launch-per-test form
+(first tests of every level, sampled, the rest batched so the stage stays exact), the
+counting kernel's achieved bandwidth against HBM peak, and the CPU restatement timed on
+a bounded sample of the same levels on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fastpgm_amd import _lib, configs  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def adjacency_before_levels(n, sepsets, n_levels):
+    """Adjacency at the start of each level, rebuilt from the sepsets' removal levels."""
+    full = np.ones((n, n), dtype=np.uint8) - np.eye(n, dtype=np.uint8)
+    out = []
+    for lvl in range(n_levels):
+        a = full.copy()
+        for (x, y), (rl, _z) in sepsets.items():
+            if rl < lvl:
+                a[x, y] = a[y, x] = 0
+        out.append(a)
+    return out
+
+
+def cpu_baseline(sample, w, g, seconds, threads):
+    """The oracle (CPU restatement) on a bounded sample of every level: the first edges of
+    the level's frozen adjacency, sized to about seconds / levels each; the stage's CPU
+    time is then extrapolated level by level from the measured rates."""
+    from oracle import oracle as O
+    D = O.Data.of(sample)
+    L = len(g["tests_per_level"])
+    adjs = adjacency_before_levels(sample.n_vars, g["sepsets"], L)
+    per = max(0.5, seconds / max(1, sum(1 for t in g["tests_per_level"] if t > 0)))
+    est, sampled, sampled_s = 0.0, 0, 0.0
+    for lvl in range(L):
+        tests_l = g["tests_per_level"][lvl]
+        if tests_l == 0:
+            continue
+        n_edges = int(adjs[lvl][np.triu_indices(sample.n_vars, 1)].sum())
+        m = min(n_edges, 16)
+        t0 = time.perf_counter()
+        k, _ = O.level_sample(D, w.alpha, lvl, adjs[lvl], m, threads)
+        dt = time.perf_counter() - t0
+        if dt < per and m < n_edges:
+            m = min(n_edges, max(m + 1, int(m * per / max(dt, 1e-4))))
+            t0 = time.perf_counter()
+            k, _ = O.level_sample(D, w.alpha, lvl, adjs[lvl], m, threads)
+            dt = time.perf_counter() - t0
+        rate = k / dt if dt > 0 and k > 0 else float("inf")
+        est += tests_l / rate
+        sampled += k
+        sampled_s += dt
+        log(f"cpu level {lvl}: {k} tests on {m} edges in {dt:.2f} s ({rate:.0f} tests/s)")
+    value = g["num_ci_test"] / est if est > 0 else None
+    return {"value": value, "unit": "CI tests/s", "cores": threads, "kind": "port",
+            "sample": f"oracle CheckEdge on the first edges of each level's frozen adjacency: {sampled} tests "
+                      f"in {sampled_s:.1f} s on {threads} threads, stage time extrapolated per level "
+                      f"(est. {est:.1f} s for {g['num_ci_test']} tests)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=3)
+    ap.add_argument("--plain-sample", type=int, default=2000, help="plain-form tests per level")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the CI stage of one PC-stable run is not sharded across GPUs in "
+              "this build (see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    w = configs.get(args.workload)
+    t0 = time.perf_counter()
+    sample = w.sample()
+    log(f"workload {w.name}: {sample.n_vars} vars x {sample.n_rows} rows generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    ctx = _lib.Context.of(sample)   # uploads the columns: not timed
+
+    for i in range(args.warmup):
+        g = ctx.pc_skeleton(w.alpha, w.depth)
+        log(f"warmup {i}: {g['num_ci_test']} tests in {g['ci_seconds']:.3f} s")
+
+    total_tests, kernel_ms, alg_bytes, launches = 0, 0.0, 0.0, 0
+    ref = None
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        g = ctx.pc_skeleton(w.alpha, w.depth)
+        ks = ctx.kernel_stats()
+        total_tests += g["num_ci_test"]
+        kernel_ms += ks["kernel_ms"]
+        alg_bytes += ks["alg_bytes"]
+        launches += ks["launches"]
+        if ref is None:
+            ref = g
+        elif not (np.array_equal(ref["adj"], g["adj"]) and ref["sepsets"] == g["sepsets"]):
+            raise RuntimeError("skeleton differs between steps")
+    elapsed = time.perf_counter() - t_start
+    value = total_tests / elapsed
+    log(f"timed {args.steps} steps: {elapsed:.3f} s, {value:.0f} tests/s; per level "
+        f"{g['tests_per_level']} executed {g['executed_per_level']}")
+
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    out = {
+        "metric": "CI tests/sec (reference-equivalent, PC-stable CI stage, G2)",
+        "value": value,
+        "unit": "CI tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded random DAG, Dirichlet CPTs, forward-sampled uint8 columns)",
+        "config": {"workload": w.name, "n_vars": w.n_vars, "n_rows": w.n_rows, "alpha": w.alpha,
+                   "depth": w.depth, "dag_edges": w.n_edges, "max_in_degree": w.max_in,
+                   "card_law": list(w.card_law), "dirichlet": w.conc, "data_sha256": sample.sha256()[:16]},
+        "pc_stable_step1_seconds": elapsed / args.steps,
+        "num_ci_test": g["num_ci_test"],
+        "tests_per_level": g["tests_per_level"],
+        "executed_per_level": g["executed_per_level"],
+        "in_band_per_level": g["in_band_per_level"],
+        "edges_after_level": g["edges_after_level"],
+        "max_rel_err_g2_fp32_vs_fp64": g["max_rel_err_g2_fp32"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_count_stat", "kernel_ms_per_step": kernel_ms / args.steps,
+                     "launches_per_step": launches / args.steps,
+                     "alg_bytes_per_step": alg_bytes / args.steps},
+    }
+
+    if not args.no_plain:
+        gp = ctx.pc_skeleton(w.alpha, w.depth, mode=_lib.FPGM_MODE_PLAIN, plain_sample=args.plain_sample)
+        same = np.array_equal(gp["adj"], ref["adj"]) and gp["sepsets"] == ref["sepsets"]
+        est = 0.0
+        for lvl, tests_l in enumerate(gp["tests_per_level"]):
+            pt, ps = gp["plain_tests_per_level"][lvl], gp["plain_seconds_per_level"][lvl]
+            if tests_l and pt:
+                est += tests_l * ps / pt
+        plain_rate = gp["num_ci_test"] / est if est > 0 else None
+        out["plain"] = {"value": plain_rate, "unit": "CI tests/s", "est_stage_seconds": est,
+                        "sampled_tests": sum(gp["plain_tests_per_level"]),
+                        "sample": f"first {args.plain_sample} tests of each level, one launch + host sync each",
+                        "skeleton_identical": bool(same)}
+        out["batched_over_plain"] = value / plain_rate if plain_rate else None
+        log(f"plain form: {plain_rate:.0f} tests/s (sampled), identical skeleton: {same}")
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(sample, w, ref, args.cpu_seconds, threads)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,14 +50,16 @@ class Result(C.Structure):
 class PcOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("n_threads", C.c_int32), ("group_size", C.c_int32),
                 ("verbose", C.c_int32), ("round_tests", C.c_int64), ("max_rounds", C.c_int64),
-                ("flags", C.c_uint32), ("band", C.c_float)]
+                ("plain_sample", C.c_int64), ("flags", C.c_uint32), ("band", C.c_float)]
 
 
 class Stats(C.Structure):
     _fields_ = [("num_ci_test", C.c_int64), ("tests_per_level", C.c_int64 * 64),
                 ("edges_after_level", C.c_int64 * 64), ("executed_per_level", C.c_int64 * 64),
                 ("in_band_per_level", C.c_int64 * 64), ("rounds_per_level", C.c_int64 * 64),
-                ("seconds_per_level", C.c_double * 64), ("n_levels", C.c_int32), ("pad", C.c_int32),
+                ("seconds_per_level", C.c_double * 64), ("plain_tests_per_level", C.c_int64 * 64),
+                ("plain_seconds_per_level", C.c_double * 64), ("kernel_seconds_per_level", C.c_double * 64),
+                ("n_levels", C.c_int32), ("pad", C.c_int32),
                 ("n_sepsets", C.c_int64), ("sep_z_total", C.c_int64), ("ci_seconds", C.c_double),
                 ("max_rel_err_g2_fp32", C.c_double)]
 
@@ -186,11 +188,11 @@ class Context:
         return out[:nc.value].reshape(dimz, int(self.card[x]), int(self.card[y]))
 
     def pc_skeleton(self, alpha, depth=1000, mode=FPGM_MODE_BATCHED, flags=0, round_tests=0, verbose=0,
-                    band=0.0, max_rounds=0):
+                    band=0.0, max_rounds=0, plain_sample=0):
         n = self.n_vars
         adj = np.zeros((n, n), dtype=np.uint8)
         o = PcOpts(mode=mode, n_threads=1, group_size=1, verbose=verbose, round_tests=round_tests,
-                   max_rounds=max_rounds, flags=flags, band=band)
+                   max_rounds=max_rounds, plain_sample=plain_sample, flags=flags, band=band)
         st = Stats()
         self._check(self._L.fpgm_pc_skeleton(self.h, float(alpha), int(depth), C.byref(o), adj.ctypes.data,
                                              C.byref(st)))
@@ -215,6 +217,9 @@ class Context:
             "in_band_per_level": [int(v) for v in st.in_band_per_level[:L]],
             "rounds_per_level": [int(v) for v in st.rounds_per_level[:L]],
             "seconds_per_level": [float(v) for v in st.seconds_per_level[:L]],
+            "plain_tests_per_level": [int(v) for v in st.plain_tests_per_level[:L]],
+            "plain_seconds_per_level": [float(v) for v in st.plain_seconds_per_level[:L]],
+            "kernel_seconds_per_level": [float(v) for v in st.kernel_seconds_per_level[:L]],
             "ci_seconds": float(st.ci_seconds),
             "max_rel_err_g2_fp32": float(st.max_rel_err_g2_fp32),
         }

@@ -10,6 +10,10 @@ constexpr int kWG = 256;                  // threads per workgroup (4 wave64)
 constexpr int kRowsPerThread = 16;        // one 16-byte load per column per step
 constexpr int kRowsPerStep = kWG * kRowsPerThread;  // 4096 rows per workgroup step
 constexpr int kLdsCells = 8192;           // LDS table capacity in int32 cells (32 KiB)
+constexpr int kLaneWords = kLdsCells / (4 * 64);    // 16-bit counter pairs per lane (4 waves)
+constexpr int kLaneCells = 2 * kLaneWords;          // largest table counted per lane (64)
+constexpr int kSmallCells = 512;          // LDS table + marginals of per-lane tests
+constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bit counters never wrap
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
 constexpr int kCritTab = 4096;            // critical values tabulated for df = 1..kCritTab
 
@@ -28,13 +32,14 @@ constexpr int32_t kTFLdsTable = 1;   // table counted in LDS
 constexpr int32_t kTFLdsMarg = 2;    // single chunk and table+marginals fit in LDS
 constexpr int32_t kTFExact = 4;      // FP64 statistic and p-value regardless of the band
 constexpr int32_t kTFKeep = 8;       // leave the counted table in the workspace (parity reads)
+constexpr int32_t kTFLane = 16;      // per-lane 16-bit LDS counters (cells <= kLaneCells)
 
 // Result of one test on the device (32 bytes).
 struct DevResult {
     float g2_32;      // FP32 compensated G^2
     float abs_sum;    // sum of |terms|, feeds the FP32 error bound
     int32_t df;
-    uint8_t indep, in_band, exact, pad;
+    uint8_t indep, in_band, exact, need_p;
     double g2_64;     // FP64 G^2 (exact/in-band tests)
     double p64;       // FP64 p-value (exact/in-band tests), NaN otherwise
 };
@@ -47,6 +52,7 @@ struct LaunchArgs {
     const int32_t *zpool;
     const int32_t *card;
     const int32_t *item_test;
+    int64_t item_begin;      // this launch's items: [item_begin, item_begin + n_items)
     int64_t n_items;
     int64_t rows_per_chunk;
     int32_t *ws;
@@ -58,7 +64,13 @@ struct LaunchArgs {
     DevResult *res;
 };
 
-hipError_t launch_count_stat(const LaunchArgs &a, int grid, hipStream_t s);
+// Counting sinks (one kernel each; a batch launches each sink over its items).
+constexpr int kSinkShared = 0;  // LDS histogram shared by waves (copies per wave when small)
+constexpr int kSinkGlobal = 1;  // device-memory table, device atomics (beyond LDS)
+constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
+
+hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s);
+hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);
 

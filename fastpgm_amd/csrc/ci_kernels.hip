@@ -94,10 +94,92 @@ __device__ __forceinline__ float term32(int32_t n, int32_t ni, int32_t nj, int32
     return 2.0f * (float)n * L;
 }
 
-// Rows [r_begin, r_end) of test T into histogram h (LDS or global).
-template <bool kGlobal>
-__device__ __forceinline__ void count_rows(const LaunchArgs &a, const DevTest &T, const Shared &sh,
-                                           int64_t r_begin, int64_t r_end, int32_t *h)
+// Counting sinks.  kSinkShared: one LDS histogram per wave (or shared by several
+// waves), LDS atomics.  kSinkGlobal: the test's global table, device atomics (tables
+// too large for LDS).  kSinkLane: a private histogram per lane, two 16-bit counters
+// per 32-bit LDS word, word-major with the lane index fastest, so the 64 lanes of a
+// wave always hit 64 distinct words in distinct banks: no conflicts and no atomic
+// collisions however skewed the data (tables of at most kLaneCells cells).
+
+template <int kSink>
+__device__ __forceinline__ void sink(int32_t *h, uint32_t key)
+{
+    if (kSink == kSinkLane) {
+        // h = this lane's column of the wave's [word][lane] block
+        atomicAdd(reinterpret_cast<uint32_t *>(h) + (key >> 1) * 64u, 1u << ((key & 1u) << 4));
+    } else {
+        atomicAdd(&h[key], 1);
+    }
+}
+
+template <int D>
+struct Cols {
+    uint4 v[(D > 0 ? D : 1) + 2];
+};
+
+// Loads the 16-byte groups at row r of Z (compile-time count D), X and Y.
+template <int D>
+__device__ __forceinline__ void load_cols(const LaunchArgs &a, const uint8_t *const *zp, const uint8_t *colx,
+                                          const uint8_t *coly, int64_t r, Cols<D> &c)
+{
+#pragma unroll
+    for (int j = 0; j < D; ++j) c.v[j] = *reinterpret_cast<const uint4 *>(zp[j] + r);
+    c.v[(D > 0 ? D : 1)] = *reinterpret_cast<const uint4 *>(colx + r);
+    c.v[(D > 0 ? D : 1) + 1] = *reinterpret_cast<const uint4 *>(coly + r);
+}
+
+// Rows [r_begin, r_end) of test T into the sink, |Z| = D known at compile time.
+// Loads for the next step are issued before the current step is counted.
+template <int D, int kSink>
+__device__ __forceinline__ void count_rows_d(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+                                             int64_t r_begin, int64_t r_end, int32_t *h)
+{
+    const uint8_t *colx = a.cols + (int64_t)T.x * a.stride;
+    const uint8_t *coly = a.cols + (int64_t)T.y * a.stride;
+    const uint8_t *zp[D > 0 ? D : 1];
+    uint32_t zc[D > 0 ? D : 1];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        zp[j] = a.cols + (int64_t)sh.zcol[j] * a.stride;
+        zc[j] = (uint32_t)sh.zcard[j];
+    }
+    const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
+    constexpr int X = (D > 0 ? D : 1);
+    int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
+    Cols<D> cur, nxt;
+    if (r < r_end) load_cols<D>(a, zp, colx, coly, r, cur);
+    while (r < r_end) {
+        const int64_t rn = r + kRowsPerStep;
+        if (rn < r_end) load_cols<D>(a, zp, colx, coly, rn, nxt);
+        uint32_t key[kRowsPerThread];
+#pragma unroll
+        for (int b = 0; b < kRowsPerThread; ++b) {
+            uint32_t k = 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                k = (kSink == kSinkGlobal) ? k * zc[j] + byte_of(cur.v[j], b) : __umul24(k, zc[j]) + byte_of(cur.v[j], b);
+            if (kSink == kSinkGlobal) k = (k * cx + byte_of(cur.v[X], b)) * cy + byte_of(cur.v[X + 1], b);
+            else k = __umul24(__umul24(k, cx) + byte_of(cur.v[X], b), cy) + byte_of(cur.v[X + 1], b);
+            key[b] = k;
+        }
+        const int64_t nv = r_end - r;
+        if (nv >= kRowsPerThread) {
+#pragma unroll
+            for (int b = 0; b < kRowsPerThread; ++b) sink<kSink>(h, key[b]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < kRowsPerThread; ++b)
+                if (b < nv) sink<kSink>(h, key[b]);
+        }
+        cur = nxt;
+        r = rn;
+    }
+}
+
+// Runtime |Z| (d > 3).
+template <int kSink>
+__device__ __forceinline__ void count_rows_any(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+                                               int64_t r_begin, int64_t r_end, int32_t *h)
 {
     const uint8_t *colx = a.cols + (int64_t)T.x * a.stride;
     const uint8_t *coly = a.cols + (int64_t)T.y * a.stride;
@@ -112,24 +194,37 @@ __device__ __forceinline__ void count_rows(const LaunchArgs &a, const DevTest &T
             const uint32_t c = (uint32_t)sh.zcard[j];
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                key[b] = kGlobal ? key[b] * c + byte_of(v, b) : __umul24(key[b], c) + byte_of(v, b);
+                key[b] = (kSink == kSinkGlobal) ? key[b] * c + byte_of(v, b) : __umul24(key[b], c) + byte_of(v, b);
         }
         const uint4 vx = *reinterpret_cast<const uint4 *>(colx + r);
         const uint4 vy = *reinterpret_cast<const uint4 *>(coly + r);
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
-            if (kGlobal) key[b] = (key[b] * cx + byte_of(vx, b)) * cy + byte_of(vy, b);
+            if (kSink == kSinkGlobal) key[b] = (key[b] * cx + byte_of(vx, b)) * cy + byte_of(vy, b);
             else key[b] = __umul24(__umul24(key[b], cx) + byte_of(vx, b), cy) + byte_of(vy, b);
         }
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int b = 0; b < kRowsPerThread; ++b) atomicAdd(&h[key[b]], 1);
+            for (int b = 0; b < kRowsPerThread; ++b) sink<kSink>(h, key[b]);
         } else {
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                if (b < nv) atomicAdd(&h[key[b]], 1);
+                if (b < nv) sink<kSink>(h, key[b]);
         }
+    }
+}
+
+template <int kSink>
+__device__ __forceinline__ void count_rows(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+                                           int64_t r_begin, int64_t r_end, int32_t *h)
+{
+    switch (T.d) {
+    case 0: count_rows_d<0, kSink>(a, T, sh, r_begin, r_end, h); break;
+    case 1: count_rows_d<1, kSink>(a, T, sh, r_begin, r_end, h); break;
+    case 2: count_rows_d<2, kSink>(a, T, sh, r_begin, r_end, h); break;
+    case 3: count_rows_d<3, kSink>(a, T, sh, r_begin, r_end, h); break;
+    default: count_rows_any<kSink>(a, T, sh, r_begin, r_end, h); break;
     }
 }
 
@@ -239,6 +334,7 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
         R.indep = indep;
         R.in_band = in_band;
         R.exact = 0;
+        R.need_p = 0;
         R.g2_64 = NAN;
         R.p64 = (df == 0) ? 1.0 : NAN;
     }
@@ -263,32 +359,27 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
             DevResult &R = a.res[t];
             R.g2_64 = G;
             R.exact = 1;
-            if (sh.df == 0) {
-                R.p64 = 1.0;
-                R.indep = 1;
-            } else {
-                const double p = 1.0 - pchisq(G, sh.df);  // IndependenceTest.cpp:146
-                R.p64 = p;
-                R.indep = (p > a.alpha) ? 1 : 0;
-            }
+            R.need_p = (sh.df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
         }
     }
     __syncthreads();
 }
 
+template <int kMode>
 __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
 {
     __shared__ int32_t lds[kLdsCells];
+    __shared__ int32_t small[kSmallCells];
     __shared__ Shared sh;
-    const int tid = threadIdx.x, wave = tid >> 6;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so give
     // each XCD a contiguous slab of work items (neighbouring tests share columns).
     const int64_t nx = 8;
     const int64_t xcd = blockIdx.x % nx, local = blockIdx.x / nx, per = gridDim.x / nx;
     const int64_t slab = (a.n_items + nx - 1) / nx;
-    const int64_t lo = xcd * slab;
-    const int64_t hi = (lo + slab < a.n_items) ? lo + slab : a.n_items;
+    const int64_t lo = a.item_begin + xcd * slab;
+    const int64_t hi = (xcd * slab + slab < a.n_items) ? lo + slab : a.item_begin + a.n_items;
 
     for (int64_t item = lo + local; item < hi; item += per) {
         const int t = a.item_test[item];
@@ -304,12 +395,47 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
         }
         __syncthreads();
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
-        if (T.flags & kTFLdsTable) {
+        const int32_t *tab = nullptr;  // the chunk's table in LDS, when there is one
+        if (kMode == kSinkLane) {
+            // per-lane 16-bit counters: [wave][word][lane], words = ceil(cells / 2)
+            const int c32 = (int)cells;
+            const int words = (c32 + 1) >> 1;
+            uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
+            for (int i = tid; i < 4 * kLaneWords * 64; i += kWG) pl[i] = 0;
+            __syncthreads();
+            count_rows<kSinkLane>(a, T, sh, r_begin, r_end,
+                                  reinterpret_cast<int32_t *>(pl + wave * kLaneWords * 64 + lane));
+            __syncthreads();
+            // sum the 256 private counters of every cell: 8 threads per word, each over
+            // 8 lanes of all 4 waves, then a 3-step shuffle reduction
+            for (int idx = tid; idx < words * 8; idx += kWG) {
+                const int w = idx >> 3, part = idx & 7;
+                uint32_t lo16 = 0, hi16 = 0;
+#pragma unroll
+                for (int wv = 0; wv < 4; ++wv)
+#pragma unroll
+                    for (int l = 0; l < 8; ++l) {
+                        const uint32_t v = pl[(wv * kLaneWords + w) * 64 + part * 8 + l];
+                        lo16 += v & 0xffffu;
+                        hi16 += v >> 16;
+                    }
+                for (int o = 1; o < 8; o <<= 1) {
+                    lo16 += __shfl_xor(lo16, o, 64);
+                    hi16 += __shfl_xor(hi16, o, 64);
+                }
+                if (part == 0) {
+                    small[2 * w] = (int32_t)lo16;
+                    if (2 * w + 1 < c32) small[2 * w + 1] = (int32_t)hi16;
+                }
+            }
+            __syncthreads();
+            tab = small;
+        } else if (kMode == kSinkShared) {
             const int c32 = (int)cells;
             const int copies = (c32 <= kLdsCells / 4) ? 4 : (c32 <= kLdsCells / 2) ? 2 : 1;
             for (int i = tid; i < copies * c32; i += kWG) lds[i] = 0;
             __syncthreads();
-            count_rows<false>(a, T, sh, r_begin, r_end, lds + (wave % copies) * c32);
+            count_rows<kSinkShared>(a, T, sh, r_begin, r_end, lds + (wave % copies) * c32);
             __syncthreads();
             if (copies > 1) {
                 for (int i = tid; i < c32; i += kWG) {
@@ -319,16 +445,19 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
                 }
                 __syncthreads();
             }
+            tab = lds;
+        } else {
+            count_rows<kSinkGlobal>(a, T, sh, r_begin, r_end, g);
+        }
+        if (tab) {
             if (T.flags & kTFLdsMarg) {
-                finalize(a, T, t, lds, lds + c32, sh);
+                finalize(a, T, t, tab, const_cast<int32_t *>(tab) + cells, sh);
                 continue;
             }
-            for (int i = tid; i < c32; i += kWG) {
-                const int32_t v = lds[i];
+            for (int i = tid; i < (int)cells; i += kWG) {
+                const int32_t v = tab[i];
                 if (v) atomicAdd(&g[i], v);
             }
-        } else {
-            count_rows<true>(a, T, sh, r_begin, r_end, g);
         }
         // retire the chunk; the workgroup retiring the last chunk owns the statistic
         __threadfence();
@@ -372,11 +501,38 @@ __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64
     if (flag) atomicOr(bad, 1u);
 }
 
+// FP64 p-value of the tests the statistic kernel rechecked (one thread per test):
+// p = 1 - pchisq(G^2, df), independent iff p > alpha (IndependenceTest.cpp:146-154).
+__global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, double alpha)
+{
+    const int64_t t = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (t >= n) return;
+    DevResult &R = res[t];
+    if (!R.need_p) return;
+    const double p = 1.0 - pchisq(R.g2_64, R.df);
+    R.p64 = p;
+    R.indep = (p > alpha) ? 1 : 0;
+    R.need_p = 0;
+}
+
 }  // namespace
 
-hipError_t launch_count_stat(const LaunchArgs &a, int grid, hipStream_t s)
+hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_count_stat, dim3(grid), dim3(kWG), 0, s, a);
+    if (a.n_items <= 0) return hipSuccess;
+    switch (mode) {
+    case kSinkLane: hipLaunchKernelGGL(k_count_stat<kSinkLane>, dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkShared: hipLaunchKernelGGL(k_count_stat<kSinkShared>, dim3(grid), dim3(kWG), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_count_stat<kSinkGlobal>, dim3(grid), dim3(kWG), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + kWG - 1) / kWG;
+    hipLaunchKernelGGL(k_pvalue, dim3((unsigned)blocks), dim3(kWG), 0, s, res, n, alpha);
     return hipGetLastError();
 }
 

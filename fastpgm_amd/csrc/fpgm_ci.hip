@@ -176,6 +176,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         cpt = std::min(cpt, steps);
         rpc = (steps + cpt - 1) / cpt * kRowsPerStep;
     }
+    rpc = std::min<int64_t>(rpc, kLaneMaxRowsPerChunk);
     const int64_t n_chunks = std::max<int64_t>(1, (N + rpc - 1) / rpc);
 
     HIP_TRY(ctx->d_zpool.ensure(std::max<int64_t>(zlen, 1), false, ctx->stream), "alloc z pool");
@@ -184,12 +185,12 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
                 "upload z pool");
 
     std::vector<DevTest> dt;
-    std::vector<int32_t> item_test;
+    std::vector<int32_t> item_test, mode_of;
     int64_t i0 = 0;
     while (i0 < n) {
         dt.clear();
-        item_test.clear();
-        int64_t ws_used = 0;
+        mode_of.clear();
+        int64_t ws_used = 0, n_items_total = 0;
         int64_t i1 = i0;
         for (; i1 < n; ++i1) {
             const HostTest &h = tests[i1];
@@ -212,8 +213,21 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             T.n_chunks = (int32_t)n_chunks;
             const int64_t marg = dimz * (T.cx + T.cy + 1);
             T.flags = tflags;
-            if (cells <= kLdsCells) T.flags |= kTFLdsTable;
-            const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || cells > kLdsCells || cells + marg > kLdsCells;
+            bool fits;
+            int mode;
+            if (cells <= kLaneCells) {
+                mode = kSinkLane;
+                T.flags |= kTFLane;
+                fits = cells + marg <= kSmallCells;
+            } else if (cells <= kLdsCells) {
+                mode = kSinkShared;
+                T.flags |= kTFLdsTable;
+                fits = cells + marg <= kLdsCells;
+            } else {
+                mode = kSinkGlobal;
+                fits = false;
+            }
+            const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fits;
             if (!need_ws) T.flags |= kTFLdsMarg;
             T.ws_off = -1;
             if (need_ws) {
@@ -223,15 +237,29 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
                 T.ws_off = ws_used;
                 ws_used += need;
             }
-            T.item0 = (int32_t)item_test.size();
-            for (int64_t c = 0; c < n_chunks; ++c) item_test.push_back((int32_t)dt.size());
             dt.push_back(T);
-            if ((int64_t)item_test.size() > ((int64_t)1 << 30)) {
+            mode_of.push_back(mode);
+            n_items_total += n_chunks;
+            if (n_items_total > ((int64_t)1 << 30)) {
                 ++i1;
                 break;
             }
         }
         const int64_t nb = (int64_t)dt.size();
+        // work items grouped by sink: [lane | shared | global], one launch per group
+        int64_t grp_begin[3] = {0, 0, 0}, grp_len[3] = {0, 0, 0};
+        const int order[3] = {kSinkLane, kSinkShared, kSinkGlobal};
+        item_test.clear();
+        item_test.reserve((size_t)n_items_total);
+        for (int gi = 0; gi < 3; ++gi) {
+            grp_begin[gi] = (int64_t)item_test.size();
+            for (int64_t k = 0; k < nb; ++k) {
+                if (mode_of[k] != order[gi]) continue;
+                dt[k].item0 = (int32_t)item_test.size();
+                for (int64_t c = 0; c < n_chunks; ++c) item_test.push_back((int32_t)k);
+            }
+            grp_len[gi] = (int64_t)item_test.size() - grp_begin[gi];
+        }
         HIP_TRY(ctx->d_tests.ensure(nb, false, ctx->stream), "alloc tests");
         HIP_TRY(ctx->d_item_test.ensure((int64_t)item_test.size(), false, ctx->stream), "alloc items");
         HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
@@ -250,7 +278,6 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         a.zpool = ctx->d_zpool.p;
         a.card = ctx->d_card;
         a.item_test = ctx->d_item_test.p;
-        a.n_items = (int64_t)item_test.size();
         a.rows_per_chunk = rpc;
         a.ws = ctx->d_ws.p;
         a.done = ctx->d_done.p;
@@ -259,11 +286,19 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         a.band = band > 0.f ? band : 0.02f;
         a.flags = flags;
         a.res = ctx->d_res.p;
-        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * 8);
-        grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-        HIP_TRY(launch_count_stat(a, (int)grid, ctx->stream), "launch count_stat");
+        for (int gi = 0; gi < 3; ++gi) {
+            if (grp_len[gi] == 0) continue;
+            a.item_begin = grp_begin[gi];
+            a.n_items = grp_len[gi];
+            const int64_t waves_per_cu = (order[gi] == kSinkGlobal) ? 8 : 4;
+            int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * waves_per_cu);
+            grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+            HIP_TRY(launch_count_stat(a, order[gi], (int)grid, ctx->stream), "launch count_stat");
+            ctx->k_launches += 1;
+        }
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+        HIP_TRY(launch_pvalue(ctx->d_res.p, nb, alpha, ctx->stream), "launch pvalue");
         HIP_TRY(hipMemcpyAsync(out.data() + i0, ctx->d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost,
                                ctx->stream),
                 "download results");
@@ -271,7 +306,6 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
         ctx->k_ms += ms;
-        ctx->k_launches += 1;
         for (int64_t k = i0; k < i1; ++k) ctx->k_bytes += (double)(tests[k].d + 2) * (double)N;
         if (keep_ws) *keep_ws = dt[0].ws_off;
         i0 = i1;
@@ -353,6 +387,41 @@ bool next_choice(int a, int b, int32_t *c)
         }
     }
     return false;
+}
+
+// Runs one round of tests.  Plain form: one launch per test with a host
+// synchronisation after each (the baseline), for at most the level's remaining
+// plain_sample budget; the rest of the round (and the batched form) goes through
+// batched launches.
+fpgm_status run_tests(fpgm_ctx *ctx, bool plain, const fpgm_pc_opts &o, int level, const std::vector<HostTest> &tv,
+                      const int32_t *zpool, int64_t zlen, double alpha, float band, std::vector<DevResult> &res,
+                      fpgm_stats &S)
+{
+    const int64_t n = (int64_t)tv.size();
+    res.resize((size_t)n);
+    int64_t k = 0;
+    const double ms0 = ctx->k_ms;
+    if (plain) {
+        const int64_t left = o.plain_sample > 0 ? o.plain_sample - S.plain_tests_per_level[level] : n;
+        const int64_t m = std::min<int64_t>(n, std::max<int64_t>(0, left));
+        std::vector<DevResult> one;
+        const double t0 = now_s();
+        for (; k < m; ++k) {
+            fpgm_status st = run_batch(ctx, &tv[k], 1, zpool, zlen, alpha, o.flags | FPGM_F_EXACT, band, kTFExact, one);
+            if (st != FPGM_OK) return st;
+            res[k] = one[0];
+        }
+        S.plain_tests_per_level[level] += m;
+        S.plain_seconds_per_level[level] += now_s() - t0;
+    }
+    if (k < n) {
+        std::vector<DevResult> rest;
+        fpgm_status st = run_batch(ctx, tv.data() + k, n - k, zpool, zlen, alpha, o.flags, band, 0, rest);
+        if (st != FPGM_OK) return st;
+        std::copy(rest.begin(), rest.end(), res.begin() + k);
+    }
+    S.kernel_seconds_per_level[level] += (ctx->k_ms - ms0) * 1e-3;
+    return FPGM_OK;
 }
 
 void log_level(const fpgm_pc_opts &o, int d, int64_t tests, int64_t edges, double t)
@@ -651,19 +720,8 @@ fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const f
         const double tl = now_s();
         std::vector<HostTest> tv((size_t)E0);
         for (int64_t e = 0; e < E0; ++e) tv[e] = HostTest{ex[e], ey[e], 0, 0};
-        const uint32_t fl = o.flags | (plain ? FPGM_F_EXACT : 0u);
-        if (plain) {
-            res.resize((size_t)E0);
-            std::vector<DevResult> one;
-            for (int64_t e = 0; e < E0; ++e) {
-                st = run_batch(ctx, &tv[e], 1, nullptr, 0, alpha, fl, band, kTFExact, one);
-                if (st != FPGM_OK) return st;
-                res[e] = one[0];
-            }
-        } else {
-            st = run_batch(ctx, tv.data(), E0, nullptr, 0, alpha, fl, band, 0, res);
-            if (st != FPGM_OK) return st;
-        }
+        st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
+        if (st != FPGM_OK) return st;
         int64_t w = 0, inb = 0;
         for (int64_t e = 0; e < E0; ++e) {
             inb += res[e].in_band;
@@ -734,8 +792,10 @@ fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const f
             if (o.max_rounds > 0 && rounds >= o.max_rounds)
                 return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
             ++rounds;
-            int64_t W = plain ? 1 : std::max<int64_t>(1, budget / (int64_t)active.size());
-            if (!plain) W = std::max<int64_t>(W, std::min<int64_t>((int64_t)1 << std::min<int64_t>(rounds - 1, 20), 4096));
+            const bool plain_round = plain && (o.plain_sample <= 0 || S.plain_tests_per_level[d] < o.plain_sample);
+            int64_t W = plain_round ? 1 : std::max<int64_t>(1, budget / (int64_t)active.size());
+            if (!plain_round)
+                W = std::max<int64_t>(W, std::min<int64_t>((int64_t)1 << std::min<int64_t>(rounds - 1, 20), 4096));
             tv.clear();
             zpool.clear();
             t_edge.clear();
@@ -749,20 +809,8 @@ fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const f
                     if (!next_choice(w.a, d, c)) start_side(w, F, d, w.side + 1, c);
                 }
             }
-            if (plain) {
-                res.resize(tv.size());
-                std::vector<DevResult> one;
-                for (size_t k = 0; k < tv.size(); ++k) {
-                    st = run_batch(ctx, &tv[k], 1, zpool.data(), (int64_t)zpool.size(), alpha,
-                                   o.flags | FPGM_F_EXACT, band, kTFExact, one);
-                    if (st != FPGM_OK) return st;
-                    res[k] = one[0];
-                }
-            } else {
-                st = run_batch(ctx, tv.data(), (int64_t)tv.size(), zpool.data(), (int64_t)zpool.size(), alpha,
-                               o.flags, band, 0, res);
-                if (st != FPGM_OK) return st;
-            }
+            st = run_tests(ctx, plain_round, o, d, tv, zpool.data(), (int64_t)zpool.size(), alpha, band, res, S);
+            if (st != FPGM_OK) return st;
             executed += (int64_t)tv.size();
             std::vector<int64_t> still;
             size_t k = 0;

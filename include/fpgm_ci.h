@@ -62,6 +62,9 @@ typedef struct {
     int32_t verbose;       /* 0 silent, 1 per-level lines on stderr                       */
     int64_t round_tests;   /* batched: target tests per speculative round (0 = auto)      */
     int64_t max_rounds;    /* batched: per-level round cap, 0 = none (FPGM_ETIMEOUT)      */
+    int64_t plain_sample;  /* plain mode: > 0 runs only the first plain_sample tests of    */
+                           /* each level one launch per test and the rest batched, so the */
+                           /* plain rate is sampled while the results stay exact           */
     uint32_t flags;        /* FPGM_F_* below                                              */
     float band;            /* relative half-width of the FP64 recheck band (0 = 0.02)     */
 } fpgm_pc_opts;
@@ -81,6 +84,9 @@ typedef struct {
     int64_t in_band_per_level[64];  /* tests decided by the FP64 recheck                  */
     int64_t rounds_per_level[64];
     double seconds_per_level[64];
+    int64_t plain_tests_per_level[64];   /* tests run one launch per test (plain mode)     */
+    double plain_seconds_per_level[64];  /* wall time of those single-test launches        */
+    double kernel_seconds_per_level[64]; /* counting-kernel time (HIP events)               */
     int32_t n_levels;
     int32_t pad;
     int64_t n_sepsets;

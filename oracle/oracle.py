@@ -69,6 +69,9 @@ def lib():
                                       C.POINTER(OrcStats)]
         L.orc_level0_sample.restype = C.c_int64
         L.orc_level0_sample.argtypes = [C.POINTER(OrcData), C.c_double, C.c_int64, C.c_int, C.c_void_p]
+        L.orc_level_sample.restype = C.c_int64
+        L.orc_level_sample.argtypes = [C.POINTER(OrcData), C.c_double, C.c_int, C.c_void_p, C.c_int64, C.c_int,
+                                       C.POINTER(C.c_int64)]
         _LIB = L
     return _LIB
 
@@ -189,3 +192,12 @@ def level0_sample(data: Data, alpha: float, n_pairs: int, n_threads: int):
     out = np.zeros(max(1, n_pairs), dtype=np.int32)
     k = lib().orc_level0_sample(data.ref, float(alpha), int(n_pairs), int(n_threads), out.ctypes.data)
     return out[:k]
+
+
+def level_sample(data: Data, alpha: float, d: int, adj_before: np.ndarray, max_edges: int, n_threads: int):
+    """Runs level d's CheckEdge on the first max_edges edges of adj_before; returns (tests, removed)."""
+    a = np.ascontiguousarray(adj_before, dtype=np.uint8)
+    rem = C.c_int64()
+    t = lib().orc_level_sample(data.ref, float(alpha), int(d), a.ctypes.data, int(max_edges), int(n_threads),
+                               C.byref(rem))
+    return int(t), int(rem.value)

@@ -543,3 +543,62 @@ int64_t orc_level0_sample(const orc_data *D, double alpha, int64_t n_pairs, int 
     }
     return n_pairs;
 }
+
+/*
+ * Bounded timing sample of one level: CheckEdge (PCStable.cpp:399-494) for the first
+ * max_edges edges (lexicographic order) of the adjacency adj_before, which is frozen for
+ * the level as at PCStable.cpp:272.  Level 0 tests the pairs themselves.  Returns the
+ * number of CI tests run; *removed receives the number of edges found independent.
+ */
+int64_t orc_level_sample(const orc_data *D, double alpha, int d, const uint8_t *adj_before, int64_t max_edges,
+                         int n_threads, int64_t *removed)
+{
+    const int n = D->n_vars;
+    int64_t ne = 0;
+    int32_t *ex = (int32_t *)malloc(sizeof(int32_t) * (max_edges > 0 ? max_edges : 1));
+    int32_t *ey = (int32_t *)malloc(sizeof(int32_t) * (max_edges > 0 ? max_edges : 1));
+    for (int i = 0; i < n && ne < max_edges; ++i)
+        for (int j = i + 1; j < n && ne < max_edges; ++j)
+            if (adj_before[(int64_t)i * n + j]) {
+                ex[ne] = i;
+                ey[ne] = j;
+                ++ne;
+            }
+    int32_t *nb_off = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+    int32_t *nb = (int32_t *)malloc(sizeof(int32_t) * ((int64_t)n * n + 1));
+    nb_off[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        int32_t k = nb_off[i];
+        for (int j = 0; j < n; ++j)
+            if (adj_before[(int64_t)i * n + j]) nb[k++] = j;
+        nb_off[i + 1] = k;
+    }
+    int64_t tests = 0, rem = 0;
+#pragma omp parallel num_threads(n_threads > 0 ? n_threads : 1) reduction(+ : tests, rem)
+    {
+        orc_buf tb = {NULL, 0};
+        int32_t *sbuf = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+        orc_edge_res r;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t e = 0; e < ne; ++e) {
+            if (d == 0) {
+                int32_t *tbl = orc_buf_get(&tb, (int64_t)D->card[ex[e]] * D->card[ey[e]]);
+                orc_result q = orc_ci_test_buf(D, ex[e], ey[e], NULL, 0, alpha, tbl);
+                tests += 1;
+                rem += q.indep;
+            } else {
+                orc_check_edge(D, alpha, d, ex[e], ey[e], nb_off, nb, &tb, sbuf, &r);
+                tests += r.tests;
+                rem += r.removed;
+            }
+        }
+        free(sbuf);
+        free(tb.p);
+    }
+    free(nb);
+    free(nb_off);
+    free(ey);
+    free(ex);
+    if (removed) *removed = rem;
+    return tests;
+}

@@ -83,6 +83,10 @@ int orc_pc_skeleton(const orc_data *D, double alpha, int depth, int n_threads,
 int64_t orc_level0_sample(const orc_data *D, double alpha, int64_t n_pairs, int n_threads,
                           int32_t *indep_out);
 
+/* Bounded timing sample of level d over the first max_edges edges of adj_before. */
+int64_t orc_level_sample(const orc_data *D, double alpha, int d, const uint8_t *adj_before, int64_t max_edges,
+                         int n_threads, int64_t *removed);
+
 #ifdef __cplusplus
 }
 #endif

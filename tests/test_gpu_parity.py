@@ -161,3 +161,13 @@ def test_errors_are_status_codes():
             ctx.ci_test(0, 0, ())
         with pytest.raises(_lib.FpgmError):
             ctx.pc_skeleton(-0.1)
+
+
+def test_plain_sample_keeps_results():
+    s = configs.get(1).sample()
+    with _ctx(s) as ctx:
+        a = ctx.pc_skeleton(0.05, 1000)
+        b = ctx.pc_skeleton(0.05, 1000, mode=_lib.FPGM_MODE_PLAIN, plain_sample=50)
+    _assert_same(a, b)
+    assert b["plain_tests_per_level"][0] == 50
+    assert all(p <= 50 for p in b["plain_tests_per_level"])
