@@ -83,6 +83,42 @@ def cpu_baseline(sample, w, g, seconds, threads):
                       f"(est. {est:.1f} s for {g['num_ci_test']} tests)"}
 
 
+def hbm_traffic_from_counters(fetch_csv, write_csv, copy_bytes=1 << 30):
+    """HBM bytes moved by the counting kernels, from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE counter_collection.csv files (KiB per dispatch) of a run made with
+    --copy-calibration: each counter is scaled by copy_bytes over what it read for the
+    largest copy dispatch (the calibration copy)."""
+    import csv
+
+    def totals(path):
+        rows = list(csv.DictReader(open(path)))
+        copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+        kern = [float(r["Counter_Value"]) for r in rows if "k_count_stat" in r["Kernel_Name"]]
+        return sum(kern) * 1024.0 * (copy_bytes / (copy * 1024.0)), len(kern), copy_bytes / (copy * 1024.0)
+
+    f, n, ff = totals(fetch_csv)
+    wr, _, wf = totals(write_csv)
+    return {"fetch_bytes": f, "write_bytes": wr, "dispatches": n, "fetch_scale": ff, "write_scale": wf,
+            "bytes_per_launch": (f + wr) / max(n, 1)}
+
+
+def copy_calibration(n_bytes=1 << 30):
+    """One device-to-device copy of n_bytes (the runtime's copy kernel), so that HBM
+    counters read under rocprofv3 can be scaled by what a plain copy of known size reads."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n_bytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(n_bytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n_bytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(n_bytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {n_bytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +130,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--pmc-dir", default="",
+                    help="directory with fetch/ and write/ rocprofv3 counter CSVs of this workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -101,6 +141,8 @@ def main():
               "this build (see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
     w = configs.get(args.workload)
     t0 = time.perf_counter()
     sample = w.sample()
@@ -112,22 +154,26 @@ def main():
         g = ctx.pc_skeleton(w.alpha, w.depth)
         log(f"warmup {i}: {g['num_ci_test']} tests in {g['ci_seconds']:.3f} s")
 
+    # timed region: the CI stage only (fpgm_pc_skeleton), nothing read back but the
+    # adjacency and the stats struct
     total_tests, kernel_ms, alg_bytes, launches = 0, 0.0, 0.0, 0
-    ref = None
+    adjs = [np.zeros((sample.n_vars, sample.n_vars), dtype=np.uint8) for _ in range(args.steps)]
+    n_sep = []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        g = ctx.pc_skeleton(w.alpha, w.depth)
+        _, st = ctx.run_stage(w.alpha, w.depth, adj=adjs[i])
         ks = ctx.kernel_stats()
-        total_tests += g["num_ci_test"]
+        total_tests += st.num_ci_test
         kernel_ms += ks["kernel_ms"]
         alg_bytes += ks["alg_bytes"]
         launches += ks["launches"]
-        if ref is None:
-            ref = g
-        elif not (np.array_equal(ref["adj"], g["adj"]) and ref["sepsets"] == g["sepsets"]):
-            raise RuntimeError("skeleton differs between steps")
+        n_sep.append(st.n_sepsets)
     elapsed = time.perf_counter() - t_start
     value = total_tests / elapsed
+    g = ctx.pc_skeleton(w.alpha, w.depth)   # untimed: full read-back for the checks below
+    ref = g
+    if not all(np.array_equal(a, g["adj"]) for a in adjs) or any(k != len(g["sepsets"]) for k in n_sep):
+        raise RuntimeError("skeleton differs between steps")
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {value:.0f} tests/s; per level "
         f"{g['tests_per_level']} executed {g['executed_per_level']}")
 
@@ -161,6 +207,23 @@ def main():
                      "launches_per_step": launches / args.steps,
                      "alg_bytes_per_step": alg_bytes / args.steps},
     }
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if args.pmc_dir:
+        import glob
+        fc = glob.glob(os.path.join(args.pmc_dir, "fetch", "*counter_collection.csv"))
+        wc = glob.glob(os.path.join(args.pmc_dir, "write", "*counter_collection.csv"))
+        if fc and wc:
+            traffic = hbm_traffic_from_counters(fc[0], wc[0])
+            traffic["alg_bytes"] = alg_bytes / args.steps
+    elif os.path.exists(tfile):
+        rec = json.load(open(tfile)).get(w.name)
+        if rec:
+            traffic = rec
+    if traffic:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        out["roofline"]["traffic_detail"] = traffic
 
     if not args.no_plain:
         gp = ctx.pc_skeleton(w.alpha, w.depth, mode=_lib.FPGM_MODE_PLAIN, plain_sample=args.plain_sample)

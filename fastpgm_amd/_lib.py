@@ -187,15 +187,22 @@ class Context:
         dimz = int(np.prod([self.card[v] for v in z])) if len(z) else 1
         return out[:nc.value].reshape(dimz, int(self.card[x]), int(self.card[y]))
 
-    def pc_skeleton(self, alpha, depth=1000, mode=FPGM_MODE_BATCHED, flags=0, round_tests=0, verbose=0,
-                    band=0.0, max_rounds=0, plain_sample=0):
+    def run_stage(self, alpha, depth=1000, mode=FPGM_MODE_BATCHED, flags=0, round_tests=0, verbose=0,
+                  band=0.0, max_rounds=0, plain_sample=0, adj=None):
+        """One fpgm_pc_skeleton call; returns (adjacency, raw Stats).  Nothing else is read back."""
         n = self.n_vars
-        adj = np.zeros((n, n), dtype=np.uint8)
+        if adj is None:
+            adj = np.zeros((n, n), dtype=np.uint8)
         o = PcOpts(mode=mode, n_threads=1, group_size=1, verbose=verbose, round_tests=round_tests,
                    max_rounds=max_rounds, plain_sample=plain_sample, flags=flags, band=band)
         st = Stats()
         self._check(self._L.fpgm_pc_skeleton(self.h, float(alpha), int(depth), C.byref(o), adj.ctypes.data,
                                              C.byref(st)))
+        return adj, st
+
+    def pc_skeleton(self, alpha, depth=1000, mode=FPGM_MODE_BATCHED, flags=0, round_tests=0, verbose=0,
+                    band=0.0, max_rounds=0, plain_sample=0):
+        adj, st = self.run_stage(alpha, depth, mode, flags, round_tests, verbose, band, max_rounds, plain_sample)
         nr, nz = C.c_int64(), C.c_int64()
         self._check(self._L.fpgm_sepset_count(self.h, C.byref(nr), C.byref(nz)))
         recs = (Sepset * max(1, nr.value))()

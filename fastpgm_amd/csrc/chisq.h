@@ -59,19 +59,25 @@ __host__ inline double dchisq(double x, int df)
     return 0.5 * exp((a - 1.0) * log(0.5 * x) - 0.5 * x - lgamma(a));
 }
 
+// z with P(N(0,1) > z) = alpha, by bisection on erfc (host only)
+__host__ inline double norm_upper_quantile(double alpha)
+{
+    double zlo = -40.0, zhi = 40.0;
+    for (int i = 0; i < 200; ++i) {
+        const double zm = 0.5 * (zlo + zhi);
+        if (zm <= zlo || zm >= zhi) break;
+        if (0.5 * erfc(zm / sqrt(2.0)) > alpha) zlo = zm; else zhi = zm;
+    }
+    return 0.5 * (zlo + zhi);
+}
+
 // critical value c with 1 - pchisq(c, df) = alpha: Wilson-Hilferty start, then
 // Newton steps kept inside a bisection bracket.
 __host__ inline double chisq_crit(int df, double alpha)
 {
     if (df <= 0 || !(alpha > 0.0) || !(alpha < 1.0)) return NAN;
     const double target = 1.0 - alpha;
-    // normal quantile of 1-alpha by bisection on erfc (host only, a few dozen steps)
-    double zlo = -10.0, zhi = 10.0;
-    for (int i = 0; i < 80; ++i) {
-        const double zm = 0.5 * (zlo + zhi);
-        if (0.5 * erfc(-zm / sqrt(2.0)) < target) zlo = zm; else zhi = zm;
-    }
-    const double z = 0.5 * (zlo + zhi);
+    const double z = norm_upper_quantile(alpha);
     const double k = df;
     const double t = 1.0 - 2.0 / (9.0 * k) + z * sqrt(2.0 / (9.0 * k));
     double x = k * t * t * t;

@@ -59,6 +59,7 @@ struct LaunchArgs {
     uint32_t *done;
     const double *crit;      // crit[df-1] for df = 1..kCritTab at this alpha
     double alpha;
+    double z_alpha;          // standard normal quantile of 1 - alpha (critical values beyond the table)
     float band;
     uint32_t flags;          // FPGM_F_* of the call
     DevResult *res;

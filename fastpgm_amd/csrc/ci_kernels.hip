@@ -228,66 +228,129 @@ __device__ __forceinline__ void count_rows(const LaunchArgs &a, const DevTest &T
     }
 }
 
-// Statistic of test t from its full table (LDS or global) with marginal scratch.
+// Critical value of the chi-square test at the call's alpha: tabulated (exact FP64
+// inversion on the host) for df <= kCritTab, Wilson-Hilferty beyond, whose relative
+// error there is below 1e-6 (far inside the recheck band).
+__device__ __forceinline__ double crit_of(const LaunchArgs &a, int df)
+{
+    if (df <= kCritTab) return a.crit[df - 1];
+    const double k = (double)df;
+    const double q = 2.0 / (9.0 * k);
+    const double t = 1.0 - q + a.z_alpha * sqrt(q);
+    return k * t * t * t;
+}
+
+// FP64 term with the reference's expression (IndependenceTest.cpp:134-135).
+__device__ __forceinline__ double term64(int32_t n, int32_t ni, int32_t nj, int32_t nk)
+{
+    const double expected = (double)nj * (double)ni / (double)nk;
+    return 2.0 * (double)n * log((double)n / expected);
+}
+
+// One z slice: marginals (CellTable.cpp:242-250), df contribution
+// (IndependenceTest.cpp:96-110) and compensated FP32 terms, by one thread.
+__device__ __forceinline__ void zslice32(const int32_t *tz, int cx, int cy, int32_t *ni, int32_t *nj, int &df,
+                                         float &s, float &c, float &as)
+{
+    int32_t nk = 0;
+    int alx = 0, aly = 0;
+    for (int i = 0; i < cx; ++i) {
+        int32_t v = 0;
+        for (int j = 0; j < cy; ++j) v += tz[i * cy + j];
+        ni[i] = v;
+        nk += v;
+        alx += (v > 0);
+    }
+    for (int j = 0; j < cy; ++j) {
+        int32_t v = 0;
+        for (int i = 0; i < cx; ++i) v += tz[i * cy + j];
+        nj[j] = v;
+        aly += (v > 0);
+    }
+    alx = alx >= 1 ? alx : 1;
+    aly = aly >= 1 ? aly : 1;
+    df += (alx - 1) * (aly - 1);
+    if (nk == 0) return;
+    for (int i = 0; i < cx; ++i) {
+        const int32_t ri = ni[i];
+        if (ri == 0) continue;
+        for (int j = 0; j < cy; ++j) {
+            const int32_t n = tz[i * cy + j];
+            if (n == 0) continue;
+            const float tm = term32(n, ri, nj[j], nk);
+            nadd(s, c, tm);
+            as += fabsf(tm);
+        }
+    }
+}
+
+// Statistic of test t from its full table (LDS or global) with marginal scratch
+// marg = [ni: dimz*cx | nj: dimz*cy | nk: dimz].  Tables with many z slices are
+// processed one slice per thread in a single pass; small ones cell-parallel.
 __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int32_t *tab,
                          int32_t *marg, Shared &sh)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cx = T.cx, cy = T.cy;
-    const int64_t dimz = T.dimz, cxy = (int64_t)cx * cy, cells = dimz * cxy;
+    const int32_t dimz = T.dimz, cxy = cx * cy, cells = dimz * cxy;
     int32_t *ni = marg;
-    int32_t *nj = marg + dimz * cx;
-    int32_t *nk = nj + dimz * cy;
+    int32_t *nj = marg + (int64_t)dimz * cx;
+    int32_t *nk = nj + (int64_t)dimz * cy;
+    const bool zpar = dimz >= 64;
 
-    // marginals n_{x+z}, n_{+yz} (CellTable.cpp:242-250)
-    for (int64_t idx = tid; idx < dimz * cx; idx += kWG) {
-        const int64_t z = idx / cx, i = idx - z * cx;
-        const int32_t *row = tab + z * cxy + i * cy;
-        int32_t s = 0;
-        for (int j = 0; j < cy; ++j) s += row[j];
-        ni[idx] = s;
-    }
-    for (int64_t idx = tid; idx < dimz * cy; idx += kWG) {
-        const int64_t z = idx / cy, j = idx - z * cy;
-        const int32_t *col = tab + z * cxy + j;
-        int32_t s = 0;
-        for (int i = 0; i < cx; ++i) s += col[(int64_t)i * cy];
-        nj[idx] = s;
-    }
-    __syncthreads();
-    // n_{++z} and the adjusted degrees of freedom (IndependenceTest.cpp:96-110)
     int df_loc = 0;
-    for (int64_t z = tid; z < dimz; z += kWG) {
-        int32_t s = 0;
-        int alx = 0, aly = 0;
-        for (int i = 0; i < cx; ++i) {
-            const int32_t v = ni[z * cx + i];
-            s += v;
-            alx += (v > 0);
+    float s = 0.f, c = 0.f, as = 0.f;
+    if (zpar) {
+        for (int32_t z = tid; z < dimz; z += kWG)
+            zslice32(tab + (int64_t)z * cxy, cx, cy, ni + (int64_t)z * cx, nj + (int64_t)z * cy, df_loc, s, c, as);
+    } else {
+        // marginals n_{x+z}, n_{+yz}
+        for (int32_t idx = tid; idx < dimz * cx; idx += kWG) {
+            const int32_t z = idx / cx, i = idx - z * cx;
+            const int32_t *row = tab + z * cxy + i * cy;
+            int32_t v = 0;
+            for (int j = 0; j < cy; ++j) v += row[j];
+            ni[idx] = v;
         }
-        for (int j = 0; j < cy; ++j) aly += (nj[z * cy + j] > 0);
-        alx = alx >= 1 ? alx : 1;
-        aly = aly >= 1 ? aly : 1;
-        df_loc += (alx - 1) * (aly - 1);
-        nk[z] = s;
+        for (int32_t idx = tid; idx < dimz * cy; idx += kWG) {
+            const int32_t z = idx / cy, j = idx - z * cy;
+            const int32_t *col = tab + z * cxy + j;
+            int32_t v = 0;
+            for (int i = 0; i < cx; ++i) v += col[i * cy];
+            nj[idx] = v;
+        }
+        __syncthreads();
+        // n_{++z} and the adjusted degrees of freedom
+        for (int32_t z = tid; z < dimz; z += kWG) {
+            int32_t v = 0;
+            int alx = 0, aly = 0;
+            for (int i = 0; i < cx; ++i) {
+                const int32_t u = ni[z * cx + i];
+                v += u;
+                alx += (u > 0);
+            }
+            for (int j = 0; j < cy; ++j) aly += (nj[z * cy + j] > 0);
+            alx = alx >= 1 ? alx : 1;
+            aly = aly >= 1 ? aly : 1;
+            df_loc += (alx - 1) * (aly - 1);
+            nk[z] = v;
+        }
+        __syncthreads();
+        for (int32_t cell = tid; cell < cells; cell += kWG) {
+            const int32_t n = tab[cell];
+            if (n == 0) continue;
+            const int32_t z = cell / cxy, rem = cell - z * cxy;
+            const int32_t i = rem / cy, j = rem - i * cy;
+            const float tm = term32(n, ni[z * cx + i], nj[z * cy + j], nk[z]);
+            nadd(s, c, tm);
+            as += fabsf(tm);
+        }
     }
     df_loc = wave_sum_i(df_loc);
-    if (lane == 0) sh.red_i[wave] = df_loc;
-    __syncthreads();
-    // FP32 G^2, compensated
-    float s = 0.f, c = 0.f, as = 0.f;
-    for (int64_t cell = tid; cell < cells; cell += kWG) {
-        const int32_t n = tab[cell];
-        if (n == 0) continue;
-        const int64_t z = cell / cxy, rem = cell - z * cxy;
-        const int64_t i = rem / cy, j = rem - i * cy;
-        const float tm = term32(n, ni[z * cx + i], nj[z * cy + j], nk[z]);
-        nadd(s, c, tm);
-        as += fabsf(tm);
-    }
     wave_nsum(s, c);
     as = wave_sum_f(as);
     if (lane == 0) {
+        sh.red_i[wave] = df_loc;
         sh.red_f[3 * wave] = s;
         sh.red_f[3 * wave + 1] = c;
         sh.red_f[3 * wave + 2] = as;
@@ -304,8 +367,6 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
         }
         const float g2 = S + Cc;
         sh.df = df;
-        sh.g2_32 = g2;
-        sh.abs_sum = AS;
         int need64 = 0;
         uint8_t indep = 0, in_band = 0;
         const bool exact = (T.flags & kTFExact) || (a.flags & FPGM_F_EXACT);
@@ -313,9 +374,12 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
             indep = 1;  // IndependenceTest.cpp:140-142
             need64 = exact;
         } else {
-            const double crit = (df <= kCritTab) ? a.crit[df - 1] : NAN;
-            const double bound = 4e-6 * (double)AS + 1e-30;
+            const double crit = crit_of(a, df);
+            // FP32 error bound: a few ulp of every term, summed, plus the critical-value
+            // approximation beyond the table
+            const double bound = 4e-6 * (double)AS + (df > kCritTab ? 1e-5 * crit : 0.0) + 1e-30;
             const double width = fmax((double)a.band * crit, bound);
+            // alpha outside (0, 1) has no critical value: the FP64 p-value decides
             in_band = (isnan(crit) || fabs((double)g2 - crit) <= width) ? 1 : 0;
             if ((a.flags & FPGM_F_FP32ONLY) && !isnan(crit)) {
                 indep = (double)g2 < crit;
@@ -340,15 +404,31 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
     }
     __syncthreads();
     if (sh.need64) {
-        // FP64 G^2 with the reference's expression (IndependenceTest.cpp:134-135)
+        // FP64 G^2, same term order within each z slice as IndependenceTest.cpp:112-137
         double g = 0.0;
-        for (int64_t cell = tid; cell < cells; cell += kWG) {
-            const int32_t n = tab[cell];
-            if (n == 0) continue;
-            const int64_t z = cell / cxy, rem = cell - z * cxy;
-            const int64_t i = rem / cy, j = rem - i * cy;
-            const double expected = (double)nj[z * cy + j] * (double)ni[z * cx + i] / (double)nk[z];
-            g += 2.0 * (double)n * log((double)n / expected);
+        if (zpar) {
+            for (int32_t z = tid; z < dimz; z += kWG) {
+                const int32_t *tz = tab + (int64_t)z * cxy;
+                const int32_t *zi = ni + (int64_t)z * cx, *zj = nj + (int64_t)z * cy;
+                int32_t nkz = 0;
+                for (int i = 0; i < cx; ++i) nkz += zi[i];
+                if (nkz == 0) continue;
+                for (int i = 0; i < cx; ++i) {
+                    if (zi[i] == 0) continue;
+                    for (int j = 0; j < cy; ++j) {
+                        const int32_t n = tz[i * cy + j];
+                        if (n) g += term64(n, zi[i], zj[j], nkz);
+                    }
+                }
+            }
+        } else {
+            for (int32_t cell = tid; cell < cells; cell += kWG) {
+                const int32_t n = tab[cell];
+                if (n == 0) continue;
+                const int32_t z = cell / cxy, rem = cell - z * cxy;
+                const int32_t i = rem / cy, j = rem - i * cy;
+                g += term64(n, ni[z * cx + i], nj[z * cy + j], nk[z]);
+            }
         }
         g = wave_sum_d(g);
         if (lane == 0) sh.red_d[wave] = g;

@@ -98,6 +98,7 @@ struct fpgm_ctx {
     int n_cu = 256;
 
     double crit_alpha = -1.0;
+    double z_alpha = 0.0;
     std::vector<double> crit_host;
     double *d_crit = nullptr;
 
@@ -148,6 +149,7 @@ fpgm_status ensure_crit(fpgm_ctx *ctx, double alpha)
     HIP_TRY(hipMemcpyAsync(ctx->d_crit, ctx->crit_host.data(), sizeof(double) * kCritTab, hipMemcpyHostToDevice,
                            ctx->stream),
             "upload crit");
+    ctx->z_alpha = (alpha > 0.0 && alpha < 1.0) ? norm_upper_quantile(alpha) : 0.0;
     ctx->crit_alpha = alpha;
     return FPGM_OK;
 }
@@ -283,6 +285,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         a.done = ctx->d_done.p;
         a.crit = ctx->d_crit;
         a.alpha = alpha;
+        a.z_alpha = ctx->z_alpha;
         a.band = band > 0.f ? band : 0.02f;
         a.flags = flags;
         a.res = ctx->d_res.p;

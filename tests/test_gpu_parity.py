@@ -171,3 +171,31 @@ def test_plain_sample_keeps_results():
     _assert_same(a, b)
     assert b["plain_tests_per_level"][0] == 50
     assert all(p <= 50 for p in b["plain_tests_per_level"])
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.0, 0.5])
+def test_extreme_alpha(alpha):
+    s = configs.get(0).sample()
+    D = O.Data.of(s)
+    with _ctx(s) as ctx:
+        _assert_same(ctx.pc_skeleton(alpha, 3), O.pc_skeleton(D, alpha, 3, n_threads=4))
+
+
+def test_large_tables_and_high_df():
+    # card 8 everywhere: d = 3 tables have 32768 cells (global sink), df up to 3136;
+    # d = 4 reaches df > 4096 (critical value beyond the tabulated range)
+    s = datagen.random_sample(9, 9, 120_000, 8, 8)
+    D = O.Data.of(s)
+    rng = np.random.default_rng(3)
+    tests = []
+    for d in (3, 4):
+        for _ in range(6):
+            vs = rng.choice(9, size=d + 2, replace=False)
+            tests.append((int(min(vs[0], vs[1])), int(max(vs[0], vs[1])), tuple(sorted(int(v) for v in vs[2:]))))
+    arr = np.array([[x, y, len(z)] + list(z) + [0] * (4 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 4, 0.05, n_threads=8)
+    with _ctx(s) as ctx:
+        for flags in (0, _lib.FPGM_F_EXACT):
+            r = ctx.ci_batch(tests, alpha=0.05, flags=flags)
+            assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+    assert (o["df"] > 4096).any()
