@@ -29,6 +29,7 @@ EXPORTS = [
     "fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ctx_destroy", "fpgm_last_error", "fpgm_status_str",
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats",
+    "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi",
 ]
 
 
@@ -97,8 +98,11 @@ def load(path: str = LIB_PATH):
     L.fpgm_chisq_crit.argtypes = [i32, f64]
     L.fpgm_chisq_crit.restype = f64
     L.fpgm_last_kernel_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
+    L.fpgm_ctx_create_multi.argtypes = [vp, vp, vp, i32, i32, vp, C.POINTER(vp)]
+    L.fpgm_pc_skeleton_multi.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, vp, vp, C.POINTER(Stats)]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
-                 "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats"):
+                 "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
+                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -126,10 +130,34 @@ class Context:
             raise FpgmError(st, L.fpgm_status_str(st).decode())
         self.h = h
         self.n_rows = int(n_rows)
+        self.n_inst = 1
 
     @classmethod
     def of(cls, sample):
         return cls(sample.cols, sample.n_rows, sample.card)
+
+    @classmethod
+    def of_many(cls, samples):
+        """One context over several samples of the same variables (fpgm_ctx_create_multi)."""
+        self = cls.__new__(cls)
+        L = load()
+        self._L = L
+        self.card = np.ascontiguousarray(samples[0].card, dtype=np.int32)
+        self._keep = [np.ascontiguousarray(s.cols, dtype=np.uint8) for s in samples]
+        ptrs = (C.c_void_p * len(samples))(*[c.ctypes.data for c in self._keep])
+        rows = np.array([s.n_rows for s in samples], dtype=np.int64)
+        strides = np.array([c.shape[1] for c in self._keep], dtype=np.int64)
+        h = C.c_void_p()
+        st = L.fpgm_ctx_create_multi(C.cast(ptrs, C.c_void_p), rows.ctypes.data, strides.ctypes.data, len(samples),
+                                     self._keep[0].shape[0], self.card.ctypes.data, C.byref(h))
+        self._keep = None
+        if st != FPGM_OK:
+            raise FpgmError(st, L.fpgm_status_str(st).decode())
+        self.h = h
+        self.n_vars = int(samples[0].n_vars)
+        self.n_rows = int(rows.max()) if len(rows) else 0
+        self.n_inst = len(samples)
+        return self
 
     def close(self):
         if getattr(self, "h", None):
@@ -230,6 +258,29 @@ class Context:
             "ci_seconds": float(st.ci_seconds),
             "max_rel_err_g2_fp32": float(st.max_rel_err_g2_fp32),
         }
+
+    def pc_skeleton_multi(self, alpha, depth=1000, mode=FPGM_MODE_BATCHED, flags=0, round_tests=0, verbose=0,
+                          band=0.0, plain_sample=0, want_adj=True):
+        """Every instance's CI stage; returns the edge-frequency matrix, per-instance
+        skeletons and test counts, and the per-level sums."""
+        n, k = self.n_vars, self.n_inst
+        freq = np.zeros((n, n), dtype=np.int32)
+        adj = np.zeros((k, n, n), dtype=np.uint8) if want_adj else None
+        tests = np.zeros(k, dtype=np.int64)
+        o = PcOpts(mode=mode, n_threads=1, group_size=1, verbose=verbose, round_tests=round_tests,
+                   max_rounds=0, plain_sample=plain_sample, flags=flags, band=band)
+        st = Stats()
+        self._check(self._L.fpgm_pc_skeleton_multi(self.h, float(alpha), int(depth), C.byref(o), freq.ctypes.data,
+                                                   adj.ctypes.data if want_adj else None, tests.ctypes.data,
+                                                   C.byref(st)))
+        L = st.n_levels
+        return {"edge_freq": freq, "adj": adj, "num_ci_test_per_instance": tests,
+                "num_ci_test": int(st.num_ci_test), "ci_seconds": float(st.ci_seconds),
+                "tests_per_level": [int(v) for v in st.tests_per_level[:L]],
+                "executed_per_level": [int(v) for v in st.executed_per_level[:L]],
+                "plain_tests_per_level": [int(v) for v in st.plain_tests_per_level[:L]],
+                "plain_seconds_per_level": [float(v) for v in st.plain_seconds_per_level[:L]],
+                "kernel_seconds_per_level": [float(v) for v in st.kernel_seconds_per_level[:L]]}
 
     def kernel_stats(self):
         ms, b, n = C.c_double(), C.c_double(), C.c_int64()

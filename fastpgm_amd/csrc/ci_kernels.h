@@ -17,9 +17,17 @@ constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bi
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
 constexpr int kCritTab = 4096;            // critical values tabulated for df = 1..kCritTab
 
-// One test as the device sees it (32 bytes).
+// One dataset instance: uint8 columns, column v at cols + v*stride, n_rows valid bytes.
+struct DevInst {
+    const uint8_t *cols;
+    int64_t stride;
+    int64_t n_rows;
+};
+
+// One test as the device sees it (48 bytes).
 struct DevTest {
     int32_t x, y, d, z_off;  // z_off into the device z pool
+    int32_t inst, pad0;      // dataset instance
     int32_t cx, cy;
     int32_t dimz;            // product of the Z cardinalities
     int32_t n_chunks;        // row chunks of this test (work items)
@@ -45,9 +53,7 @@ struct DevResult {
 };
 
 struct LaunchArgs {
-    const uint8_t *cols;
-    int64_t stride;
-    int64_t n_rows;
+    const DevInst *inst;
     const DevTest *tests;
     const int32_t *zpool;
     const int32_t *card;

@@ -119,7 +119,7 @@ struct Cols {
 
 // Loads the 16-byte groups at row r of Z (compile-time count D), X and Y.
 template <int D>
-__device__ __forceinline__ void load_cols(const LaunchArgs &a, const uint8_t *const *zp, const uint8_t *colx,
+__device__ __forceinline__ void load_cols(const uint8_t *const *zp, const uint8_t *colx,
                                           const uint8_t *coly, int64_t r, Cols<D> &c)
 {
 #pragma unroll
@@ -131,26 +131,26 @@ __device__ __forceinline__ void load_cols(const LaunchArgs &a, const uint8_t *co
 // Rows [r_begin, r_end) of test T into the sink, |Z| = D known at compile time.
 // Loads for the next step are issued before the current step is counted.
 template <int D, int kSink>
-__device__ __forceinline__ void count_rows_d(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+__device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T, const Shared &sh,
                                              int64_t r_begin, int64_t r_end, int32_t *h)
 {
-    const uint8_t *colx = a.cols + (int64_t)T.x * a.stride;
-    const uint8_t *coly = a.cols + (int64_t)T.y * a.stride;
+    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
+    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
     const uint8_t *zp[D > 0 ? D : 1];
     uint32_t zc[D > 0 ? D : 1];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        zp[j] = a.cols + (int64_t)sh.zcol[j] * a.stride;
+        zp[j] = I.cols + (int64_t)sh.zcol[j] * I.stride;
         zc[j] = (uint32_t)sh.zcard[j];
     }
     const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
     constexpr int X = (D > 0 ? D : 1);
     int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
     Cols<D> cur, nxt;
-    if (r < r_end) load_cols<D>(a, zp, colx, coly, r, cur);
+    if (r < r_end) load_cols<D>(zp, colx, coly, r, cur);
     while (r < r_end) {
         const int64_t rn = r + kRowsPerStep;
-        if (rn < r_end) load_cols<D>(a, zp, colx, coly, rn, nxt);
+        if (rn < r_end) load_cols<D>(zp, colx, coly, rn, nxt);
         uint32_t key[kRowsPerThread];
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
@@ -178,11 +178,11 @@ __device__ __forceinline__ void count_rows_d(const LaunchArgs &a, const DevTest 
 
 // Runtime |Z| (d > 3).
 template <int kSink>
-__device__ __forceinline__ void count_rows_any(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+__device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &T, const Shared &sh,
                                                int64_t r_begin, int64_t r_end, int32_t *h)
 {
-    const uint8_t *colx = a.cols + (int64_t)T.x * a.stride;
-    const uint8_t *coly = a.cols + (int64_t)T.y * a.stride;
+    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
+    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
     const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
     const int d = T.d;
     for (int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread; r < r_end; r += kRowsPerStep) {
@@ -190,7 +190,7 @@ __device__ __forceinline__ void count_rows_any(const LaunchArgs &a, const DevTes
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) key[b] = 0;
         for (int j = 0; j < d; ++j) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(a.cols + (int64_t)sh.zcol[j] * a.stride + r);
+            const uint4 v = *reinterpret_cast<const uint4 *>(I.cols + (int64_t)sh.zcol[j] * I.stride + r);
             const uint32_t c = (uint32_t)sh.zcard[j];
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
@@ -216,15 +216,15 @@ __device__ __forceinline__ void count_rows_any(const LaunchArgs &a, const DevTes
 }
 
 template <int kSink>
-__device__ __forceinline__ void count_rows(const LaunchArgs &a, const DevTest &T, const Shared &sh,
+__device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, const Shared &sh,
                                            int64_t r_begin, int64_t r_end, int32_t *h)
 {
     switch (T.d) {
-    case 0: count_rows_d<0, kSink>(a, T, sh, r_begin, r_end, h); break;
-    case 1: count_rows_d<1, kSink>(a, T, sh, r_begin, r_end, h); break;
-    case 2: count_rows_d<2, kSink>(a, T, sh, r_begin, r_end, h); break;
-    case 3: count_rows_d<3, kSink>(a, T, sh, r_begin, r_end, h); break;
-    default: count_rows_any<kSink>(a, T, sh, r_begin, r_end, h); break;
+    case 0: count_rows_d<0, kSink>(I, T, sh, r_begin, r_end, h); break;
+    case 1: count_rows_d<1, kSink>(I, T, sh, r_begin, r_end, h); break;
+    case 2: count_rows_d<2, kSink>(I, T, sh, r_begin, r_end, h); break;
+    case 3: count_rows_d<3, kSink>(I, T, sh, r_begin, r_end, h); break;
+    default: count_rows_any<kSink>(I, T, sh, r_begin, r_end, h); break;
     }
 }
 
@@ -464,9 +464,10 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
     for (int64_t item = lo + local; item < hi; item += per) {
         const int t = a.item_test[item];
         const DevTest T = a.tests[t];
+        const DevInst I = a.inst[T.inst];
         const int64_t chunk = item - T.item0;
         const int64_t r_begin = chunk * a.rows_per_chunk;
-        const int64_t r_end = (r_begin + a.rows_per_chunk < a.n_rows) ? r_begin + a.rows_per_chunk : a.n_rows;
+        const int64_t r_end = (r_begin + a.rows_per_chunk < I.n_rows) ? r_begin + a.rows_per_chunk : I.n_rows;
         const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
         for (int j = tid; j < T.d; j += kWG) {
             const int32_t zc = a.zpool[T.z_off + j];
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
             uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
             for (int i = tid; i < 4 * kLaneWords * 64; i += kWG) pl[i] = 0;
             __syncthreads();
-            count_rows<kSinkLane>(a, T, sh, r_begin, r_end,
+            count_rows<kSinkLane>(I, T, sh, r_begin, r_end,
                                   reinterpret_cast<int32_t *>(pl + wave * kLaneWords * 64 + lane));
             __syncthreads();
             // sum the 256 private counters of every cell: 8 threads per word, each over
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
             const int copies = (c32 <= kLdsCells / 4) ? 4 : (c32 <= kLdsCells / 2) ? 2 : 1;
             for (int i = tid; i < copies * c32; i += kWG) lds[i] = 0;
             __syncthreads();
-            count_rows<kSinkShared>(a, T, sh, r_begin, r_end, lds + (wave % copies) * c32);
+            count_rows<kSinkShared>(I, T, sh, r_begin, r_end, lds + (wave % copies) * c32);
             __syncthreads();
             if (copies > 1) {
                 for (int i = tid; i < c32; i += kWG) {
@@ -527,7 +528,7 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
             }
             tab = lds;
         } else {
-            count_rows<kSinkGlobal>(a, T, sh, r_begin, r_end, g);
+            count_rows<kSinkGlobal>(I, T, sh, r_begin, r_end, g);
         }
         if (tab) {
             if (T.flags & kTFLdsMarg) {

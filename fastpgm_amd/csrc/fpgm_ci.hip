@@ -79,7 +79,13 @@ struct DevBuf {
 
 struct HostTest {
     int32_t x, y, d;
+    int32_t inst;   // dataset instance
     int64_t z_off;  // into the host z pool
+};
+
+struct InstHost {
+    const uint8_t *cols;  // device address
+    int64_t stride, n_rows;
 };
 
 }  // namespace
@@ -87,9 +93,10 @@ struct HostTest {
 struct fpgm_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
-    const uint8_t *d_cols = nullptr;
+    const uint8_t *d_cols = nullptr;  // all instances' columns (owned unless borrowed)
     bool owns_cols = false;
-    int64_t stride = 0, n_rows = 0;
+    std::vector<InstHost> inst;       // dataset instances (1, or the bootstrap ensemble)
+    DevInst *d_inst = nullptr;
     int32_t n_vars = 0;
     std::vector<int32_t> card;
     int32_t *d_card = nullptr;
@@ -165,7 +172,8 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
     if (n == 0) return FPGM_OK;
     fpgm_status st = ensure_crit(ctx, alpha);
     if (st != FPGM_OK) return st;
-    const int64_t N = ctx->n_rows;
+    int64_t N = 0;  // longest instance in the batch
+    for (int64_t i = 0; i < n; ++i) N = std::max(N, ctx->inst[tests[i].inst].n_rows);
     const int64_t steps = std::max<int64_t>(1, (N + kRowsPerStep - 1) / kRowsPerStep);
 
     // rows per chunk, common to the launch: enough work items to fill every CU
@@ -179,7 +187,6 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         rpc = (steps + cpt - 1) / cpt * kRowsPerStep;
     }
     rpc = std::min<int64_t>(rpc, kLaneMaxRowsPerChunk);
-    const int64_t n_chunks = std::max<int64_t>(1, (N + rpc - 1) / rpc);
 
     HIP_TRY(ctx->d_zpool.ensure(std::max<int64_t>(zlen, 1), false, ctx->stream), "alloc z pool");
     if (zlen > 0)
@@ -200,6 +207,8 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             T.x = h.x;
             T.y = h.y;
             T.d = h.d;
+            T.inst = h.inst;
+            const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[h.inst].n_rows + rpc - 1) / rpc);
             T.z_off = (int32_t)h.z_off;
             T.cx = ctx->card[h.x];
             T.cy = ctx->card[h.y];
@@ -248,7 +257,8 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             }
         }
         const int64_t nb = (int64_t)dt.size();
-        // work items grouped by sink: [lane | shared | global], one launch per group
+        // work items grouped by sink: [lane | shared | global], one launch per group;
+        // within a group, tests keep the caller's order (neighbouring tests share columns)
         int64_t grp_begin[3] = {0, 0, 0}, grp_len[3] = {0, 0, 0};
         const int order[3] = {kSinkLane, kSinkShared, kSinkGlobal};
         item_test.clear();
@@ -258,7 +268,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             for (int64_t k = 0; k < nb; ++k) {
                 if (mode_of[k] != order[gi]) continue;
                 dt[k].item0 = (int32_t)item_test.size();
-                for (int64_t c = 0; c < n_chunks; ++c) item_test.push_back((int32_t)k);
+                for (int64_t c = 0; c < dt[k].n_chunks; ++c) item_test.push_back((int32_t)k);
             }
             grp_len[gi] = (int64_t)item_test.size() - grp_begin[gi];
         }
@@ -273,9 +283,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
                                hipMemcpyHostToDevice, ctx->stream),
                 "upload items");
         LaunchArgs a{};
-        a.cols = ctx->d_cols;
-        a.stride = ctx->stride;
-        a.n_rows = N;
+        a.inst = ctx->d_inst;
         a.tests = ctx->d_tests.p;
         a.zpool = ctx->d_zpool.p;
         a.card = ctx->d_card;
@@ -309,7 +317,8 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
         ctx->k_ms += ms;
-        for (int64_t k = i0; k < i1; ++k) ctx->k_bytes += (double)(tests[k].d + 2) * (double)N;
+        for (int64_t k = i0; k < i1; ++k)
+            ctx->k_bytes += (double)(tests[k].d + 2) * (double)ctx->inst[tests[k].inst].n_rows;
         if (keep_ws) *keep_ws = dt[0].ws_off;
         i0 = i1;
     }
@@ -466,7 +475,7 @@ double fpgm_chisq_crit(int32_t df, double alpha)
     return chisq_crit(df, alpha);
 }
 
-static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, int64_t n_rows, int64_t row_stride, const int32_t *card)
+static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
 {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
@@ -480,8 +489,6 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, int64_t n_rows, int64
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ctx->ev1), "hipEventCreate");
     ctx->n_vars = n_vars;
-    ctx->n_rows = n_rows;
-    ctx->stride = row_stride;
     ctx->card.assign(card, card + n_vars);
     HIP_TRY(hipMalloc((void **)&ctx->d_card, sizeof(int32_t) * std::max(1, n_vars)), "hipMalloc card");
     HIP_TRY(hipMemcpyAsync(ctx->d_card, card, sizeof(int32_t) * n_vars, hipMemcpyHostToDevice, ctx->stream),
@@ -489,32 +496,32 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, int64_t n_rows, int64
     return FPGM_OK;
 }
 
-static fpgm_status check_args(int32_t n_vars, int64_t n_rows, int64_t row_stride, const int32_t *card,
-                              std::string &msg)
+static fpgm_status check_card(int32_t n_vars, const int32_t *card)
 {
-    if (n_vars < 2 || n_rows < 0 || row_stride < n_rows || row_stride % 16 != 0 || !card) {
-        msg = "need n_vars >= 2, 0 <= n_rows <= row_stride, row_stride % 16 == 0, card";
-        return FPGM_EINVAL;
-    }
-    if (n_rows > ((int64_t)1 << 31) - 1) {
-        msg = "n_rows must fit int32 counts";
-        return FPGM_EINVAL;
-    }
+    if (n_vars < 2 || !card) return FPGM_EINVAL;
     for (int v = 0; v < n_vars; ++v)
-        if (card[v] < 1 || card[v] > 255) {
-            msg = "cardinalities must be in 1..255";
-            return FPGM_EINVAL;
-        }
+        if (card[v] < 1 || card[v] > 255) return FPGM_EINVAL;
     return FPGM_OK;
 }
 
-static fpgm_status validate_values(fpgm_ctx *ctx)
+static bool rows_ok(int64_t n_rows, int64_t row_stride)
 {
+    return n_rows >= 0 && row_stride >= n_rows && n_rows <= ((int64_t)1 << 31) - 1;
+}
+
+// Uploads the instance table and scans every column for values >= card.
+static fpgm_status finish_ctx(fpgm_ctx *ctx)
+{
+    std::vector<DevInst> di(ctx->inst.size());
+    for (size_t i = 0; i < di.size(); ++i) di[i] = DevInst{ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows};
+    HIP_TRY(hipMalloc((void **)&ctx->d_inst, sizeof(DevInst) * di.size()), "hipMalloc instances");
+    HIP_TRY(hipMemcpyAsync(ctx->d_inst, di.data(), sizeof(DevInst) * di.size(), hipMemcpyHostToDevice, ctx->stream),
+            "upload instances");
     HIP_TRY(ctx->d_flag.ensure(1, true, ctx->stream), "alloc flag");
     HIP_TRY(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(uint32_t), ctx->stream), "memset flag");
-    HIP_TRY(launch_check_values(ctx->d_cols, ctx->stride, ctx->n_rows, ctx->n_vars, ctx->d_card, ctx->d_flag.p,
-                                ctx->stream),
-            "launch check");
+    for (const InstHost &I : ctx->inst)
+        HIP_TRY(launch_check_values(I.cols, I.stride, I.n_rows, ctx->n_vars, ctx->d_card, ctx->d_flag.p, ctx->stream),
+                "launch check");
     uint32_t bad = 0;
     HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "flag");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -522,34 +529,43 @@ static fpgm_status validate_values(fpgm_ctx *ctx)
     return FPGM_OK;
 }
 
-fpgm_status fpgm_ctx_create(const uint8_t *cols, int32_t n_vars, int64_t n_rows, int64_t row_stride,
-                            const int32_t *card, fpgm_ctx **out)
+fpgm_status fpgm_ctx_create_multi(const uint8_t *const *cols, const int64_t *n_rows, const int64_t *row_stride,
+                                  int32_t n_inst, int32_t n_vars, const int32_t *card, fpgm_ctx **out)
 {
     if (!out) return FPGM_EINVAL;
     *out = nullptr;
-    std::string msg;
-    if (!cols || row_stride < n_rows || n_vars < 2) return FPGM_EINVAL;
-    if (check_args(n_vars, n_rows, (row_stride + 255) / 256 * 256, card, msg) != FPGM_OK) return FPGM_EINVAL;
+    if (!cols || !n_rows || !row_stride || n_inst < 1 || check_card(n_vars, card) != FPGM_OK) return FPGM_EINVAL;
+    for (int32_t i = 0; i < n_inst; ++i)
+        if (!cols[i] || !rows_ok(n_rows[i], row_stride[i])) return FPGM_EINVAL;
     fpgm_ctx *ctx = new (std::nothrow) fpgm_ctx();
     if (!ctx) return FPGM_ENOMEM;
-    fpgm_status st = ctx_init(ctx, n_vars, n_rows, std::max<int64_t>(256, (n_rows + 255) / 256 * 256), card);
+    fpgm_status st = ctx_init(ctx, n_vars, card);
     if (st == FPGM_OK) {
+        // one allocation holding every instance, each with a 256-byte-aligned stride
+        std::vector<int64_t> off(n_inst + 1, 0);
+        for (int32_t i = 0; i < n_inst; ++i) {
+            const int64_t stride = std::max<int64_t>(256, (n_rows[i] + 255) / 256 * 256);
+            off[i + 1] = off[i] + stride * n_vars;
+        }
         uint8_t *d = nullptr;
-        const size_t bytes = (size_t)ctx->stride * n_vars;
-        hipError_t e = hipMalloc((void **)&d, bytes);
+        hipError_t e = hipMalloc((void **)&d, (size_t)off[n_inst]);
         if (e != hipSuccess) {
             st = hip_fail(ctx, e, "hipMalloc columns");
         } else {
             ctx->d_cols = d;
             ctx->owns_cols = true;
-            e = hipMemsetAsync(d, 0, bytes, ctx->stream);
-            if (e == hipSuccess && n_rows > 0)
-                e = hipMemcpy2DAsync(d, (size_t)ctx->stride, cols, (size_t)row_stride, (size_t)n_rows, (size_t)n_vars,
-                                     hipMemcpyHostToDevice, ctx->stream);
+            e = hipMemsetAsync(d, 0, (size_t)off[n_inst], ctx->stream);
+            for (int32_t i = 0; i < n_inst && e == hipSuccess; ++i) {
+                const int64_t stride = (off[i + 1] - off[i]) / n_vars;
+                ctx->inst.push_back(InstHost{d + off[i], stride, n_rows[i]});
+                if (n_rows[i] > 0)
+                    e = hipMemcpy2DAsync(d + off[i], (size_t)stride, cols[i], (size_t)row_stride[i], (size_t)n_rows[i],
+                                         (size_t)n_vars, hipMemcpyHostToDevice, ctx->stream);
+            }
             if (e != hipSuccess) st = hip_fail(ctx, e, "upload columns");
         }
     }
-    if (st == FPGM_OK) st = validate_values(ctx);
+    if (st == FPGM_OK) st = finish_ctx(ctx);
     if (st != FPGM_OK) {
         fpgm_ctx_destroy(ctx);
         return st;
@@ -558,21 +574,28 @@ fpgm_status fpgm_ctx_create(const uint8_t *cols, int32_t n_vars, int64_t n_rows,
     return FPGM_OK;
 }
 
+fpgm_status fpgm_ctx_create(const uint8_t *cols, int32_t n_vars, int64_t n_rows, int64_t row_stride,
+                            const int32_t *card, fpgm_ctx **out)
+{
+    return fpgm_ctx_create_multi(&cols, &n_rows, &row_stride, 1, n_vars, card, out);
+}
+
 fpgm_status fpgm_ctx_create_device(const void *d_cols, int32_t n_vars, int64_t n_rows, int64_t row_stride,
                                    const int32_t *card, fpgm_ctx **out)
 {
     if (!out) return FPGM_EINVAL;
     *out = nullptr;
-    std::string msg;
-    if (!d_cols || ((uintptr_t)d_cols % 16) != 0) return FPGM_EINVAL;
-    if (check_args(n_vars, n_rows, row_stride, card, msg) != FPGM_OK) return FPGM_EINVAL;
+    if (!d_cols || ((uintptr_t)d_cols % 16) != 0 || row_stride % 16 != 0 || !rows_ok(n_rows, row_stride) ||
+        check_card(n_vars, card) != FPGM_OK)
+        return FPGM_EINVAL;
     fpgm_ctx *ctx = new (std::nothrow) fpgm_ctx();
     if (!ctx) return FPGM_ENOMEM;
-    fpgm_status st = ctx_init(ctx, n_vars, n_rows, row_stride, card);
+    fpgm_status st = ctx_init(ctx, n_vars, card);
     if (st == FPGM_OK) {
         ctx->d_cols = (const uint8_t *)d_cols;
         ctx->owns_cols = false;
-        st = validate_values(ctx);
+        ctx->inst.push_back(InstHost{(const uint8_t *)d_cols, row_stride, n_rows});
+        st = finish_ctx(ctx);
     }
     if (st != FPGM_OK) {
         fpgm_ctx_destroy(ctx);
@@ -587,6 +610,7 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->owns_cols && ctx->d_cols) (void)hipFree((void *)ctx->d_cols);
+    if (ctx->d_inst) (void)hipFree(ctx->d_inst);
     if (ctx->d_card) (void)hipFree(ctx->d_card);
     if (ctx->d_crit) (void)hipFree(ctx->d_crit);
     ctx->d_tests.release();
@@ -604,6 +628,7 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
 
 static fpgm_status check_test(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, int32_t d, double alpha)
 {
+    if (ctx->inst.size() != 1) return fail(ctx, FPGM_EINVAL, "single-test calls need a one-instance context");
     if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(ctx, FPGM_EINVAL, "alpha must lie in [0, 1]");
     if (x < 0 || x >= ctx->n_vars || y < 0 || y >= ctx->n_vars || x == y)
         return fail(ctx, FPGM_EINVAL, "x, y must be distinct node indices");
@@ -620,7 +645,7 @@ fpgm_status fpgm_ci_test(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, 
     std::lock_guard<std::mutex> lk(ctx->mu);
     fpgm_status st = check_test(ctx, x, y, z, d, alpha);
     if (st != FPGM_OK) return st;
-    HostTest h{x, y, d, 0};
+    HostTest h{x, y, d, 0, 0};
     std::vector<DevResult> r;
     st = run_batch(ctx, &h, 1, z, d, alpha, FPGM_F_EXACT, 0.02f, kTFExact, r);
     if (st != FPGM_OK) return st;
@@ -640,7 +665,7 @@ fpgm_status fpgm_ci_batch(fpgm_ctx *ctx, const fpgm_test *tests, int64_t n, cons
             return fail(ctx, FPGM_EINVAL, "z_off/d outside the z pool");
         fpgm_status st = check_test(ctx, t.x, t.y, t.d > 0 ? z_pool + t.z_off : nullptr, t.d, alpha);
         if (st != FPGM_OK) return st;
-        h[i] = HostTest{t.x, t.y, t.d, t.z_off};
+        h[i] = HostTest{t.x, t.y, t.d, 0, t.z_off};
     }
     reset_kernel_stats(ctx);
     std::vector<DevResult> r;
@@ -661,7 +686,7 @@ fpgm_status fpgm_ci_counts(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z
     for (int j = 0; j < d; ++j) nc *= ctx->card[z[j]];
     *cells = nc;
     if (!table || cap < nc) return fail(ctx, FPGM_EINVAL, "table capacity too small");
-    HostTest h{x, y, d, 0};
+    HostTest h{x, y, d, 0, 0};
     std::vector<DevResult> r;
     int64_t off = -1;
     st = run_batch(ctx, &h, 1, z, d, 0.05, 0, 0.02f, kTFKeep, r, 0, &off);
@@ -676,121 +701,143 @@ fpgm_status fpgm_ci_counts(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z
     return FPGM_OK;
 }
 
-fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
-                             uint8_t *adj_out, fpgm_stats *stats)
-{
-    if (!ctx || !adj_out) return FPGM_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(ctx, FPGM_EINVAL, "alpha must lie in [0, 1]");
-    if (depth < 0) return fail(ctx, FPGM_EINVAL, "depth must be >= 0");
-    fpgm_pc_opts o{};
-    if (opts) o = *opts;
-    const float band = o.band > 0.f ? o.band : 0.02f;
-    const bool plain = (o.mode == FPGM_MODE_PLAIN);
-    const int n = ctx->n_vars;
-    const int64_t N = ctx->n_rows;
-    fpgm_stats S{};
-    reset_kernel_stats(ctx);
-    fpgm_status st = ensure_crit(ctx, alpha);
-    if (st != FPGM_OK) return st;
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
+namespace {
 
-    std::vector<uint8_t> adj((size_t)n * n, 0);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) adj[(size_t)i * n + j] = (i != j);
-    ctx->seps.clear();
-    ctx->sep_z.clear();
+// PC-stable state of one dataset instance across levels.
+struct PcInst {
+    std::vector<uint8_t> adj;       // n*n
+    std::vector<int32_t> ex, ey;    // surviving edges, lexicographic
     struct Sep {
         int32_t x, y, level, d;
         int64_t zoff;
     };
     std::vector<Sep> seps;
     std::vector<int32_t> sepz;
-
-    const double t_start = now_s();
-    // ---- level 0: every pair of the complete graph (PCStable.cpp:133-195)
-    std::vector<int32_t> ex, ey;
-    ex.reserve((size_t)n * (n - 1) / 2);
-    ey.reserve((size_t)n * (n - 1) / 2);
-    for (int i = 0; i < n; ++i)
-        for (int j = i + 1; j < n; ++j) {
-            ex.push_back(i);
-            ey.push_back(j);
-        }
-    const int64_t E0 = (int64_t)ex.size();
-    std::vector<DevResult> res;
-    {
-        const double tl = now_s();
-        std::vector<HostTest> tv((size_t)E0);
-        for (int64_t e = 0; e < E0; ++e) tv[e] = HostTest{ex[e], ey[e], 0, 0};
-        st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
-        if (st != FPGM_OK) return st;
-        int64_t w = 0, inb = 0;
-        for (int64_t e = 0; e < E0; ++e) {
-            inb += res[e].in_band;
-            if (res[e].exact && res[e].df > 0 && res[e].g2_64 > 0.0) {
-                const double rel = std::fabs((double)res[e].g2_32 - res[e].g2_64) / res[e].g2_64;
-                S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, rel);
-            }
-            if (res[e].indep) {
-                seps.push_back(Sep{ex[e], ey[e], 0, 0, (int64_t)sepz.size()});
-                adj[(size_t)ex[e] * n + ey[e]] = 0;
-                adj[(size_t)ey[e] * n + ex[e]] = 0;
-            } else {
-                ex[w] = ex[e];
-                ey[w] = ey[e];
-                ++w;
-            }
-        }
-        ex.resize(w);
-        ey.resize(w);
-        S.tests_per_level[0] = E0;
-        S.executed_per_level[0] = E0;
-        S.in_band_per_level[0] = inb;
-        S.rounds_per_level[0] = 1;
-        S.edges_after_level[0] = w;
-        S.seconds_per_level[0] = now_s() - tl;
-        S.n_levels = 1;
-        log_level(o, 0, E0, w, now_s() - t_start);
-    }
-
-    // ---- levels d >= 1 (PCStable.cpp:209-231, SearchAtDepth :265-388)
+    int64_t num_ci_test = 0;
+    bool cont = true;               // continues to the next level
     Frozen F;
     std::vector<EdgeWalk> walks;
     std::vector<int32_t> choice, zlevel;
+};
+
+// The CI stage of every instance, level-synchronously: level d's tests of all
+// instances that reached level d share the same rounds and launches.
+fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts &o, std::vector<PcInst> &P,
+                   fpgm_stats &S)
+{
+    const float band = o.band > 0.f ? o.band : 0.02f;
+    const bool plain = (o.mode == FPGM_MODE_PLAIN);
+    const int n = ctx->n_vars;
+    const int32_t n_inst = (int32_t)P.size();
+    fpgm_status st = ensure_crit(ctx, alpha);
+    if (st != FPGM_OK) return st;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
+    double avg_rows = 0.0;
+    for (const InstHost &I : ctx->inst) avg_rows += (double)I.n_rows / (double)ctx->inst.size();
+
+    const double t_start = now_s();
+    for (PcInst &I : P) {
+        I.adj.assign((size_t)n * n, 0);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) I.adj[(size_t)i * n + j] = (i != j);
+        I.ex.clear();
+        I.ey.clear();
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                I.ex.push_back(i);
+                I.ey.push_back(j);
+            }
+        I.seps.clear();
+        I.sepz.clear();
+        I.num_ci_test = 0;
+        I.cont = true;
+    }
+    std::vector<DevResult> res;
     std::vector<HostTest> tv;
     std::vector<int32_t> zpool;
-    std::vector<int64_t> t_edge;  // edge of each test in the round
+    std::vector<int64_t> t_edge;  // (instance, edge) key of each test in the round
+
+    // ---- level 0: every pair of the complete graph (PCStable.cpp:133-195)
+    {
+        const double tl = now_s();
+        tv.clear();
+        for (int32_t i = 0; i < n_inst; ++i)
+            for (size_t e = 0; e < P[i].ex.size(); ++e) tv.push_back(HostTest{P[i].ex[e], P[i].ey[e], 0, i, 0});
+        st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
+        if (st != FPGM_OK) return st;
+        int64_t k = 0, inb = 0, left = 0;
+        for (int32_t i = 0; i < n_inst; ++i) {
+            PcInst &I = P[i];
+            int64_t w = 0;
+            for (size_t e = 0; e < I.ex.size(); ++e, ++k) {
+                inb += res[k].in_band;
+                if (res[k].exact && res[k].df > 0 && res[k].g2_64 > 0.0)
+                    S.max_rel_err_g2_fp32 =
+                        std::max(S.max_rel_err_g2_fp32, std::fabs((double)res[k].g2_32 - res[k].g2_64) / res[k].g2_64);
+                if (res[k].indep) {
+                    I.seps.push_back(PcInst::Sep{I.ex[e], I.ey[e], 0, 0, (int64_t)I.sepz.size()});
+                    I.adj[(size_t)I.ex[e] * n + I.ey[e]] = 0;
+                    I.adj[(size_t)I.ey[e] * n + I.ex[e]] = 0;
+                } else {
+                    I.ex[w] = I.ex[e];
+                    I.ey[w] = I.ey[e];
+                    ++w;
+                }
+            }
+            I.num_ci_test += (int64_t)I.ex.size();
+            I.ex.resize(w);
+            I.ey.resize(w);
+            left += w;
+        }
+        S.tests_per_level[0] = (int64_t)tv.size();
+        S.executed_per_level[0] = (int64_t)tv.size();
+        S.in_band_per_level[0] = inb;
+        S.rounds_per_level[0] = 1;
+        S.edges_after_level[0] = left;
+        S.seconds_per_level[0] = now_s() - tl;
+        S.n_levels = 1;
+        log_level(o, 0, (int64_t)tv.size(), left, now_s() - t_start);
+    }
+
+    // ---- levels d >= 1 (PCStable.cpp:209-231, SearchAtDepth :265-388)
     for (int d = 1; d < depth && d < 64; ++d) {
+        bool any = false;
+        for (const PcInst &I : P) any = any || I.cont;
+        if (!any) break;
         if (d > kMaxD) return fail(ctx, FPGM_ETOOBIG, "conditioning depth exceeds the device limit");
         const double tl = now_s();
-        F.off.assign(n + 1, 0);
-        F.nb.clear();
-        for (int i = 0; i < n; ++i) {
-            for (int j = 0; j < n; ++j)
-                if (adj[(size_t)i * n + j]) F.nb.push_back(j);
-            F.off[i + 1] = (int32_t)F.nb.size();
-        }
-        const int64_t E = (int64_t)ex.size();
-        walks.assign((size_t)E, EdgeWalk{});
-        choice.assign((size_t)E * d, 0);
-        zlevel.clear();
-        std::vector<int64_t> active;
-        for (int64_t e = 0; e < E; ++e) {
-            EdgeWalk &w = walks[e];
-            w.x = ex[e];
-            w.y = ey[e];
-            w.tests = 0;
-            w.removed = 0;
-            w.zsep = -1;
-            if (start_side(w, F, d, 0, &choice[(size_t)e * d])) active.push_back(e);
+        std::vector<std::pair<int32_t, int64_t>> active;  // (instance, edge)
+        for (int32_t i = 0; i < n_inst; ++i) {
+            PcInst &I = P[i];
+            if (!I.cont) continue;
+            // frozen copy of the adjacency (PCStable.cpp:272)
+            I.F.off.assign(n + 1, 0);
+            I.F.nb.clear();
+            for (int a = 0; a < n; ++a) {
+                for (int b = 0; b < n; ++b)
+                    if (I.adj[(size_t)a * n + b]) I.F.nb.push_back(b);
+                I.F.off[a + 1] = (int32_t)I.F.nb.size();
+            }
+            const int64_t E = (int64_t)I.ex.size();
+            I.walks.assign((size_t)E, EdgeWalk{});
+            I.choice.assign((size_t)E * d, 0);
+            I.zlevel.clear();
+            for (int64_t e = 0; e < E; ++e) {
+                EdgeWalk &w = I.walks[e];
+                w.x = I.ex[e];
+                w.y = I.ey[e];
+                w.tests = 0;
+                w.removed = 0;
+                w.zsep = -1;
+                if (start_side(w, I.F, d, 0, &I.choice[(size_t)e * d])) active.emplace_back(i, e);
+            }
         }
         int64_t executed = 0, inb = 0, rounds = 0;
-        const int64_t budget = o.round_tests > 0
-                                   ? o.round_tests
-                                   : std::min<int64_t>((int64_t)1 << 20,
-                                                       std::max<int64_t>(8192, ((int64_t)1 << 31) /
-                                                                                   std::max<int64_t>(1, (d + 2) * N)));
+        const int64_t budget =
+            o.round_tests > 0
+                ? o.round_tests
+                : std::min<int64_t>((int64_t)1 << 20,
+                                    std::max<int64_t>(8192, (int64_t)(2147483648.0 / std::max(1.0, (d + 2) * avg_rows))));
         while (!active.empty()) {
             if (o.max_rounds > 0 && rounds >= o.max_rounds)
                 return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
@@ -802,94 +849,167 @@ fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const f
             tv.clear();
             zpool.clear();
             t_edge.clear();
-            for (int64_t e : active) {
-                EdgeWalk &w = walks[e];
-                int32_t *c = &choice[(size_t)e * d];
+            for (size_t ai = 0; ai < active.size(); ++ai) {
+                PcInst &I = P[active[ai].first];
+                const int64_t e = active[ai].second;
+                EdgeWalk &w = I.walks[e];
+                int32_t *c = &I.choice[(size_t)e * d];
                 for (int64_t k = 0; k < W && w.side < 2; ++k) {
-                    tv.push_back(HostTest{w.x, w.y, d, (int64_t)zpool.size()});
-                    for (int i = 0; i < d; ++i) zpool.push_back(F.at(w.u, w.skip, c[i]));
-                    t_edge.push_back(e);
-                    if (!next_choice(w.a, d, c)) start_side(w, F, d, w.side + 1, c);
+                    tv.push_back(HostTest{w.x, w.y, d, active[ai].first, (int64_t)zpool.size()});
+                    for (int i = 0; i < d; ++i) zpool.push_back(I.F.at(w.u, w.skip, c[i]));
+                    t_edge.push_back((int64_t)ai);
+                    if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
                 }
             }
             st = run_tests(ctx, plain_round, o, d, tv, zpool.data(), (int64_t)zpool.size(), alpha, band, res, S);
             if (st != FPGM_OK) return st;
             executed += (int64_t)tv.size();
-            std::vector<int64_t> still;
+            std::vector<std::pair<int32_t, int64_t>> still;
             size_t k = 0;
-            for (int64_t e : active) {
-                EdgeWalk &w = walks[e];
+            for (size_t ai = 0; ai < active.size(); ++ai) {
+                PcInst &I = P[active[ai].first];
+                EdgeWalk &w = I.walks[active[ai].second];
                 bool done = false;
-                for (; k < tv.size() && t_edge[k] == e; ++k) {
+                for (; k < tv.size() && t_edge[k] == (int64_t)ai; ++k) {
                     inb += res[k].in_band;
-                    if (res[k].exact && res[k].df > 0 && res[k].g2_64 > 0.0) {
-                        const double rel = std::fabs((double)res[k].g2_32 - res[k].g2_64) / res[k].g2_64;
-                        S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, rel);
-                    }
+                    if (res[k].exact && res[k].df > 0 && res[k].g2_64 > 0.0)
+                        S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32,
+                                                         std::fabs((double)res[k].g2_32 - res[k].g2_64) / res[k].g2_64);
                     if (done) continue;
                     w.tests += 1;
                     if (res[k].indep) {
                         done = true;
                         w.removed = 1;
-                        w.zsep = (int64_t)zlevel.size();
-                        for (int i = 0; i < d; ++i) zlevel.push_back(zpool[tv[k].z_off + i]);
+                        w.zsep = (int64_t)I.zlevel.size();
+                        for (int i = 0; i < d; ++i) I.zlevel.push_back(zpool[tv[k].z_off + i]);
                     }
                 }
-                if (!done && w.side < 2) still.push_back(e);
+                if (!done && w.side < 2) still.push_back(active[ai]);
             }
             active.swap(still);
         }
-        int64_t tests = 0, wkeep = 0;
-        for (int64_t e = 0; e < E; ++e) {
-            const EdgeWalk &w = walks[e];
-            tests += w.tests;
-            if (w.removed) {
-                seps.push_back(Sep{w.x, w.y, d, d, (int64_t)sepz.size()});
-                for (int i = 0; i < d; ++i) sepz.push_back(zlevel[w.zsep + i]);
-                adj[(size_t)w.x * n + w.y] = 0;
-                adj[(size_t)w.y * n + w.x] = 0;
-            } else {
-                ex[wkeep] = w.x;
-                ey[wkeep] = w.y;
-                ++wkeep;
+        int64_t tests = 0, left = 0;
+        for (PcInst &I : P) {
+            if (!I.cont) {
+                left += (int64_t)I.ex.size();
+                continue;
             }
+            int64_t wkeep = 0, lt = 0;
+            for (size_t e = 0; e < I.walks.size(); ++e) {
+                const EdgeWalk &w = I.walks[e];
+                lt += w.tests;
+                if (w.removed) {
+                    I.seps.push_back(PcInst::Sep{w.x, w.y, d, d, (int64_t)I.sepz.size()});
+                    for (int i = 0; i < d; ++i) I.sepz.push_back(I.zlevel[w.zsep + i]);
+                    I.adj[(size_t)w.x * n + w.y] = 0;
+                    I.adj[(size_t)w.y * n + w.x] = 0;
+                } else {
+                    I.ex[wkeep] = w.x;
+                    I.ey[wkeep] = w.y;
+                    ++wkeep;
+                }
+            }
+            I.ex.resize(wkeep);
+            I.ey.resize(wkeep);
+            I.num_ci_test += lt;
+            tests += lt;
+            left += wkeep;
+            // FreeDegree (PCStable.cpp:620-626): continue while max degree - 1 > d
+            int maxdeg = 0;
+            for (int a = 0; a < n; ++a) {
+                int deg = 0;
+                for (int b = 0; b < n; ++b) deg += I.adj[(size_t)a * n + b];
+                maxdeg = std::max(maxdeg, deg);
+            }
+            I.cont = (maxdeg - 1 > d);
         }
-        ex.resize(wkeep);
-        ey.resize(wkeep);
         S.tests_per_level[d] = tests;
         S.executed_per_level[d] = executed;
         S.in_band_per_level[d] = inb;
         S.rounds_per_level[d] = rounds;
-        S.edges_after_level[d] = wkeep;
+        S.edges_after_level[d] = left;
         S.seconds_per_level[d] = now_s() - tl;
         S.n_levels = d + 1;
-        log_level(o, d, tests, wkeep, now_s() - t_start);
-        int maxdeg = 0;
-        for (int i = 0; i < n; ++i) {
-            int deg = 0;
-            for (int j = 0; j < n; ++j) deg += adj[(size_t)i * n + j];
-            maxdeg = std::max(maxdeg, deg);
-        }
-        if (!(maxdeg - 1 > d)) break;
+        log_level(o, d, tests, left, now_s() - t_start);
     }
     S.ci_seconds = now_s() - t_start;
+    for (int l = 0; l < S.n_levels; ++l) S.num_ci_test += S.tests_per_level[l];
+    return FPGM_OK;
+}
 
-    std::sort(seps.begin(), seps.end(),
-              [](const Sep &a, const Sep &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-    for (const Sep &s : seps) {
+fpgm_status check_stage_args(fpgm_ctx *ctx, double alpha, int32_t depth)
+{
+    if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(ctx, FPGM_EINVAL, "alpha must lie in [0, 1]");
+    if (depth < 0) return fail(ctx, FPGM_EINVAL, "depth must be >= 0");
+    return FPGM_OK;
+}
+
+}  // namespace
+
+fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
+                             uint8_t *adj_out, fpgm_stats *stats)
+{
+    if (!ctx || !adj_out) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->inst.size() != 1) return fail(ctx, FPGM_EINVAL, "fpgm_pc_skeleton needs a one-instance context");
+    fpgm_status st = check_stage_args(ctx, alpha, depth);
+    if (st != FPGM_OK) return st;
+    fpgm_pc_opts o{};
+    if (opts) o = *opts;
+    fpgm_stats S{};
+    reset_kernel_stats(ctx);
+    std::vector<PcInst> P(1);
+    st = run_pc(ctx, alpha, depth, o, P, S);
+    if (st != FPGM_OK) return st;
+    PcInst &I = P[0];
+    std::sort(I.seps.begin(), I.seps.end(),
+              [](const PcInst::Sep &a, const PcInst::Sep &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    ctx->seps.clear();
+    ctx->sep_z.clear();
+    for (const PcInst::Sep &s : I.seps) {
         fpgm_sepset r{};
         r.x = s.x;
         r.y = s.y;
         r.level = s.level;
         r.d = s.d;
         r.z_off = (int64_t)ctx->sep_z.size();
-        for (int i = 0; i < s.d; ++i) ctx->sep_z.push_back(sepz[s.zoff + i]);
+        for (int i = 0; i < s.d; ++i) ctx->sep_z.push_back(I.sepz[s.zoff + i]);
         ctx->seps.push_back(r);
     }
     S.n_sepsets = (int64_t)ctx->seps.size();
     S.sep_z_total = (int64_t)ctx->sep_z.size();
-    for (int l = 0; l < S.n_levels; ++l) S.num_ci_test += S.tests_per_level[l];
-    std::memcpy(adj_out, adj.data(), adj.size());
+    std::memcpy(adj_out, I.adj.data(), I.adj.size());
+    if (stats) *stats = S;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_pc_skeleton_multi(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
+                                   int32_t *edge_freq, uint8_t *adj_out, int64_t *num_ci_test_out,
+                                   fpgm_stats *stats)
+{
+    if (!ctx || !edge_freq) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    fpgm_status st = check_stage_args(ctx, alpha, depth);
+    if (st != FPGM_OK) return st;
+    fpgm_pc_opts o{};
+    if (opts) o = *opts;
+    fpgm_stats S{};
+    reset_kernel_stats(ctx);
+    std::vector<PcInst> P(ctx->inst.size());
+    st = run_pc(ctx, alpha, depth, o, P, S);
+    if (st != FPGM_OK) return st;
+    const size_t nn = (size_t)ctx->n_vars * ctx->n_vars;
+    std::memset(edge_freq, 0, sizeof(int32_t) * nn);
+    int64_t nsep = 0;
+    for (size_t i = 0; i < P.size(); ++i) {
+        for (size_t k = 0; k < nn; ++k) edge_freq[k] += P[i].adj[k];
+        if (adj_out) std::memcpy(adj_out + i * nn, P[i].adj.data(), nn);
+        if (num_ci_test_out) num_ci_test_out[i] = P[i].num_ci_test;
+        nsep += (int64_t)P[i].seps.size();
+    }
+    ctx->seps.clear();
+    ctx->sep_z.clear();
+    S.n_sepsets = nsep;
     if (stats) *stats = S;
     return FPGM_OK;
 }

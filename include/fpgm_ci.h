@@ -114,6 +114,13 @@ fpgm_status fpgm_ctx_create(const uint8_t *cols, int32_t n_vars, int64_t n_rows,
                             int64_t row_stride, const int32_t *card, fpgm_ctx **out);
 fpgm_status fpgm_ctx_create_device(const void *d_cols, int32_t n_vars, int64_t n_rows,
                                    int64_t row_stride, const int32_t *card, fpgm_ctx **out);
+/*
+ * Context over several datasets with the same variables (e.g. bootstrap replicates):
+ * instance i has n_rows[i] rows at cols[i] with stride row_stride[i].  Used by
+ * fpgm_pc_skeleton_multi; the single-test calls need a one-instance context.
+ */
+fpgm_status fpgm_ctx_create_multi(const uint8_t *const *cols, const int64_t *n_rows, const int64_t *row_stride,
+                                  int32_t n_inst, int32_t n_vars, const int32_t *card, fpgm_ctx **out);
 void fpgm_ctx_destroy(fpgm_ctx *ctx);
 const char *fpgm_last_error(const fpgm_ctx *ctx);
 const char *fpgm_status_str(fpgm_status s);
@@ -154,6 +161,19 @@ fpgm_status fpgm_ci_counts(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z
  */
 fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
                              uint8_t *adj_out, fpgm_stats *stats);
+
+/*
+ * The CI stage of every instance of a multi-instance context, each an independent
+ * PC-stable run with the reference's results (PCStable.cpp:109-232 per instance), run
+ * level-synchronously so that one level's tests of all instances share the launches.
+ * edge_freq (n_vars*n_vars int32) receives, per ordered pair, the number of instances
+ * whose skeleton keeps the edge; adj_out (n_inst*n_vars*n_vars uint8, may be NULL) the
+ * per-instance skeletons; num_ci_test_out (n_inst, may be NULL) per-instance test counts;
+ * stats the per-level sums over instances.  Sepsets are not kept.
+ */
+fpgm_status fpgm_pc_skeleton_multi(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
+                                   int32_t *edge_freq, uint8_t *adj_out, int64_t *num_ci_test_out,
+                                   fpgm_stats *stats);
 
 /* Sizes of the sepset map of the last fpgm_pc_skeleton call (PCStable::sepset). */
 fpgm_status fpgm_sepset_count(const fpgm_ctx *ctx, int64_t *n_records, int64_t *n_z);

@@ -199,3 +199,44 @@ def test_large_tables_and_high_df():
             r = ctx.ci_batch(tests, alpha=0.05, flags=flags)
             assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
     assert (o["df"] > 4096).any()
+
+
+def _ensemble(n_take, max_rows=None):
+    w = configs.get(4)
+    base = w.sample()          # 50 000-row base sample of the 100-variable network
+    reps = []
+    for r in w.replicates(base):
+        if max_rows is None or r.n_rows <= max_rows:
+            reps.append(r)
+        if len(reps) == n_take:
+            break
+    return w, reps
+
+
+def _check_ensemble(w, reps, depth):
+    with _lib.Context.of_many(reps) as ctx:
+        g = ctx.pc_skeleton_multi(w.alpha, depth)
+    freq = np.zeros((w.n_vars, w.n_vars), dtype=np.int32)
+    total = 0
+    for i, r in enumerate(reps):
+        o = O.pc_skeleton(O.Data.of(r), w.alpha, depth, n_threads=8)
+        assert np.array_equal(g["adj"][i], o["adj"]), i
+        assert g["num_ci_test_per_instance"][i] == o["num_ci_test"], i
+        freq += o["adj"]
+        total += o["num_ci_test"]
+    assert np.array_equal(g["edge_freq"], freq)
+    assert g["num_ci_test"] == total
+
+
+def test_bootstrap_ensemble_levels_0_1():
+    # the first 24 bootstrap replicates (2 000 - 50 000 rows), levels 0 and 1, each
+    # checked against its own oracle run; the edge-frequency matrix is their sum
+    w, reps = _ensemble(24)
+    _check_ensemble(w, reps, 2)
+
+
+def test_bootstrap_ensemble_full_depth_small_replicates():
+    # full depth on replicates of at most 12 000 rows (larger bootstrap replicates
+    # of the 50 000-row base inflate G^2 and need millions of tests from level 2 on)
+    w, reps = _ensemble(8, max_rows=12_000)
+    _check_ensemble(w, reps, w.depth)
