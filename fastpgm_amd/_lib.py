@@ -24,12 +24,13 @@ FPGM_MODE_BATCHED = 0
 FPGM_MODE_PLAIN = 1
 FPGM_F_EXACT = 1
 FPGM_F_FP32ONLY = 2
+FPGM_F_NO_GRAM = 4
 
 EXPORTS = [
     "fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ctx_destroy", "fpgm_last_error", "fpgm_status_str",
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats",
-    "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi",
+    "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats",
 ]
 
 
@@ -99,10 +100,11 @@ def load(path: str = LIB_PATH):
     L.fpgm_chisq_crit.restype = f64
     L.fpgm_last_kernel_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
     L.fpgm_ctx_create_multi.argtypes = [vp, vp, vp, i32, i32, vp, C.POINTER(vp)]
+    L.fpgm_last_gram_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
     L.fpgm_pc_skeleton_multi.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, vp, vp, C.POINTER(Stats)]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
-                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi"):
+                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -281,6 +283,11 @@ class Context:
                 "plain_tests_per_level": [int(v) for v in st.plain_tests_per_level[:L]],
                 "plain_seconds_per_level": [float(v) for v in st.plain_seconds_per_level[:L]],
                 "kernel_seconds_per_level": [float(v) for v in st.kernel_seconds_per_level[:L]]}
+
+    def gram_stats(self):
+        ms, ops, pms, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        self._check(self._L.fpgm_last_gram_stats(self.h, C.byref(ms), C.byref(ops), C.byref(pms), C.byref(n)))
+        return {"gram_ms": ms.value, "int8_ops": ops.value, "pairs_ms": pms.value, "launches": n.value}
 
     def kernel_stats(self):
         ms, b, n = C.c_double(), C.c_double(), C.c_int64()

@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfpgm_ci.so")
 GEN_LIB = os.path.join(HERE, "libfpgm_gen.so")
 
-SOURCES = ["ci_kernels.hip", "fpgm_ci.hip"]
-HEADERS = ["ci_kernels.h", "chisq.h"]
+SOURCES = ["ci_kernels.hip", "gram_kernels.hip", "fpgm_ci.hip"]
+HEADERS = ["ci_kernels.h", "gram_kernels.h", "chisq.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -19,6 +19,7 @@
 #include "ci_kernels.h"
 #include "chisq.h"
 #include "fpgm_ci.h"
+#include "gram_kernels.h"
 
 #include <algorithm>
 #include <chrono>
@@ -117,6 +118,12 @@ struct fpgm_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double k_ms = 0.0, k_bytes = 0.0;
     int64_t k_launches = 0;
+
+    // level-0 Gram path
+    DevBuf<int32_t> d_C, d_rowvar, d_tiles, d_off, d_pairs;
+    DevBuf<uint8_t> d_rowstate;
+    double g_ms = 0.0, g_ops = 0.0, g_pair_ms = 0.0;
+    int64_t g_launches = 0;
 
     std::vector<fpgm_sepset> seps;
     std::vector<int32_t> sep_z;
@@ -330,6 +337,120 @@ void reset_kernel_stats(fpgm_ctx *ctx)
     ctx->k_ms = 0.0;
     ctx->k_bytes = 0.0;
     ctx->k_launches = 0;
+    ctx->g_ms = 0.0;
+    ctx->g_ops = 0.0;
+    ctx->g_pair_ms = 0.0;
+    ctx->g_launches = 0;
+}
+
+// Level 0 of one instance on the MFMA units: C = O^T O, then one thread per pair
+// (pairs in tv order, all of instance `inst`).  Returns false in *used when the
+// one-hot width is too large for a dense C (the caller then counts pair by pair).
+fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest> &tv, double alpha, uint32_t flags,
+                            float band, std::vector<DevResult> &res, bool *used)
+{
+    *used = false;
+    const int n = ctx->n_vars;
+    std::vector<int32_t> off(n + 1, 0);
+    for (int v = 0; v < n; ++v) off[v + 1] = off[v] + ctx->card[v];
+    const int64_t W = off[n];
+    const int64_t Wp = std::max<int64_t>(kGramTile, (W + kGramTile - 1) / kGramTile * kGramTile);
+    if (Wp > 32768) return FPGM_OK;
+    *used = true;
+    fpgm_status st = ensure_crit(ctx, alpha);
+    if (st != FPGM_OK) return st;
+    const InstHost &I = ctx->inst[inst];
+    std::vector<int32_t> rowvar((size_t)Wp, 0);
+    std::vector<uint8_t> rowstate((size_t)Wp, 255);
+    for (int v = 0; v < n; ++v)
+        for (int a = 0; a < ctx->card[v]; ++a) {
+            rowvar[off[v] + a] = v;
+            rowstate[off[v] + a] = (uint8_t)a;
+        }
+    const int nt = (int)(Wp / kGramTile);
+    std::vector<int32_t> tiles;
+    for (int i = 0; i < nt; ++i)
+        for (int j = i; j < nt; ++j) {
+            tiles.push_back(i);
+            tiles.push_back(j);
+        }
+    const int64_t n_tp = (int64_t)tiles.size() / 2;
+    const int64_t ksteps = std::max<int64_t>(1, (I.n_rows + 31) / 32);
+    const int64_t target = (int64_t)ctx->n_cu * 8;
+    const int64_t splits = std::min<int64_t>(ksteps, std::max<int64_t>(1, (target + n_tp - 1) / n_tp));
+    const int64_t kps = (ksteps + splits - 1) / splits * 32;
+    const int64_t n_items = n_tp * splits;
+    const int64_t np = (int64_t)tv.size();
+    std::vector<int32_t> pairs((size_t)2 * np);
+    for (int64_t k = 0; k < np; ++k) {
+        pairs[k] = tv[k].x;
+        pairs[np + k] = tv[k].y;
+    }
+    HIP_TRY(ctx->d_C.ensure(Wp * Wp, false, ctx->stream), "alloc gram");
+    HIP_TRY(ctx->d_rowvar.ensure(Wp, false, ctx->stream), "alloc gram rows");
+    HIP_TRY(ctx->d_rowstate.ensure(Wp, false, ctx->stream), "alloc gram rows");
+    HIP_TRY(ctx->d_tiles.ensure((int64_t)tiles.size(), false, ctx->stream), "alloc gram tiles");
+    HIP_TRY(ctx->d_off.ensure(n + 1, false, ctx->stream), "alloc gram offsets");
+    HIP_TRY(ctx->d_pairs.ensure(2 * np, false, ctx->stream), "alloc pairs");
+    HIP_TRY(ctx->d_res.ensure(np, false, ctx->stream), "alloc results");
+    HIP_TRY(hipMemcpyAsync(ctx->d_rowvar.p, rowvar.data(), sizeof(int32_t) * Wp, hipMemcpyHostToDevice, ctx->stream),
+            "upload gram rows");
+    HIP_TRY(hipMemcpyAsync(ctx->d_rowstate.p, rowstate.data(), Wp, hipMemcpyHostToDevice, ctx->stream),
+            "upload gram rows");
+    HIP_TRY(hipMemcpyAsync(ctx->d_tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "upload gram tiles");
+    HIP_TRY(hipMemcpyAsync(ctx->d_off.p, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream),
+            "upload offsets");
+    HIP_TRY(hipMemcpyAsync(ctx->d_pairs.p, pairs.data(), sizeof(int32_t) * 2 * np, hipMemcpyHostToDevice, ctx->stream),
+            "upload pairs");
+    HIP_TRY(hipMemsetAsync(ctx->d_C.p, 0, sizeof(int32_t) * Wp * Wp, ctx->stream), "clear gram");
+    GramArgs g{};
+    g.cols = I.cols;
+    g.stride = I.stride;
+    g.n_rows = I.n_rows;
+    g.row_var = ctx->d_rowvar.p;
+    g.row_state = ctx->d_rowstate.p;
+    g.tiles = ctx->d_tiles.p;
+    g.Wp = (int32_t)Wp;
+    g.k_splits = (int32_t)splits;
+    g.k_per_split = kps;
+    g.C = ctx->d_C.p;
+    PairArgs p{};
+    p.C = ctx->d_C.p;
+    p.Wp = (int32_t)Wp;
+    p.off = ctx->d_off.p;
+    p.card = ctx->d_card;
+    p.px = ctx->d_pairs.p;
+    p.py = ctx->d_pairs.p + np;
+    p.n_pairs = np;
+    p.crit = ctx->d_crit;
+    p.alpha = alpha;
+    p.z_alpha = ctx->z_alpha;
+    p.band = band > 0.f ? band : 0.02f;
+    p.flags = flags;
+    p.res = ctx->d_res.p;
+    hipEvent_t e_mid = nullptr;
+    HIP_TRY(hipEventCreate(&e_mid), "event");
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    HIP_TRY(launch_gram_l0(g, (int)n_items, ctx->stream), "launch gram");
+    HIP_TRY(hipEventRecord(e_mid, ctx->stream), "event");
+    HIP_TRY(launch_pairs_l0(p, ctx->stream), "launch pairs");
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    HIP_TRY(launch_pvalue(ctx->d_res.p, np, alpha, ctx->stream), "launch pvalue");
+    res.resize((size_t)np);
+    HIP_TRY(hipMemcpyAsync(res.data(), ctx->d_res.p, sizeof(DevResult) * np, hipMemcpyDeviceToHost, ctx->stream),
+            "download results");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
+    float ms_g = 0.f, ms_p = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms_g, ctx->ev0, e_mid), "event time");
+    HIP_TRY(hipEventElapsedTime(&ms_p, e_mid, ctx->ev1), "event time");
+    (void)hipEventDestroy(e_mid);
+    ctx->g_ms += ms_g;
+    ctx->g_pair_ms += ms_p;
+    ctx->g_ops += 2.0 * (double)n_tp * kGramTile * kGramTile * (double)I.n_rows;
+    ctx->g_launches += 1;
+    return FPGM_OK;
 }
 
 fpgm_result to_result(const DevResult &r)
@@ -620,6 +741,12 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     ctx->d_done.release();
     ctx->d_flag.release();
     ctx->d_res.release();
+    ctx->d_C.release();
+    ctx->d_rowvar.release();
+    ctx->d_rowstate.release();
+    ctx->d_tiles.release();
+    ctx->d_off.release();
+    ctx->d_pairs.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -763,8 +890,17 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         tv.clear();
         for (int32_t i = 0; i < n_inst; ++i)
             for (size_t e = 0; e < P[i].ex.size(); ++e) tv.push_back(HostTest{P[i].ex[e], P[i].ey[e], 0, i, 0});
-        st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
-        if (st != FPGM_OK) return st;
+        bool gram = false;
+        if (!plain && n_inst == 1 && !(o.flags & FPGM_F_NO_GRAM)) {
+            const double ms0 = ctx->g_ms + ctx->g_pair_ms;
+            st = run_level0_gram(ctx, 0, tv, alpha, o.flags, band, res, &gram);
+            if (st != FPGM_OK) return st;
+            S.kernel_seconds_per_level[0] += (ctx->g_ms + ctx->g_pair_ms - ms0) * 1e-3;
+        }
+        if (!gram) {
+            st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
+            if (st != FPGM_OK) return st;
+        }
         int64_t k = 0, inb = 0, left = 0;
         for (int32_t i = 0; i < n_inst; ++i) {
             PcInst &I = P[i];
@@ -1035,6 +1171,17 @@ fpgm_status fpgm_sepset_get(const fpgm_ctx *ctx, fpgm_sepset *records, int64_t c
         if (!z) return FPGM_EINVAL;
         std::memcpy(z, ctx->sep_z.data(), sizeof(int32_t) * ctx->sep_z.size());
     }
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *int8_ops, double *pairs_ms,
+                                 int64_t *launches)
+{
+    if (!ctx) return FPGM_EINVAL;
+    if (gram_ms) *gram_ms = ctx->g_ms;
+    if (int8_ops) *int8_ops = ctx->g_ops;
+    if (pairs_ms) *pairs_ms = ctx->g_pair_ms;
+    if (launches) *launches = ctx->g_launches;
     return FPGM_OK;
 }
 

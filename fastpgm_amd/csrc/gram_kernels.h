@@ -1,0 +1,44 @@
+// Level-0 one-hot Gram path: C = O^T O on the MFMA units, then per-pair statistics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ci_kernels.h"
+
+namespace fpgm {
+
+constexpr int kGramTile = 128;     // output tile edge (4 waves x 64 x 64)
+constexpr int kPairMaxCard = 256;  // per-thread marginal array of the pair kernel
+
+struct GramArgs {
+    const uint8_t *cols;
+    int64_t stride;
+    int64_t n_rows;
+    const int32_t *row_var;    // [Wp] variable of one-hot row i
+    const uint8_t *row_state;  // [Wp] state of row i; 255 for padding rows (never matches)
+    const int32_t *tiles;      // upper-triangle tile pairs, interleaved (i, j), i <= j
+    int32_t Wp;                // padded width, multiple of kGramTile
+    int32_t k_splits;          // row splits per tile pair
+    int64_t k_per_split;       // rows per split, multiple of 32
+    int32_t *C;                // [Wp][Wp] int32, zero before the launch
+};
+
+struct PairArgs {
+    const int32_t *C;
+    int32_t Wp;
+    const int32_t *off;        // one-hot offset of each variable
+    const int32_t *card;
+    const int32_t *px, *py;    // pairs x < y
+    int64_t n_pairs;
+    const double *crit;
+    double alpha, z_alpha;
+    float band;
+    uint32_t flags;
+    DevResult *res;
+};
+
+hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s);
+hipError_t launch_pairs_l0(const PairArgs &p, hipStream_t s);
+
+}  // namespace fpgm

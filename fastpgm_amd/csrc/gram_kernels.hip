@@ -1,0 +1,235 @@
+// Level 0 of the CI stage as an int8 one-hot Gram matrix on the MFMA units (gfx950).
+//
+// With O the N x W one-hot matrix of all variables (W = sum of cardinalities, column
+// o_v + a is 1 where variable v takes state a), every level-0 contingency table is a
+// block of C = O^T O: n_{vw}[a][b] = C[o_v + a][o_w + b] (the Counts2D table,
+// CellTable.cpp:430-441).  C is computed with v_mfma_i32_32x32x32_i8 (int8 x int8 ->
+// int32, exact) over 128 x 128 tiles of the upper triangle; the one-hot operands are
+// formed in registers from 16-byte loads of the uint8 columns.  A second kernel reads
+// each pair's block and evaluates df / G^2 / decision like the counting kernels.
+//
+// MFMA operand layout (checked on the device): lane l supplies row (or column) l % 32
+// and k = (l / 32) * 16 + byte; accumulator q of lane l holds row (q / 4) * 8 +
+// (l / 32) * 4 + q % 4, column l % 32.
+#include "ci_kernels.h"
+#include "fpgm_ci.h"
+#include "gram_kernels.h"
+
+namespace fpgm {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// 1 in every byte of w equal to the byte pattern pat (pat = state * 0x01010101), else 0.
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
+{
+    const uint32_t t = w ^ pat;
+    const uint32_t y = (t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return (~(y | t) & 0x80808080u) >> 7;
+}
+
+__device__ __forceinline__ v4i onehot16(const uint4 &q, uint32_t pat, uint4 mask)
+{
+    v4i r;
+    r[0] = (int)(eq_bytes(q.x, pat) & mask.x);
+    r[1] = (int)(eq_bytes(q.y, pat) & mask.y);
+    r[2] = (int)(eq_bytes(q.z, pat) & mask.z);
+    r[3] = (int)(eq_bytes(q.w, pat) & mask.w);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t byte_mask_word(int64_t valid, int word)
+{
+    // bytes [4*word, 4*word+4) of a 16-byte group, valid if their index < valid
+    const int64_t lo = (int64_t)word * 4;
+    if (valid >= lo + 4) return 0xFFFFFFFFu;
+    if (valid <= lo) return 0u;
+    return 0xFFFFFFFFu >> (8 * (4 - (int)(valid - lo)));
+}
+
+__global__ __launch_bounds__(256) void k_gram_l0(GramArgs g)
+{
+    const int item = blockIdx.x;
+    const int tp = item / g.k_splits, ks = item - tp * g.k_splits;
+    const int ti = g.tiles[2 * tp], tj = g.tiles[2 * tp + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h = lane >> 5, l32 = lane & 31;
+    const int ib = ti * kGramTile + (wave >> 1) * 64 + l32;
+    const int jb = tj * kGramTile + (wave & 1) * 64 + l32;
+    const uint8_t *pa0 = g.cols + (int64_t)g.row_var[ib] * g.stride;
+    const uint8_t *pa1 = g.cols + (int64_t)g.row_var[ib + 32] * g.stride;
+    const uint8_t *pb0 = g.cols + (int64_t)g.row_var[jb] * g.stride;
+    const uint8_t *pb1 = g.cols + (int64_t)g.row_var[jb + 32] * g.stride;
+    const uint32_t sa0 = (uint32_t)g.row_state[ib] * 0x01010101u, sa1 = (uint32_t)g.row_state[ib + 32] * 0x01010101u;
+    const uint32_t sb0 = (uint32_t)g.row_state[jb] * 0x01010101u, sb1 = (uint32_t)g.row_state[jb + 32] * 0x01010101u;
+
+    v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
+    const int64_t r_begin = (int64_t)ks * g.k_per_split;
+    const int64_t r_end = (r_begin + g.k_per_split < g.n_rows) ? r_begin + g.k_per_split : g.n_rows;
+    const uint4 full = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += 32) {
+        const int64_t rr = r0 + h * 16;
+        uint4 m = full;
+        const int64_t valid = r_end - rr;  // rows of this lane's 16-byte group that count
+        if (valid < 16) {
+            m.x = byte_mask_word(valid, 0);
+            m.y = byte_mask_word(valid, 1);
+            m.z = byte_mask_word(valid, 2);
+            m.w = byte_mask_word(valid, 3);
+        }
+        const uint4 qa0 = *reinterpret_cast<const uint4 *>(pa0 + rr);
+        const uint4 qa1 = *reinterpret_cast<const uint4 *>(pa1 + rr);
+        const uint4 qb0 = *reinterpret_cast<const uint4 *>(pb0 + rr);
+        const uint4 qb1 = *reinterpret_cast<const uint4 *>(pb1 + rr);
+        const v4i a0 = onehot16(qa0, sa0, m), a1 = onehot16(qa1, sa1, m);
+        const v4i b0 = onehot16(qb0, sb0, m), b1 = onehot16(qb1, sb1, m);
+        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+    }
+    // accumulate the split's partial tile into C
+    const int i_base = ti * kGramTile + (wave >> 1) * 64;
+    const int j_base = tj * kGramTile + (wave & 1) * 64;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int row = (q >> 2) * 8 + h * 4 + (q & 3);
+        int32_t *c_r0 = g.C + (int64_t)(i_base + row) * g.Wp + j_base + l32;
+        int32_t *c_r1 = g.C + (int64_t)(i_base + 32 + row) * g.Wp + j_base + l32;
+        if (c00[q]) atomicAdd(c_r0, c00[q]);
+        if (c01[q]) atomicAdd(c_r0 + 32, c01[q]);
+        if (c10[q]) atomicAdd(c_r1, c10[q]);
+        if (c11[q]) atomicAdd(c_r1 + 32, c11[q]);
+    }
+}
+
+__device__ __forceinline__ void nadd(float &s, float &c, float v)
+{
+    const float t = s + v;
+    if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ float term32(int32_t n, int32_t ni, int32_t nj, int32_t nk)
+{
+    const int64_t num = (int64_t)n * nk;
+    const int64_t den = (int64_t)ni * nj;
+    const int64_t diff = num - den;
+    const int64_t ad = diff < 0 ? -diff : diff;
+    float L;
+    if (2 * ad <= den) L = log1pf((float)diff / (float)den);
+    else L = logf((float)num / (float)den);
+    return 2.0f * (float)n * L;
+}
+
+// One thread per pair (x < y): its block of C is the level-0 table n[a][b]; marginals,
+// adjusted df (IndependenceTest.cpp:309-322), FP32 compensated G^2 in the reference's
+// (a, b) order, FP32 decision outside the recheck band, else FP64 G^2 with the
+// reference's expression (IndependenceTest.cpp:327-347) and k_pvalue decides.
+__global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= p.n_pairs) return;
+    const int x = p.px[t], y = p.py[t];
+    const int cx = p.card[x], cy = p.card[y];
+    const int32_t *base = p.C + (int64_t)p.off[x] * p.Wp + p.off[y];
+    int32_t nj[kPairMaxCard];
+    int32_t nk = 0;
+    int aly = 0;
+    for (int b = 0; b < cy; ++b) {
+        int32_t s = 0;
+        for (int a = 0; a < cx; ++a) s += base[(int64_t)a * p.Wp + b];
+        nj[b] = s;
+        nk += s;
+        aly += (s > 0);
+    }
+    int alx = 0;
+    float s = 0.f, c = 0.f, as = 0.f;
+    for (int a = 0; a < cx; ++a) {
+        const int32_t *row = base + (int64_t)a * p.Wp;
+        int32_t ni = 0;
+        for (int b = 0; b < cy; ++b) ni += row[b];
+        alx += (ni > 0);
+        if (ni == 0) continue;
+        for (int b = 0; b < cy; ++b) {
+            const int32_t n = row[b];
+            if (n == 0) continue;
+            const float tm = term32(n, ni, nj[b], nk);
+            nadd(s, c, tm);
+            as += fabsf(tm);
+        }
+    }
+    alx = alx >= 1 ? alx : 1;
+    aly = aly >= 1 ? aly : 1;
+    const int df = (alx - 1) * (aly - 1);
+    const float g2 = s + c;
+    DevResult R;
+    R.g2_32 = g2;
+    R.abs_sum = as;
+    R.df = df;
+    R.indep = 0;
+    R.in_band = 0;
+    R.exact = 0;
+    R.need_p = 0;
+    R.g2_64 = NAN;
+    R.p64 = (df == 0) ? 1.0 : NAN;
+    bool need64 = (p.flags & FPGM_F_EXACT) != 0;
+    if (df == 0) {
+        R.indep = 1;  // IndependenceTest.cpp:349-351
+    } else {
+        double crit;
+        if (df <= kCritTab) {
+            crit = p.crit[df - 1];
+        } else {
+            const double k = (double)df, q = 2.0 / (9.0 * k), u = 1.0 - q + p.z_alpha * sqrt(q);
+            crit = k * u * u * u;
+        }
+        const double bound = 4e-6 * (double)as + (df > kCritTab ? 1e-5 * crit : 0.0) + 1e-30;
+        const double width = fmax((double)p.band * crit, bound);
+        R.in_band = (isnan(crit) || fabs((double)g2 - crit) <= width) ? 1 : 0;
+        if ((p.flags & FPGM_F_FP32ONLY) && !isnan(crit)) R.indep = (double)g2 < crit;
+        else if (R.in_band) need64 = true;
+        else R.indep = (double)g2 < crit;
+    }
+    if (need64) {
+        double G = 0.0;
+        for (int a = 0; a < cx; ++a) {
+            const int32_t *row = base + (int64_t)a * p.Wp;
+            int32_t ni = 0;
+            for (int b = 0; b < cy; ++b) ni += row[b];
+            if (ni == 0) continue;
+            for (int b = 0; b < cy; ++b) {
+                const int32_t n = row[b];
+                if (nj[b] == 0 || n == 0) continue;
+                const double expected = (double)nj[b] * (double)ni / (double)nk;
+                G += 2.0 * (double)n * log((double)n / expected);
+            }
+        }
+        R.g2_64 = G;
+        R.exact = 1;
+        R.need_p = (df != 0) ? 1 : 0;
+    }
+    p.res[t] = R;
+}
+
+}  // namespace
+
+hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gram_l0, dim3(n_items), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_l0(const PairArgs &p, hipStream_t s)
+{
+    if (p.n_pairs <= 0) return hipSuccess;
+    const int64_t blocks = (p.n_pairs + 255) / 256;
+    hipLaunchKernelGGL(k_pairs_l0, dim3((unsigned)blocks), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace fpgm

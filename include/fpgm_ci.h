@@ -74,6 +74,7 @@ typedef struct {
 
 #define FPGM_F_EXACT 1u     /* compute FP64 G^2 and p-value for every test              */
 #define FPGM_F_FP32ONLY 2u  /* diagnostics: decide every test from FP32 alone           */
+#define FPGM_F_NO_GRAM 4u   /* level 0 by the counting kernel, not the MFMA Gram path    */
 
 /* Per-level statistics of the last fpgm_pc_skeleton call. */
 typedef struct {
@@ -191,6 +192,11 @@ double fpgm_chisq_crit(int32_t df, double alpha);
  * kernel launches of the last fpgm_pc_skeleton / fpgm_ci_batch call. */
 fpgm_status fpgm_last_kernel_stats(const fpgm_ctx *ctx, double *kernel_ms, double *alg_bytes,
                                    int64_t *launches);
+
+/* Level-0 Gram path of the last fpgm_pc_skeleton call: MFMA kernel time, int8 operations
+ * (2 * rows * 128 * 128 per upper-triangle tile), per-pair statistic kernel time. */
+fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *int8_ops, double *pairs_ms,
+                                 int64_t *launches);
 
 #ifdef __cplusplus
 }

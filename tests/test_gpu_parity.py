@@ -240,3 +240,21 @@ def test_bootstrap_ensemble_full_depth_small_replicates():
     # of the 50 000-row base inflate G^2 and need millions of tests from level 2 on)
     w, reps = _ensemble(8, max_rows=12_000)
     _check_ensemble(w, reps, w.depth)
+
+
+@pytest.mark.parametrize("n_rows", [4099, 33, 100_000])
+def test_level0_gram_matches_counting_kernel(n_rows):
+    # level 0 on the MFMA Gram path (default) and on the counting kernel give the same
+    # stage; both equal the oracle
+    w = configs.get(2)
+    s = w.sample(n_rows)
+    D = O.Data.of(s)
+    o = O.pc_skeleton(D, w.alpha, 2, n_threads=8)
+    with _ctx(s) as ctx:
+        a = ctx.pc_skeleton(w.alpha, 2)
+        gs = ctx.gram_stats()
+        b = ctx.pc_skeleton(w.alpha, 2, flags=_lib.FPGM_F_NO_GRAM)
+        gb = ctx.gram_stats()
+    assert gs["launches"] == 1 and gb["launches"] == 0
+    _assert_same(a, o)
+    _assert_same(b, o)
