@@ -12,6 +12,7 @@ constexpr int kRowsPerStep = kWG * kRowsPerThread;  // 4096 rows per workgroup s
 constexpr int kLdsCells = 8192;           // LDS table capacity in int32 cells (32 KiB)
 constexpr int kLaneWords = kLdsCells / (4 * 64);    // 16-bit counter pairs per lane (4 waves)
 constexpr int kLaneCells = 2 * kLaneWords;          // largest table counted per lane (64)
+constexpr int kLaneWideCells = 2 * kLaneCells;      // per-lane sink with 64 KiB of counters (128)
 constexpr int kSmallCells = 512;          // LDS table + marginals of per-lane tests
 constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bit counters never wrap
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
@@ -75,6 +76,7 @@ struct LaunchArgs {
 constexpr int kSinkShared = 0;  // LDS histogram shared by waves (copies per wave when small)
 constexpr int kSinkGlobal = 1;  // device-memory table, device atomics (beyond LDS)
 constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
+constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 64 KiB (<= 128 cells)
 
 hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s);
 hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s);

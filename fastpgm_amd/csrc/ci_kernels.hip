@@ -30,6 +30,14 @@ __device__ __forceinline__ uint32_t byte_of(const uint4 &v, int b)
     return __builtin_amdgcn_ubfe(w, (b & 3) * 8, 8);
 }
 
+// a * b + c on the 24-bit multiplier (v_mad_u32_u24): table keys of LDS sinks stay
+// below 2^24.  The explicit masks let the compiler prove the operand widths; without
+// them it can select the 64-bit multiply-add for a 24-bit product plus a byte.
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c)
+{
+    return (a & 0xFFFFFFu) * (b & 0xFFFFFFu) + c;
+}
+
 __device__ __forceinline__ void nadd(float &s, float &c, float v)
 {
     const float t = s + v;
@@ -105,8 +113,9 @@ template <int kSink>
 __device__ __forceinline__ void sink(int32_t *h, uint32_t key)
 {
     if (kSink == kSinkLane) {
-        // h = this lane's column of the wave's [word][lane] block
-        atomicAdd(reinterpret_cast<uint32_t *>(h) + (key >> 1) * 64u, 1u << ((key & 1u) << 4));
+        // h = this lane's column of the wave's [word][lane] block, two 16-bit counters a word
+        uint32_t *p = reinterpret_cast<uint32_t *>(h) + mad24(key >> 1, 64u, 0u);
+        atomicAdd(p, 1u << ((key & 1u) << 4));
     } else {
         atomicAdd(&h[key], 1);
     }
@@ -157,9 +166,9 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
             uint32_t k = 0;
 #pragma unroll
             for (int j = 0; j < D; ++j)
-                k = (kSink == kSinkGlobal) ? k * zc[j] + byte_of(cur.v[j], b) : __umul24(k, zc[j]) + byte_of(cur.v[j], b);
+                k = (kSink == kSinkGlobal) ? k * zc[j] + byte_of(cur.v[j], b) : mad24(k, zc[j], byte_of(cur.v[j], b));
             if (kSink == kSinkGlobal) k = (k * cx + byte_of(cur.v[X], b)) * cy + byte_of(cur.v[X + 1], b);
-            else k = __umul24(__umul24(k, cx) + byte_of(cur.v[X], b), cy) + byte_of(cur.v[X + 1], b);
+            else k = mad24(mad24(k, cx, byte_of(cur.v[X], b)), cy, byte_of(cur.v[X + 1], b));
             key[b] = k;
         }
         const int64_t nv = r_end - r;
@@ -194,14 +203,14 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
             const uint32_t c = (uint32_t)sh.zcard[j];
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                key[b] = (kSink == kSinkGlobal) ? key[b] * c + byte_of(v, b) : __umul24(key[b], c) + byte_of(v, b);
+                key[b] = (kSink == kSinkGlobal) ? key[b] * c + byte_of(v, b) : mad24(key[b], c, byte_of(v, b));
         }
         const uint4 vx = *reinterpret_cast<const uint4 *>(colx + r);
         const uint4 vy = *reinterpret_cast<const uint4 *>(coly + r);
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
             if (kSink == kSinkGlobal) key[b] = (key[b] * cx + byte_of(vx, b)) * cy + byte_of(vy, b);
-            else key[b] = __umul24(__umul24(key[b], cx) + byte_of(vx, b), cy) + byte_of(vy, b);
+            else key[b] = mad24(mad24(key[b], cx, byte_of(vx, b)), cy, byte_of(vy, b));
         }
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
@@ -446,10 +455,16 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
 }
 
 template <int kMode>
+struct LdsWords {
+    static constexpr int value = (kMode == kSinkGlobal) ? 1 : (kMode == kSinkLaneW) ? 2 * kLdsCells : kLdsCells;
+};
+
+template <int kMode>
 __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
 {
-    __shared__ int32_t lds[kLdsCells];
+    __shared__ int32_t lds[LdsWords<kMode>::value];
     __shared__ int32_t small[kSmallCells];
+
     __shared__ Shared sh;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
@@ -477,15 +492,15 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
         __syncthreads();
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
         const int32_t *tab = nullptr;  // the chunk's table in LDS, when there is one
-        if (kMode == kSinkLane) {
+        if (kMode == kSinkLane || kMode == kSinkLaneW) {
             // per-lane 16-bit counters: [wave][word][lane], words = ceil(cells / 2)
+            constexpr int kW = (kMode == kSinkLaneW) ? 2 * kLaneWords : kLaneWords;
             const int c32 = (int)cells;
             const int words = (c32 + 1) >> 1;
             uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
-            for (int i = tid; i < 4 * kLaneWords * 64; i += kWG) pl[i] = 0;
+            for (int i = tid; i < 4 * kW * 64; i += kWG) pl[i] = 0;
             __syncthreads();
-            count_rows<kSinkLane>(I, T, sh, r_begin, r_end,
-                                  reinterpret_cast<int32_t *>(pl + wave * kLaneWords * 64 + lane));
+            count_rows<kSinkLane>(I, T, sh, r_begin, r_end, reinterpret_cast<int32_t *>(pl + wave * kW * 64 + lane));
             __syncthreads();
             // sum the 256 private counters of every cell: 8 threads per word, each over
             // 8 lanes of all 4 waves, then a 3-step shuffle reduction
@@ -496,7 +511,7 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
                 for (int wv = 0; wv < 4; ++wv)
 #pragma unroll
                     for (int l = 0; l < 8; ++l) {
-                        const uint32_t v = pl[(wv * kLaneWords + w) * 64 + part * 8 + l];
+                        const uint32_t v = pl[(wv * kW + w) * 64 + part * 8 + l];
                         lo16 += v & 0xffffu;
                         hi16 += v >> 16;
                     }
@@ -603,6 +618,7 @@ hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_
     if (a.n_items <= 0) return hipSuccess;
     switch (mode) {
     case kSinkLane: hipLaunchKernelGGL(k_count_stat<kSinkLane>, dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkLaneW: hipLaunchKernelGGL(k_count_stat<kSinkLaneW>, dim3(grid), dim3(kWG), 0, s, a); break;
     case kSinkShared: hipLaunchKernelGGL(k_count_stat<kSinkShared>, dim3(grid), dim3(kWG), 0, s, a); break;
     default: hipLaunchKernelGGL(k_count_stat<kSinkGlobal>, dim3(grid), dim3(kWG), 0, s, a); break;
     }

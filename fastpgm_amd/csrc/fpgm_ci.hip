@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -104,6 +105,7 @@ struct fpgm_ctx {
     std::string err;
     std::mutex mu;
     int n_cu = 256;
+    uint32_t sink_mask = 0;  // bit 0: no wide per-lane sink (FPGM_SINKS=shared, for comparisons)
 
     double crit_alpha = -1.0;
     double z_alpha = 0.0;
@@ -237,6 +239,9 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
                 mode = kSinkLane;
                 T.flags |= kTFLane;
                 fits = cells + marg <= kSmallCells;
+            } else if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
+                mode = kSinkLaneW;
+                fits = cells + marg <= kSmallCells;
             } else if (cells <= kLdsCells) {
                 mode = kSinkShared;
                 T.flags |= kTFLdsTable;
@@ -266,11 +271,12 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         const int64_t nb = (int64_t)dt.size();
         // work items grouped by sink: [lane | shared | global], one launch per group;
         // within a group, tests keep the caller's order (neighbouring tests share columns)
-        int64_t grp_begin[3] = {0, 0, 0}, grp_len[3] = {0, 0, 0};
-        const int order[3] = {kSinkLane, kSinkShared, kSinkGlobal};
+        constexpr int kGroups = 4;
+        int64_t grp_begin[kGroups] = {0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0};
+        const int order[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkGlobal};
         item_test.clear();
         item_test.reserve((size_t)n_items_total);
-        for (int gi = 0; gi < 3; ++gi) {
+        for (int gi = 0; gi < kGroups; ++gi) {
             grp_begin[gi] = (int64_t)item_test.size();
             for (int64_t k = 0; k < nb; ++k) {
                 if (mode_of[k] != order[gi]) continue;
@@ -305,11 +311,11 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         a.flags = flags;
         a.res = ctx->d_res.p;
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-        for (int gi = 0; gi < 3; ++gi) {
+        for (int gi = 0; gi < kGroups; ++gi) {
             if (grp_len[gi] == 0) continue;
             a.item_begin = grp_begin[gi];
             a.n_items = grp_len[gi];
-            const int64_t waves_per_cu = (order[gi] == kSinkGlobal) ? 8 : 4;
+            const int64_t waves_per_cu = (order[gi] == kSinkGlobal) ? 8 : (order[gi] == kSinkLaneW) ? 2 : 4;
             int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * waves_per_cu);
             grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
             HIP_TRY(launch_count_stat(a, order[gi], (int)grid, ctx->stream), "launch count_stat");
@@ -606,6 +612,8 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         return fail(ctx, FPGM_EDEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
     ctx->dev = dev;
     ctx->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (const char *e = std::getenv("FPGM_SINKS"))
+        if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 1;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ctx->ev1), "hipEventCreate");

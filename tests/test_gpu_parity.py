@@ -258,3 +258,26 @@ def test_level0_gram_matches_counting_kernel(n_rows):
     assert gs["launches"] == 1 and gb["launches"] == 0
     _assert_same(a, o)
     _assert_same(b, o)
+
+
+def test_mid_tables_many_steps_per_chunk():
+    # 300 003 rows and thousands of tests: every chunk spans tens of 4096-row steps, so the
+    # 8-bit per-lane counters (tables of 65..256 cells) go through several flushes
+    w = configs.get(3)
+    net = w.network()
+    s = datagen.forward_sample(net, 300_003, w.seed)
+    card = s.card
+    rng = np.random.default_rng(11)
+    tests = []
+    while len(tests) < 3000:
+        vs = rng.choice(s.n_vars, size=3, replace=False)
+        if 65 <= int(np.prod(card[vs])) <= 256:
+            x, y = sorted(int(v) for v in vs[:2])
+            tests.append((x, y, (int(vs[2]),)))
+    D = O.Data.of(s)
+    arr = np.array([[x, y, 1, z[0]] for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
+    with _ctx(s) as ctx:
+        r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+    assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+    np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
