@@ -28,7 +28,10 @@ struct DevInst {
 // One test as the device sees it (48 bytes).
 struct DevTest {
     int32_t x, y, d, z_off;  // z_off into the device z pool
-    int32_t inst, pad0;      // dataset instance
+    int32_t inst;            // dataset instance
+    int32_t n_slices;        // z slices of a sliced table (1 otherwise)
+    int32_t slice_cells;     // cells per slice (whole z planes)
+    int32_t pad0;
     int32_t cx, cy;
     int32_t dimz;            // product of the Z cardinalities
     int32_t n_chunks;        // row chunks of this test (work items)
@@ -77,6 +80,9 @@ constexpr int kSinkShared = 0;  // LDS histogram shared by waves (copies per wav
 constexpr int kSinkGlobal = 1;  // device-memory table, device atomics (beyond LDS)
 constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
 constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 64 KiB (<= 128 cells)
+constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
+constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
+constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 
 hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s);
 hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s);

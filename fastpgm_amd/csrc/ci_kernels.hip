@@ -81,6 +81,7 @@ struct Shared {
     float red_f[3 * (kWG / 64)];
     double red_d[kWG / 64];
     int32_t flag;
+    int32_t slice_lo, slice_len;  // key range of this work item's slice (kSinkSlice)
     int32_t df;
     int32_t need64;
     float g2_32;
@@ -119,6 +120,23 @@ __device__ __forceinline__ void sink(int32_t *h, uint32_t key)
     } else {
         atomicAdd(&h[key], 1);
     }
+}
+
+template <int kSink>
+__device__ __forceinline__ void put(int32_t *h, uint32_t key, const Shared &sh);
+
+// kSinkSlice: only keys inside the work item's slice, relative to its first key
+__device__ __forceinline__ void sink_slice(int32_t *h, uint32_t key, uint32_t lo, uint32_t len)
+{
+    const uint32_t k = key - lo;
+    if (k < len) atomicAdd(&h[k], 1);
+}
+
+template <int kSink>
+__device__ __forceinline__ void put(int32_t *h, uint32_t key, const Shared &sh)
+{
+    if (kSink == kSinkSlice) sink_slice(h, key, (uint32_t)sh.slice_lo, (uint32_t)sh.slice_len);
+    else sink<kSink>(h, key);
 }
 
 template <int D>
@@ -174,11 +192,11 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int b = 0; b < kRowsPerThread; ++b) sink<kSink>(h, key[b]);
+            for (int b = 0; b < kRowsPerThread; ++b) put<kSink>(h, key[b], sh);
         } else {
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                if (b < nv) sink<kSink>(h, key[b]);
+                if (b < nv) put<kSink>(h, key[b], sh);
         }
         cur = nxt;
         r = rn;
@@ -215,11 +233,11 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int b = 0; b < kRowsPerThread; ++b) sink<kSink>(h, key[b]);
+            for (int b = 0; b < kRowsPerThread; ++b) put<kSink>(h, key[b], sh);
         } else {
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                if (b < nv) sink<kSink>(h, key[b]);
+                if (b < nv) put<kSink>(h, key[b], sh);
         }
     }
 }
@@ -456,7 +474,8 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
 
 template <int kMode>
 struct LdsWords {
-    static constexpr int value = (kMode == kSinkGlobal) ? 1 : (kMode == kSinkLaneW) ? 2 * kLdsCells : kLdsCells;
+    static constexpr int value =
+        (kMode == kSinkGlobal) ? 1 : (kMode == kSinkLaneW || kMode == kSinkSlice) ? 2 * kLdsCells : kLdsCells;
 };
 
 template <int kMode>
@@ -480,7 +499,9 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
         const int t = a.item_test[item];
         const DevTest T = a.tests[t];
         const DevInst I = a.inst[T.inst];
-        const int64_t chunk = item - T.item0;
+        const int64_t local = item - T.item0;  // slice * n_chunks + chunk
+        const int64_t slice = (kMode == kSinkSlice) ? local / T.n_chunks : 0;
+        const int64_t chunk = local - slice * T.n_chunks;
         const int64_t r_begin = chunk * a.rows_per_chunk;
         const int64_t r_end = (r_begin + a.rows_per_chunk < I.n_rows) ? r_begin + a.rows_per_chunk : I.n_rows;
         const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
@@ -542,6 +563,21 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
                 __syncthreads();
             }
             tab = lds;
+        } else if (kMode == kSinkSlice) {
+            const int32_t lo = (int32_t)slice * T.slice_cells;
+            const int32_t len = (int32_t)((cells - lo < T.slice_cells) ? cells - lo : T.slice_cells);
+            if (tid == 0) {
+                sh.slice_lo = lo;
+                sh.slice_len = len;
+            }
+            for (int i = tid; i < len; i += kWG) lds[i] = 0;
+            __syncthreads();
+            count_rows<kSinkSlice>(I, T, sh, r_begin, r_end, lds);
+            __syncthreads();
+            for (int i = tid; i < len; i += kWG) {
+                const int32_t v = lds[i];
+                if (v) atomicAdd(&g[lo + i], v);
+            }
         } else {
             count_rows<kSinkGlobal>(I, T, sh, r_begin, r_end, g);
         }
@@ -560,7 +596,7 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
         __syncthreads();
         if (tid == 0) {
             const uint32_t prev = atomicAdd(&a.done[t], 1u);
-            sh.flag = (prev + 1 == (uint32_t)T.n_chunks) ? 1 : 0;
+            sh.flag = (prev + 1 == (uint32_t)(T.n_chunks * T.n_slices)) ? 1 : 0;
         }
         __syncthreads();
         if (sh.flag) {
@@ -619,6 +655,7 @@ hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_
     switch (mode) {
     case kSinkLane: hipLaunchKernelGGL(k_count_stat<kSinkLane>, dim3(grid), dim3(kWG), 0, s, a); break;
     case kSinkLaneW: hipLaunchKernelGGL(k_count_stat<kSinkLaneW>, dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkSlice: hipLaunchKernelGGL(k_count_stat<kSinkSlice>, dim3(grid), dim3(kWG), 0, s, a); break;
     case kSinkShared: hipLaunchKernelGGL(k_count_stat<kSinkShared>, dim3(grid), dim3(kWG), 0, s, a); break;
     default: hipLaunchKernelGGL(k_count_stat<kSinkGlobal>, dim3(grid), dim3(kWG), 0, s, a); break;
     }

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -105,7 +106,13 @@ struct fpgm_ctx {
     std::string err;
     std::mutex mu;
     int n_cu = 256;
-    uint32_t sink_mask = 0;  // bit 0: no wide per-lane sink (FPGM_SINKS=shared, for comparisons)
+    struct {
+        double base = 2.0;    // round ramp factor
+        int64_t cap = 8;      // ramp cap (tests per edge per round); larger rounds come from the budget
+        int64_t promote = 0;  // dependent tests after which an edge runs the rest of its walk
+    } sched;
+    int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
+    uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices
 
     double crit_alpha = -1.0;
     double z_alpha = 0.0;
@@ -231,6 +238,8 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
                 return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
             T.dimz = (int32_t)dimz;
             T.n_chunks = (int32_t)n_chunks;
+            T.n_slices = 1;
+            T.slice_cells = (int32_t)cells;
             const int64_t marg = dimz * (T.cx + T.cy + 1);
             T.flags = tflags;
             bool fits;
@@ -249,6 +258,16 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             } else {
                 mode = kSinkGlobal;
                 fits = false;
+                const int64_t cxy = (int64_t)T.cx * T.cy;
+                if (cxy <= kSliceCells && !(ctx->sink_mask & 2)) {
+                    const int64_t sc = (kSliceCells / cxy) * cxy;
+                    const int64_t ns = (cells + sc - 1) / sc;
+                    if (ns <= ctx->max_slices) {
+                        mode = kSinkSlice;
+                        T.n_slices = (int32_t)ns;
+                        T.slice_cells = (int32_t)sc;
+                    }
+                }
             }
             const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fits;
             if (!need_ws) T.flags |= kTFLdsMarg;
@@ -262,7 +281,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             }
             dt.push_back(T);
             mode_of.push_back(mode);
-            n_items_total += n_chunks;
+            n_items_total += n_chunks * T.n_slices;
             if (n_items_total > ((int64_t)1 << 30)) {
                 ++i1;
                 break;
@@ -271,9 +290,9 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         const int64_t nb = (int64_t)dt.size();
         // work items grouped by sink: [lane | shared | global], one launch per group;
         // within a group, tests keep the caller's order (neighbouring tests share columns)
-        constexpr int kGroups = 4;
-        int64_t grp_begin[kGroups] = {0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0};
-        const int order[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkGlobal};
+        constexpr int kGroups = 5;
+        int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
+        const int order[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
         item_test.clear();
         item_test.reserve((size_t)n_items_total);
         for (int gi = 0; gi < kGroups; ++gi) {
@@ -281,7 +300,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             for (int64_t k = 0; k < nb; ++k) {
                 if (mode_of[k] != order[gi]) continue;
                 dt[k].item0 = (int32_t)item_test.size();
-                for (int64_t c = 0; c < dt[k].n_chunks; ++c) item_test.push_back((int32_t)k);
+                for (int64_t c = 0; c < (int64_t)dt[k].n_chunks * dt[k].n_slices; ++c) item_test.push_back((int32_t)k);
             }
             grp_len[gi] = (int64_t)item_test.size() - grp_begin[gi];
         }
@@ -315,7 +334,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             if (grp_len[gi] == 0) continue;
             a.item_begin = grp_begin[gi];
             a.n_items = grp_len[gi];
-            const int64_t waves_per_cu = (order[gi] == kSinkGlobal) ? 8 : (order[gi] == kSinkLaneW) ? 2 : 4;
+            const int64_t waves_per_cu = (order[gi] == kSinkGlobal) ? 8 : (order[gi] == kSinkLaneW || order[gi] == kSinkSlice) ? 2 : 4;
             int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * waves_per_cu);
             grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
             HIP_TRY(launch_count_stat(a, order[gi], (int)grid, ctx->stream), "launch count_stat");
@@ -612,8 +631,12 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         return fail(ctx, FPGM_EDEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
     ctx->dev = dev;
     ctx->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (const char *e = std::getenv("FPGM_ROUND_BASE")) ctx->sched.base = std::max(1.0, std::atof(e));
+    if (const char *e = std::getenv("FPGM_ROUND_CAP")) ctx->sched.cap = std::max<int64_t>(1, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_ROUND_PROMOTE")) ctx->sched.promote = std::max<int64_t>(0, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_MAX_SLICES")) ctx->max_slices = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS"))
-        if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 1;
+        if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ctx->ev1), "hipEventCreate");
@@ -987,9 +1010,13 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                 return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
             ++rounds;
             const bool plain_round = plain && (o.plain_sample <= 0 || S.plain_tests_per_level[d] < o.plain_sample);
+            // tests per edge this round: the budget shared by the active edges, at least a
+            // geometric ramp (base^(round-1), capped) so long walks finish in few rounds
             int64_t W = plain_round ? 1 : std::max<int64_t>(1, budget / (int64_t)active.size());
-            if (!plain_round)
-                W = std::max<int64_t>(W, std::min<int64_t>((int64_t)1 << std::min<int64_t>(rounds - 1, 20), 4096));
+            if (!plain_round) {
+                const double ramp = std::pow(ctx->sched.base, (double)(rounds - 1));
+                W = std::max<int64_t>(W, (int64_t)std::min<double>(ramp, (double)ctx->sched.cap));
+            }
             tv.clear();
             zpool.clear();
             t_edge.clear();
@@ -998,7 +1025,11 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                 const int64_t e = active[ai].second;
                 EdgeWalk &w = I.walks[e];
                 int32_t *c = &I.choice[(size_t)e * d];
-                for (int64_t k = 0; k < W && w.side < 2; ++k) {
+                // an edge found dependent on its first `promote` sets rarely ends early:
+                // give it the rest of its walk at once
+                int64_t We = W;
+                if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
+                for (int64_t k = 0; k < We && w.side < 2; ++k) {
                     tv.push_back(HostTest{w.x, w.y, d, active[ai].first, (int64_t)zpool.size()});
                     for (int i = 0; i < d; ++i) zpool.push_back(I.F.at(w.u, w.skip, c[i]));
                     t_edge.push_back((int64_t)ai);

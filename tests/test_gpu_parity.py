@@ -281,3 +281,24 @@ def test_mid_tables_many_steps_per_chunk():
         r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
     assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
     np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+
+
+def test_sliced_and_device_memory_sinks():
+    # card 8 throughout: |Z| = 2 -> 4096 cells (LDS), 3 -> 32768 (2 z slices),
+    # 4 -> 262144 (16 slices: device-memory sink), 5 -> 2^21 (device-memory sink)
+    s = datagen.random_sample(21, 9, 50_001, 8, 8)
+    D = O.Data.of(s)
+    rng = np.random.default_rng(5)
+    tests = []
+    for d in (2, 3, 4, 5):
+        for _ in range(4):
+            vs = rng.choice(9, size=d + 2, replace=False)
+            tests.append((int(min(vs[0], vs[1])), int(max(vs[0], vs[1])), tuple(sorted(int(v) for v in vs[2:]))))
+    arr = np.array([[x, y, len(z)] + list(z) + [0] * (5 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 5, 0.05, n_threads=8)
+    with _ctx(s) as ctx:
+        r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+        for (x, y, z) in tests[4:9]:
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
+    assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+    np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
