@@ -83,23 +83,50 @@ def cpu_baseline(sample, w, g, seconds, threads):
                       f"(est. {est:.1f} s for {g['num_ci_test']} tests)"}
 
 
-def hbm_traffic_from_counters(fetch_csv, write_csv, copy_bytes=1 << 30):
-    """HBM bytes moved by the counting kernels, from rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE counter_collection.csv files (KiB per dispatch) of a run made with
-    --copy-calibration: each counter is scaled by copy_bytes over what it read for the
-    largest copy dispatch (the calibration copy)."""
+def hbm_traffic_from_counters(fetch_csv, write_csv, stages, copy_bytes=1 << 30):
+    """HBM bytes moved by the counting kernels per CI stage, from rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE counter_collection.csv files (KiB per dispatch) of a
+    bench run made with --copy-calibration that ran `stages` CI stages: each counter
+    is scaled by copy_bytes over what it read for the largest copy dispatch (the
+    calibration copy), then divided by the stage count."""
     import csv
 
     def totals(path):
         rows = list(csv.DictReader(open(path)))
         copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
         kern = [float(r["Counter_Value"]) for r in rows if "k_count_stat" in r["Kernel_Name"]]
-        return sum(kern) * 1024.0 * (copy_bytes / (copy * 1024.0)), len(kern), copy_bytes / (copy * 1024.0)
+        scale = copy_bytes / (copy * 1024.0)
+        return sum(kern) * 1024.0 * scale / stages, len(kern) / stages, scale
 
     f, n, ff = totals(fetch_csv)
     wr, _, wf = totals(write_csv)
     return {"fetch_bytes": f, "write_bytes": wr, "dispatches": n, "fetch_scale": ff, "write_scale": wf,
             "bytes_per_launch": (f + wr) / max(n, 1)}
+
+
+def summarize_pmc(pmc_dir):
+    """Write the counting kernels' per-stage HBM traffic into profiles/hbm_traffic.json
+    from a directory holding fetch/ and write/ (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE
+    output) and run.json (the JSON line the profiled bench run printed).  No GPU use."""
+    import glob
+    run = json.load(open(os.path.join(pmc_dir, "run.json")))
+    fc = glob.glob(os.path.join(pmc_dir, "fetch", "*counter_collection.csv"))
+    wc = glob.glob(os.path.join(pmc_dir, "write", "*counter_collection.csv"))
+    if not fc or not wc:
+        raise SystemExit(f"no counter_collection.csv under {pmc_dir}/fetch and {pmc_dir}/write")
+    t = hbm_traffic_from_counters(fc[0], wc[0], run["stages_in_run"])
+    rl = run["roofline"]
+    alg, kms = rl["alg_bytes_per_step"], rl["kernel_ms_per_step"]
+    t.update({"alg_bytes": alg, "traffic_over_alg": (t["fetch_bytes"] + t["write_bytes"]) / alg,
+              "kernel_ms_step": kms,
+              "hbm_gbs_measured": (t["fetch_bytes"] + t["write_bytes"]) / (kms * 1e-3) / 1e9})
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    rec = json.load(open(path)) if os.path.exists(path) else {}
+    rec[run["config"]["workload"]] = t
+    with open(path, "w") as fh:
+        json.dump(rec, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps({run["config"]["workload"]: t}, indent=1))
 
 
 def copy_calibration(n_bytes=1 << 30):
@@ -130,11 +157,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--pmc-dir", default="",
-                    help="directory with fetch/ and write/ rocprofv3 counter CSVs of this workload")
+    ap.add_argument("--summarize-pmc", default="", metavar="DIR",
+                    help="only write profiles/hbm_traffic.json from DIR/{fetch,write}/ rocprofv3 "
+                         "--pmc CSVs and DIR/run.json (the profiled run's output line); no GPU use")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the CI stage of one PC-stable run is not sharded across GPUs in "
@@ -195,6 +226,7 @@ def main():
                    "depth": w.depth, "dag_edges": w.n_edges, "max_in_degree": w.max_in,
                    "card_law": list(w.card_law), "dirichlet": w.conc, "data_sha256": sample.sha256()[:16]},
         "pc_stable_step1_seconds": elapsed / args.steps,
+        "stages_in_run": args.warmup + args.steps + 1,
         "num_ci_test": g["num_ci_test"],
         "tests_per_level": g["tests_per_level"],
         "executed_per_level": g["executed_per_level"],
@@ -210,17 +242,8 @@ def main():
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if args.pmc_dir:
-        import glob
-        fc = glob.glob(os.path.join(args.pmc_dir, "fetch", "*counter_collection.csv"))
-        wc = glob.glob(os.path.join(args.pmc_dir, "write", "*counter_collection.csv"))
-        if fc and wc:
-            traffic = hbm_traffic_from_counters(fc[0], wc[0])
-            traffic["alg_bytes"] = alg_bytes / args.steps
-    elif os.path.exists(tfile):
-        rec = json.load(open(tfile)).get(w.name)
-        if rec:
-            traffic = rec
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(w.name)
     if traffic:
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
         out["roofline"]["traffic_detail"] = traffic
