@@ -24,6 +24,17 @@ namespace fpgm {
 
 namespace {
 
+// 16-byte load of column bytes through the global address space: the column pointers
+// come from descriptors in memory, and a generic (flat) load would also count against
+// the LDS wait counter and serialise with the LDS sinks.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 ld16(const uint8_t *p)
+{
+    const u32x4 v = *(const __attribute__((address_space(1))) u32x4 *)(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ uint32_t byte_of(const uint4 &v, int b)
 {
     const uint32_t w = (b < 4) ? v.x : (b < 8) ? v.y : (b < 12) ? v.z : v.w;
@@ -105,21 +116,13 @@ __device__ __forceinline__ float term32(int32_t n, int32_t ni, int32_t nj, int32
 
 // Counting sinks.  kSinkShared: one LDS histogram per wave (or shared by several
 // waves), LDS atomics.  kSinkGlobal: the test's global table, device atomics (tables
-// too large for LDS).  kSinkLane: a private histogram per lane, two 16-bit counters
-// per 32-bit LDS word, word-major with the lane index fastest, so the 64 lanes of a
-// wave always hit 64 distinct words in distinct banks: no conflicts and no atomic
-// collisions however skewed the data (tables of at most kLaneCells cells).
+// too large for LDS).  The per-lane sinks (kSinkLane, kSinkLaneW) have their own row
+// loop, count_rows_lane below.
 
 template <int kSink>
 __device__ __forceinline__ void sink(int32_t *h, uint32_t key)
 {
-    if (kSink == kSinkLane) {
-        // h = this lane's column of the wave's [word][lane] block, two 16-bit counters a word
-        uint32_t *p = reinterpret_cast<uint32_t *>(h) + mad24(key >> 1, 64u, 0u);
-        atomicAdd(p, 1u << ((key & 1u) << 4));
-    } else {
-        atomicAdd(&h[key], 1);
-    }
+    atomicAdd(&h[key], 1);
 }
 
 template <int kSink>
@@ -150,9 +153,9 @@ __device__ __forceinline__ void load_cols(const uint8_t *const *zp, const uint8_
                                           const uint8_t *coly, int64_t r, Cols<D> &c)
 {
 #pragma unroll
-    for (int j = 0; j < D; ++j) c.v[j] = *reinterpret_cast<const uint4 *>(zp[j] + r);
-    c.v[(D > 0 ? D : 1)] = *reinterpret_cast<const uint4 *>(colx + r);
-    c.v[(D > 0 ? D : 1) + 1] = *reinterpret_cast<const uint4 *>(coly + r);
+    for (int j = 0; j < D; ++j) c.v[j] = ld16(zp[j] + r);
+    c.v[(D > 0 ? D : 1)] = ld16(colx + r);
+    c.v[(D > 0 ? D : 1) + 1] = ld16(coly + r);
 }
 
 // Rows [r_begin, r_end) of test T into the sink, |Z| = D known at compile time.
@@ -173,11 +176,9 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
     const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
     constexpr int X = (D > 0 ? D : 1);
     int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
-    Cols<D> cur, nxt;
-    if (r < r_end) load_cols<D>(zp, colx, coly, r, cur);
-    while (r < r_end) {
-        const int64_t rn = r + kRowsPerStep;
-        if (rn < r_end) load_cols<D>(zp, colx, coly, rn, nxt);
+    // two register buffers used in turn: the loads of the next step are in flight
+    // while the current one is counted, and no register copy waits for them
+    auto step = [&](const Cols<D> &cur, int64_t r) {
         uint32_t key[kRowsPerThread];
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
@@ -198,8 +199,19 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
             for (int b = 0; b < kRowsPerThread; ++b)
                 if (b < nv) put<kSink>(h, key[b], sh);
         }
-        cur = nxt;
-        r = rn;
+    };
+    Cols<D> b0, b1;
+    if (r < r_end) load_cols<D>(zp, colx, coly, r, b0);
+    while (r < r_end) {
+        // the loads are unconditional (clamped to a valid step): no branch around them
+        const int64_t r1 = r + kRowsPerStep;
+        load_cols<D>(zp, colx, coly, (r1 < r_end) ? r1 : r, b1);
+        step(b0, r);
+        if (r1 >= r_end) break;
+        const int64_t r2 = r1 + kRowsPerStep;
+        load_cols<D>(zp, colx, coly, (r2 < r_end) ? r2 : r1, b0);
+        step(b1, r1);
+        r = r2;
     }
 }
 
@@ -217,14 +229,14 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) key[b] = 0;
         for (int j = 0; j < d; ++j) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(I.cols + (int64_t)sh.zcol[j] * I.stride + r);
+            const uint4 v = ld16(I.cols + (int64_t)sh.zcol[j] * I.stride + r);
             const uint32_t c = (uint32_t)sh.zcard[j];
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
                 key[b] = (kSink == kSinkGlobal) ? key[b] * c + byte_of(v, b) : mad24(key[b], c, byte_of(v, b));
         }
-        const uint4 vx = *reinterpret_cast<const uint4 *>(colx + r);
-        const uint4 vy = *reinterpret_cast<const uint4 *>(coly + r);
+        const uint4 vx = ld16(colx + r);
+        const uint4 vy = ld16(coly + r);
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
             if (kSink == kSinkGlobal) key[b] = (key[b] * cx + byte_of(vx, b)) * cy + byte_of(vy, b);
@@ -239,6 +251,147 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
             for (int b = 0; b < kRowsPerThread; ++b)
                 if (b < nv) put<kSink>(h, key[b], sh);
         }
+    }
+}
+
+// Per-lane sinks (kSinkLane: tables <= 64 cells, kSinkLaneW: <= 128): a private
+// histogram per lane in 16-bit LDS counters.  The two waves of a wave pair share each
+// 32-bit word, the even wave counting in the low half and the odd wave in the high half,
+// so a wave's increment is a constant (1 or 1 << 16).  Layout [pair][cell][lane]: the
+// 64 lanes of a wave always hit 64 distinct words in distinct banks, so neither bank
+// conflicts nor atomic collisions occur however skewed the data.
+//
+// Every key is below kCells <= 128, so the keys of 4 rows are formed at once in one
+// 32-bit word, one byte per row: the Horner steps run as packed 16-bit multiply-adds
+// (v_pk_mad_u16) on the raw column words, and no byte ever carries into the next
+// because every partial key stays below 256.  A row's LDS byte offset is then
+// (k << 8) | base, taken from the packed word with a shift and a mask (base holds
+// the pair and lane bits, which do not overlap).  Rows past the end of the data only
+// ever carry into later rows of the same word, which are past the end as well and
+// are not counted.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_mad(uint32_t a, uint32_t b, uint32_t c)
+{
+    const u16x2 r = __builtin_bit_cast(u16x2, a) * __builtin_bit_cast(u16x2, b) + __builtin_bit_cast(u16x2, c);
+    return __builtin_bit_cast(uint32_t, r);
+}
+
+__device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
+{
+    return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+}
+
+// Counts the (up to) 4 rows whose keys are the bytes of K; rows >= nvalid are skipped.
+template <int kCells>
+__device__ __forceinline__ void lane_put4(char *lb, uint32_t base, uint32_t inc, uint32_t K, int nvalid)
+{
+    constexpr uint32_t kM = (uint32_t)(kCells - 1) << 8;
+    const uint32_t off[4] = {((K << 8) & kM) | base, (K & kM) | base, ((K >> 8) & kM) | base,
+                             ((K >> 16) & kM) | base};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < nvalid) atomicAdd(reinterpret_cast<uint32_t *>(lb + off[j]), inc);
+}
+
+// Packed keys of the 4 rows of word q, |Z| = D known at compile time.
+template <int D>
+__device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *zc2, uint32_t cx2, uint32_t cy2,
+                                              int q)
+{
+    constexpr int X = (D > 0 ? D : 1);
+    uint32_t K = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) K = (j == 0) ? word_of(c.v[0], q) : pk_mad(K, zc2[j], word_of(c.v[j], q));
+    K = (D == 0) ? word_of(c.v[X], q) : pk_mad(K, cx2, word_of(c.v[X], q));
+    return pk_mad(K, cy2, word_of(c.v[X + 1], q));
+}
+
+template <int D, int kCells>
+__device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                  int64_t r_begin, int64_t r_end, char *lb, uint32_t base,
+                                                  uint32_t inc)
+{
+    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
+    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
+    const uint8_t *zp[D > 0 ? D : 1];
+    uint32_t zc2[D > 0 ? D : 1];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        zp[j] = I.cols + (int64_t)sh.zcol[j] * I.stride;
+        zc2[j] = (uint32_t)sh.zcard[j] * 0x10001u;
+    }
+    const uint32_t cx2 = (uint32_t)T.cx * 0x10001u, cy2 = (uint32_t)T.cy * 0x10001u;
+    int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
+    auto step = [&](const Cols<D> &cur, int64_t r) {
+        const int64_t nv = r_end - r;
+        if (nv >= kRowsPerThread) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lane_put4<kCells>(lb, base, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), 4);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                lane_put4<kCells>(lb, base, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), (int)nv - 4 * q);
+        }
+    };
+    Cols<D> b0, b1;
+    if (r < r_end) load_cols<D>(zp, colx, coly, r, b0);
+    while (r < r_end) {
+        const int64_t r1 = r + kRowsPerStep;
+        load_cols<D>(zp, colx, coly, (r1 < r_end) ? r1 : r, b1);
+        step(b0, r);
+        if (r1 >= r_end) break;
+        const int64_t r2 = r1 + kRowsPerStep;
+        load_cols<D>(zp, colx, coly, (r2 < r_end) ? r2 : r1, b0);
+        step(b1, r1);
+        r = r2;
+    }
+}
+
+// Runtime |Z| (d > 3).
+template <int kCells>
+__device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                    int64_t r_begin, int64_t r_end, char *lb, uint32_t base,
+                                                    uint32_t inc)
+{
+    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
+    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
+    const uint32_t cx2 = (uint32_t)T.cx * 0x10001u, cy2 = (uint32_t)T.cy * 0x10001u;
+    const int d = T.d;
+    for (int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread; r < r_end; r += kRowsPerStep) {
+        uint32_t K[4] = {0u, 0u, 0u, 0u};
+        for (int j = 0; j < d; ++j) {
+            const uint4 v = ld16(I.cols + (int64_t)sh.zcol[j] * I.stride + r);
+            const uint32_t c2 = (uint32_t)sh.zcard[j] * 0x10001u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) K[q] = pk_mad(K[q], c2, word_of(v, q));
+        }
+        const uint4 vx = ld16(colx + r);
+        const uint4 vy = ld16(coly + r);
+        const int64_t nv = r_end - r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q));
+            lane_put4<kCells>(lb, base, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
+        }
+    }
+}
+
+// blk: the workgroup's counters, [pair][cell][lane] 32-bit words, kCells cells a pair.
+template <int kCells>
+__device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                int64_t r_begin, int64_t r_end, uint32_t *blk)
+{
+    char *lb = reinterpret_cast<char *>(blk);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t base = ((wave >> 1) * kCells * 64 + lane) * 4;
+    const uint32_t inc = (wave & 1) ? 0x10000u : 1u;
+    switch (T.d) {
+    case 0: count_rows_lane_d<0, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
+    case 1: count_rows_lane_d<1, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
+    case 2: count_rows_lane_d<2, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
+    case 3: count_rows_lane_d<3, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
+    default: count_rows_lane_any<kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
     }
 }
 
@@ -514,36 +667,28 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
         const int32_t *tab = nullptr;  // the chunk's table in LDS, when there is one
         if (kMode == kSinkLane || kMode == kSinkLaneW) {
-            // per-lane 16-bit counters: [wave][word][lane], words = ceil(cells / 2)
-            constexpr int kW = (kMode == kSinkLaneW) ? 2 * kLaneWords : kLaneWords;
+            // per-lane 16-bit counters: [pair][cell][lane], even wave low half, odd wave high
+            constexpr int kCells = (kMode == kSinkLaneW) ? kLaneWideCells : kLaneCells;
             const int c32 = (int)cells;
-            const int words = (c32 + 1) >> 1;
             uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
-            for (int i = tid; i < 4 * kW * 64; i += kWG) pl[i] = 0;
+            for (int i = tid; i < 2 * kCells * 64; i += kWG) pl[i] = 0;
             __syncthreads();
-            count_rows<kSinkLane>(I, T, sh, r_begin, r_end, reinterpret_cast<int32_t *>(pl + wave * kW * 64 + lane));
+            count_rows_lane<kCells>(I, T, sh, r_begin, r_end, pl);
             __syncthreads();
-            // sum the 256 private counters of every cell: 8 threads per word, each over
-            // 8 lanes of all 4 waves, then a 3-step shuffle reduction
-            for (int idx = tid; idx < words * 8; idx += kWG) {
-                const int w = idx >> 3, part = idx & 7;
-                uint32_t lo16 = 0, hi16 = 0;
+            // sum the 256 private counters of every cell: 8 threads per cell, each over
+            // 8 lanes of both pairs (both halves), then a 3-step shuffle reduction
+            for (int idx = tid; idx < c32 * 8; idx += kWG) {
+                const int k = idx >> 3, part = idx & 7;
+                uint32_t sum = 0;
 #pragma unroll
-                for (int wv = 0; wv < 4; ++wv)
+                for (int pr = 0; pr < 2; ++pr)
 #pragma unroll
                     for (int l = 0; l < 8; ++l) {
-                        const uint32_t v = pl[(wv * kW + w) * 64 + part * 8 + l];
-                        lo16 += v & 0xffffu;
-                        hi16 += v >> 16;
+                        const uint32_t v = pl[(pr * kCells + k) * 64 + part * 8 + l];
+                        sum += (v & 0xffffu) + (v >> 16);
                     }
-                for (int o = 1; o < 8; o <<= 1) {
-                    lo16 += __shfl_xor(lo16, o, 64);
-                    hi16 += __shfl_xor(hi16, o, 64);
-                }
-                if (part == 0) {
-                    small[2 * w] = (int32_t)lo16;
-                    if (2 * w + 1 < c32) small[2 * w + 1] = (int32_t)hi16;
-                }
+                for (int o = 1; o < 8; o <<= 1) sum += __shfl_xor(sum, o, 64);
+                if (part == 0) small[k] = (int32_t)sum;
             }
             __syncthreads();
             tab = small;
@@ -624,7 +769,7 @@ __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64
     uint32_t flag = 0;
     for (int64_t r = ((int64_t)blockIdx.x * kWG + threadIdx.x) * 16; r < n_rows;
          r += (int64_t)gridDim.x * kWG * 16) {
-        const uint4 q = *reinterpret_cast<const uint4 *>(col + r);
+        const uint4 q = ld16(col + r);
         const int64_t nv = n_rows - r;
 #pragma unroll
         for (int b = 0; b < 16; ++b)
