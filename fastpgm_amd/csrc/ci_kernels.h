@@ -45,6 +45,26 @@ constexpr int32_t kTFLdsMarg = 2;    // single chunk and table+marginals fit in 
 constexpr int32_t kTFExact = 4;      // FP64 statistic and p-value regardless of the band
 constexpr int32_t kTFKeep = 8;       // leave the counted table in the workspace (parity reads)
 constexpr int32_t kTFLane = 16;      // per-lane 16-bit LDS counters (cells <= kLaneCells)
+constexpr int32_t kTFWide = 32;      // statistic by the grid-wide finalize kernels, not the counting kernel
+
+// Grid-wide finalize of large device-memory tables: one workgroup per range of z slices
+// writes a partial, then one thread per test sums its partials in order.
+constexpr int kWideZ = 256;          // z slices per finalize workgroup
+struct WideItem {
+    int32_t t;       // test index within the launch
+    int32_t z0, z1;  // z slices [z0, z1)
+    int32_t pad;
+};
+struct WidePart {
+    double g64;      // FP64 G^2 of the range
+    float s, c, as;  // FP32 compensated sum, compensation, sum of |terms|
+    int32_t df;
+};
+struct WideTest {
+    int32_t t;              // test index within the launch
+    int32_t part0, nparts;  // its partials
+    int32_t pad;
+};
 
 // Result of one test on the device (32 bytes).
 struct DevResult {
@@ -85,6 +105,8 @@ constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 
 hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s);
+hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
+                                const WideTest *wt, int64_t n_wide, hipStream_t s);
 hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);

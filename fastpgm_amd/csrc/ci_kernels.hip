@@ -464,6 +464,47 @@ __device__ __forceinline__ void zslice32(const int32_t *tz, int cx, int cy, int3
     }
 }
 
+// FP32 decision of a test from its FP32 G^2 (IndependenceTest.cpp:140-154): df == 0 is
+// independent; otherwise independent iff G^2 < chi-square critical value, unless G^2 lies
+// within the recheck band of it (or the caller asked for exact results), in which case
+// the FP64 statistic decides.  Writes the result record; returns 1 when FP64 is needed.
+__device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, float AS, DevResult &R)
+{
+    int need64 = 0;
+    uint8_t indep = 0, in_band = 0;
+    const bool exact = (tflags & kTFExact) || (a.flags & FPGM_F_EXACT);
+    if (df == 0) {
+        indep = 1;  // IndependenceTest.cpp:140-142
+        need64 = exact;
+    } else {
+        const double crit = crit_of(a, df);
+        // FP32 error bound: a few ulp of every term, summed, plus the critical-value
+        // approximation beyond the table
+        const double bound = 4e-6 * (double)AS + (df > kCritTab ? 1e-5 * crit : 0.0) + 1e-30;
+        const double width = fmax((double)a.band * crit, bound);
+        // alpha outside (0, 1) has no critical value: the FP64 p-value decides
+        in_band = (isnan(crit) || fabs((double)g2 - crit) <= width) ? 1 : 0;
+        if ((a.flags & FPGM_F_FP32ONLY) && !isnan(crit)) {
+            indep = (double)g2 < crit;
+            need64 = exact;
+        } else if (in_band || exact) {
+            need64 = 1;
+        } else {
+            indep = (double)g2 < crit;  // p > alpha  <=>  G^2 < critical value
+        }
+    }
+    R.g2_32 = g2;
+    R.abs_sum = AS;
+    R.df = df;
+    R.indep = indep;
+    R.in_band = in_band;
+    R.exact = 0;
+    R.need_p = 0;
+    R.g2_64 = NAN;
+    R.p64 = (df == 0) ? 1.0 : NAN;
+    return need64;
+}
+
 // Statistic of test t from its full table (LDS or global) with marginal scratch
 // marg = [ni: dimz*cx | nj: dimz*cy | nk: dimz].  Tables with many z slices are
 // processed one slice per thread in a single pass; small ones cell-parallel.
@@ -545,42 +586,8 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
             Cc += sh.red_f[3 * w + 1];
             AS += sh.red_f[3 * w + 2];
         }
-        const float g2 = S + Cc;
         sh.df = df;
-        int need64 = 0;
-        uint8_t indep = 0, in_band = 0;
-        const bool exact = (T.flags & kTFExact) || (a.flags & FPGM_F_EXACT);
-        if (df == 0) {
-            indep = 1;  // IndependenceTest.cpp:140-142
-            need64 = exact;
-        } else {
-            const double crit = crit_of(a, df);
-            // FP32 error bound: a few ulp of every term, summed, plus the critical-value
-            // approximation beyond the table
-            const double bound = 4e-6 * (double)AS + (df > kCritTab ? 1e-5 * crit : 0.0) + 1e-30;
-            const double width = fmax((double)a.band * crit, bound);
-            // alpha outside (0, 1) has no critical value: the FP64 p-value decides
-            in_band = (isnan(crit) || fabs((double)g2 - crit) <= width) ? 1 : 0;
-            if ((a.flags & FPGM_F_FP32ONLY) && !isnan(crit)) {
-                indep = (double)g2 < crit;
-                need64 = exact;
-            } else if (in_band || exact) {
-                need64 = 1;
-            } else {
-                indep = (double)g2 < crit;  // p > alpha  <=>  G^2 < critical value
-            }
-        }
-        sh.need64 = need64;
-        DevResult &R = a.res[t];
-        R.g2_32 = g2;
-        R.abs_sum = AS;
-        R.df = df;
-        R.indep = indep;
-        R.in_band = in_band;
-        R.exact = 0;
-        R.need_p = 0;
-        R.g2_64 = NAN;
-        R.p64 = (df == 0) ? 1.0 : NAN;
+        sh.need64 = decide32(a, T.flags, df, S + Cc, AS, a.res[t]);
     }
     __syncthreads();
     if (sh.need64) {
@@ -736,6 +743,10 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
                 if (v) atomicAdd(&g[i], v);
             }
         }
+        if (T.flags & kTFWide) {  // the grid-wide finalize kernels take over
+            __syncthreads();       // the next item rewrites the staged Z columns
+            continue;
+        }
         // retire the chunk; the workgroup retiring the last chunk owns the statistic
         __threadfence();
         __syncthreads();
@@ -757,6 +768,98 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
         }
         __syncthreads();
     }
+}
+
+// Grid-wide finalize, part 1: the z slices [z0, z1) of one test's device-memory table,
+// one slice per thread.  Marginals go to the test's scratch behind the table; FP32 and
+// FP64 terms (same order within a slice as IndependenceTest.cpp:112-137) and df are
+// reduced to one partial per workgroup; the slices read are cleared, so the workspace
+// reads zero again for the next launch.
+__global__ __launch_bounds__(kWG) void k_fin_wide_part(LaunchArgs a, const WideItem *items, WidePart *parts)
+{
+    __shared__ Shared sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const WideItem it = items[blockIdx.x];
+    const DevTest T = a.tests[it.t];
+    const int cx = T.cx, cy = T.cy, cxy = cx * cy;
+    int32_t *tab = a.ws + T.ws_off;
+    int32_t *ni = tab + (int64_t)T.dimz * cxy;
+    int32_t *nj = ni + (int64_t)T.dimz * cx;
+    int df = 0;
+    float s = 0.f, c = 0.f, as = 0.f;
+    double g = 0.0;
+    for (int32_t z = it.z0 + tid; z < it.z1; z += kWG) {
+        int32_t *tz = tab + (int64_t)z * cxy;
+        int32_t *zi = ni + (int64_t)z * cx, *zj = nj + (int64_t)z * cy;
+        zslice32(tz, cx, cy, zi, zj, df, s, c, as);
+        int32_t nkz = 0;
+        for (int i = 0; i < cx; ++i) nkz += zi[i];
+        if (nkz != 0) {
+            for (int i = 0; i < cx; ++i) {
+                if (zi[i] == 0) continue;
+                for (int j = 0; j < cy; ++j) {
+                    const int32_t n = tz[i * cy + j];
+                    if (n) g += term64(n, zi[i], zj[j], nkz);
+                }
+            }
+        }
+        for (int k = 0; k < cxy; ++k) tz[k] = 0;
+        for (int i = 0; i < cx; ++i) zi[i] = 0;
+        for (int j = 0; j < cy; ++j) zj[j] = 0;
+    }
+    df = wave_sum_i(df);
+    wave_nsum(s, c);
+    as = wave_sum_f(as);
+    g = wave_sum_d(g);
+    if (lane == 0) {
+        sh.red_i[wave] = df;
+        sh.red_f[3 * wave] = s;
+        sh.red_f[3 * wave + 1] = c;
+        sh.red_f[3 * wave + 2] = as;
+        sh.red_d[wave] = g;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        WidePart P{0.0, 0.f, 0.f, 0.f, 0};
+        for (int w = 0; w < kWG / 64; ++w) {
+            P.df += sh.red_i[w];
+            nadd(P.s, P.c, sh.red_f[3 * w]);
+            P.c += sh.red_f[3 * w + 1];
+            P.as += sh.red_f[3 * w + 2];
+            P.g64 += sh.red_d[w];
+        }
+        parts[blockIdx.x] = P;
+    }
+}
+
+// Grid-wide finalize, part 2: one thread per test sums its partials in order and decides
+// like the counting kernels' finalize; the FP64 G^2 is always available here.
+__global__ __launch_bounds__(kWG) void k_fin_wide_reduce(LaunchArgs a, const WidePart *parts, const WideTest *wt,
+                                                         int64_t n_wide)
+{
+    const int64_t k = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (k >= n_wide) return;
+    const WideTest W = wt[k];
+    const DevTest T = a.tests[W.t];
+    int df = 0;
+    float S = 0.f, C = 0.f, AS = 0.f;
+    double G = 0.0;
+    for (int i = 0; i < W.nparts; ++i) {
+        const WidePart P = parts[W.part0 + i];
+        df += P.df;
+        nadd(S, C, P.s);
+        C += P.c;
+        AS += P.as;
+        G += P.g64;
+    }
+    DevResult R;
+    const int need64 = decide32(a, T.flags, df, S + C, AS, R);
+    if (need64) {
+        R.g2_64 = G;
+        R.exact = 1;
+        R.need_p = (df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
+    }
+    a.res[W.t] = R;
 }
 
 __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows,
@@ -804,6 +907,18 @@ hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_
     case kSinkShared: hipLaunchKernelGGL(k_count_stat<kSinkShared>, dim3(grid), dim3(kWG), 0, s, a); break;
     default: hipLaunchKernelGGL(k_count_stat<kSinkGlobal>, dim3(grid), dim3(kWG), 0, s, a); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
+                                const WideTest *wt, int64_t n_wide, hipStream_t s)
+{
+    if (n_items <= 0 || n_wide <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fin_wide_part, dim3((unsigned)n_items), dim3(kWG), 0, s, a, items, parts);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fin_wide_reduce, dim3((unsigned)((n_wide + kWG - 1) / kWG)), dim3(kWG), 0, s, a, parts, wt,
+                       n_wide);
     return hipGetLastError();
 }
 

@@ -123,8 +123,13 @@ struct fpgm_ctx {
     DevBuf<int32_t> d_zpool, d_item_test, d_ws;
     DevBuf<uint32_t> d_done, d_flag;
     DevBuf<DevResult> d_res;
+    DevBuf<WideItem> d_wide_items;
+    DevBuf<WidePart> d_wide_parts;
+    DevBuf<WideTest> d_wide_tests;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_grp[6] = {};  // per-sink launch timing when tracing
+    int trace = 0;               // > 0: one line per counting launch on stderr
     double k_ms = 0.0, k_bytes = 0.0;
     int64_t k_launches = 0;
 
@@ -211,6 +216,8 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
 
     std::vector<DevTest> dt;
     std::vector<int32_t> item_test, mode_of;
+    std::vector<WideItem> wide_items;
+    std::vector<WideTest> wide_tests;
     int64_t i0 = 0;
     while (i0 < n) {
         dt.clear();
@@ -258,6 +265,9 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             } else {
                 mode = kSinkGlobal;
                 fits = false;
+                // large device-memory tables get their statistic from the grid-wide
+                // finalize kernels (the counted table of a parity read stays in place)
+                if (!(tflags & kTFKeep)) T.flags |= kTFWide;
                 const int64_t cxy = (int64_t)T.cx * T.cy;
                 if (cxy <= kSliceCells && !(ctx->sink_mask & 2)) {
                     const int64_t sc = (kSliceCells / cxy) * cxy;
@@ -266,6 +276,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
                         mode = kSinkSlice;
                         T.n_slices = (int32_t)ns;
                         T.slice_cells = (int32_t)sc;
+                        T.flags &= ~kTFWide;
                     }
                 }
             }
@@ -304,6 +315,28 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             }
             grp_len[gi] = (int64_t)item_test.size() - grp_begin[gi];
         }
+        wide_items.clear();
+        wide_tests.clear();
+        for (int64_t k = 0; k < nb; ++k) {
+            if (!(dt[k].flags & kTFWide)) continue;
+            WideTest W{(int32_t)k, (int32_t)wide_items.size(), 0, 0};
+            for (int64_t z0 = 0; z0 < dt[k].dimz; z0 += kWideZ)
+                wide_items.push_back(
+                    WideItem{(int32_t)k, (int32_t)z0, (int32_t)std::min<int64_t>(z0 + kWideZ, dt[k].dimz), 0});
+            W.nparts = (int32_t)wide_items.size() - W.part0;
+            wide_tests.push_back(W);
+        }
+        if (!wide_items.empty()) {
+            HIP_TRY(ctx->d_wide_items.ensure((int64_t)wide_items.size(), false, ctx->stream), "alloc finalize items");
+            HIP_TRY(ctx->d_wide_parts.ensure((int64_t)wide_items.size(), false, ctx->stream), "alloc finalize parts");
+            HIP_TRY(ctx->d_wide_tests.ensure((int64_t)wide_tests.size(), false, ctx->stream), "alloc finalize tests");
+            HIP_TRY(hipMemcpyAsync(ctx->d_wide_items.p, wide_items.data(), sizeof(WideItem) * wide_items.size(),
+                                   hipMemcpyHostToDevice, ctx->stream),
+                    "upload finalize items");
+            HIP_TRY(hipMemcpyAsync(ctx->d_wide_tests.p, wide_tests.data(), sizeof(WideTest) * wide_tests.size(),
+                                   hipMemcpyHostToDevice, ctx->stream),
+                    "upload finalize tests");
+        }
         HIP_TRY(ctx->d_tests.ensure(nb, false, ctx->stream), "alloc tests");
         HIP_TRY(ctx->d_item_test.ensure((int64_t)item_test.size(), false, ctx->stream), "alloc items");
         HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
@@ -331,6 +364,7 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         a.res = ctx->d_res.p;
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
         for (int gi = 0; gi < kGroups; ++gi) {
+            if (ctx->trace) HIP_TRY(hipEventRecord(ctx->ev_grp[gi], ctx->stream), "event");
             if (grp_len[gi] == 0) continue;
             a.item_begin = grp_begin[gi];
             a.n_items = grp_len[gi];
@@ -339,7 +373,14 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
             grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
             HIP_TRY(launch_count_stat(a, order[gi], (int)grid, ctx->stream), "launch count_stat");
             ctx->k_launches += 1;
+            if (order[gi] == kSinkGlobal && !wide_items.empty()) {
+                HIP_TRY(launch_finalize_wide(a, ctx->d_wide_items.p, (int64_t)wide_items.size(), ctx->d_wide_parts.p,
+                                             ctx->d_wide_tests.p, (int64_t)wide_tests.size(), ctx->stream),
+                        "launch finalize");
+                ctx->k_launches += 2;
+            }
         }
+        if (ctx->trace) HIP_TRY(hipEventRecord(ctx->ev_grp[kGroups], ctx->stream), "event");
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
         HIP_TRY(launch_pvalue(ctx->d_res.p, nb, alpha, ctx->stream), "launch pvalue");
         HIP_TRY(hipMemcpyAsync(out.data() + i0, ctx->d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost,
@@ -351,6 +392,30 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
         ctx->k_ms += ms;
         for (int64_t k = i0; k < i1; ++k)
             ctx->k_bytes += (double)(tests[k].d + 2) * (double)ctx->inst[tests[k].inst].n_rows;
+        if (ctx->trace) {
+            static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
+            for (int gi = 0; gi < kGroups; ++gi) {
+                if (grp_len[gi] == 0) continue;
+                int64_t nt = 0, dsum = 0;
+                double csum = 0.0, cmax = 0.0, bytes = 0.0;
+                for (int64_t k = 0; k < nb; ++k) {
+                    if (mode_of[k] != order[gi]) continue;
+                    const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
+                    ++nt;
+                    dsum += dt[k].d;
+                    csum += c;
+                    cmax = std::max(cmax, c);
+                    bytes += (double)(dt[k].d + 2) * (double)ctx->inst[dt[k].inst].n_rows;
+                }
+                float gms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&gms, ctx->ev_grp[gi], ctx->ev_grp[gi + 1]), "event time");
+                fprintf(stderr,
+                        "[fpgm]   launch %-9s tests %7lld items %8lld mean |Z| %.2f cells mean %.0f max %.0f: "
+                        "%8.3f ms, %7.0f GB/s\n",
+                        names[gi], (long long)nt, (long long)grp_len[gi], (double)dsum / nt, csum / nt, cmax, gms,
+                        gms > 0 ? bytes / (gms * 1e6) : 0.0);
+            }
+        }
         if (keep_ws) *keep_ws = dt[0].ws_off;
         i0 = i1;
     }
@@ -640,6 +705,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ctx->ev1), "hipEventCreate");
+    for (hipEvent_t &e : ctx->ev_grp) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
     ctx->n_vars = n_vars;
     ctx->card.assign(card, card + n_vars);
     HIP_TRY(hipMalloc((void **)&ctx->d_card, sizeof(int32_t) * std::max(1, n_vars)), "hipMalloc card");
@@ -780,6 +846,8 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     ctx->d_pairs.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->ev_grp)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -889,6 +957,11 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
     const int32_t n_inst = (int32_t)P.size();
     fpgm_status st = ensure_crit(ctx, alpha);
     if (st != FPGM_OK) return st;
+    struct TraceScope {  // launch tracing for this stage only
+        fpgm_ctx *c;
+        ~TraceScope() { c->trace = 0; }
+    } trace_scope{ctx};
+    ctx->trace = (o.verbose >= 2) ? 1 : 0;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
     double avg_rows = 0.0;
     for (const InstHost &I : ctx->inst) avg_rows += (double)I.n_rows / (double)ctx->inst.size();
