@@ -14,6 +14,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfpgm_ci.so")
 GEN_LIB = os.path.join(HERE, "libfpgm_gen.so")
+HOST_LIB = os.path.join(HERE, "libfpgm_host.so")
+HOST_CLI = os.path.join(HERE, "fpgm_pcstable")
+HOST_SOURCES = ["dataset.cpp", "network.cpp", "pc_stable.cpp"]
 
 SOURCES = ["ci_kernels.hip", "gram_kernels.hip", "fpgm_ci.hip"]
 HEADERS = ["ci_kernels.h", "gram_kernels.h", "chisq.h"]
@@ -71,5 +74,30 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_host(verbose: bool = False) -> str:
+    """C++ host mirror (host/: Dataset, Network, PCStable) over libfpgm_ci.so, and its
+    command-line driver.  Host code only: g++, no device code."""
+    cxx = os.environ.get("CXX", "g++")
+    hdir = os.path.join(HERE, "host")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", hdir]
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
+    link = ["-L", HERE, "-Wl,-rpath,$ORIGIN", "-lfpgm_ci"]
+    tmp = HOST_LIB + ".tmp"
+    cmd = [cxx, *flags, *inc, "-shared", "-o", tmp, *[os.path.join(hdir, f) for f in HOST_SOURCES], *link]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOST_LIB)
+    tmp = HOST_CLI + ".tmp"
+    cmd = [cxx, *flags, *inc, "-o", tmp, os.path.join(hdir, "pcstable_cli.cpp"), "-L", HERE,
+           "-Wl,-rpath,$ORIGIN", "-lfpgm_host", "-lfpgm_ci"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOST_CLI)
+    return HOST_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_host(verbose="-v" in sys.argv))

@@ -80,6 +80,100 @@ struct DevBuf {
     }
 };
 
+// Pinned host staging buffer (asynchronous copies to and from the device).
+template <typename T>
+struct HostBuf {
+    T *p = nullptr;
+    int64_t cap = 0;
+    hipError_t ensure(int64_t n, hipStream_t s)
+    {
+        if (n <= cap) return hipSuccess;
+        if (p) {
+            (void)hipStreamSynchronize(s);  // no copy may still read or write the old buffer
+            (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        int64_t want = std::max<int64_t>(n, 1024);
+        want = want + want / 2;
+        hipError_t e = hipHostMalloc((void **)&p, (size_t)want * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Counting launches are grouped by sink kind, in this order.
+constexpr int kGroups = 5;
+constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
+
+// Buffers of one in-flight batch.  Two slots let the host build one batch while the
+// device runs the other; the workspace and completion counters are shared, since both
+// slots launch on the context's one stream and every launch leaves them zero.
+struct Slot {
+    DevBuf<DevTest> d_tests;
+    DevBuf<int32_t> d_zpool, d_item_test;
+    DevBuf<DevResult> d_res;
+    DevBuf<WideItem> d_wide_items;
+    DevBuf<WidePart> d_wide_parts;
+    DevBuf<WideTest> d_wide_tests;
+    HostBuf<DevTest> h_tests;
+    HostBuf<int32_t> h_items, h_zpool;
+    HostBuf<DevResult> h_res;
+    HostBuf<WideItem> h_wide_items;
+    HostBuf<WideTest> h_wide_tests;
+    std::vector<DevTest> dt;
+    std::vector<int32_t> mode_of;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing
+    bool pending = false;
+    int64_t n = 0;       // tests of the pending launch sequence
+    double bytes = 0.0;  // their algorithmic bytes
+    struct GroupInfo {
+        int64_t tests = 0, items = 0, dsum = 0;
+        double csum = 0.0, cmax = 0.0, bytes = 0.0;
+    } grp[kGroups];
+
+    hipError_t init()
+    {
+        hipError_t e = hipSuccess;
+        for (hipEvent_t *ev : {&ev0, &ev1, &ev_done})
+            if ((e = hipEventCreate(ev)) != hipSuccess) return e;
+        for (hipEvent_t &ev : ev_grp)
+            if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+        return e;
+    }
+    void release()
+    {
+        d_tests.release();
+        d_zpool.release();
+        d_item_test.release();
+        d_res.release();
+        d_wide_items.release();
+        d_wide_parts.release();
+        d_wide_tests.release();
+        h_tests.release();
+        h_items.release();
+        h_zpool.release();
+        h_res.release();
+        h_wide_items.release();
+        h_wide_tests.release();
+        for (hipEvent_t ev : {ev0, ev1, ev_done})
+            if (ev) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : ev_grp)
+            if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
 struct HostTest {
     int32_t x, y, d;
     int32_t inst;   // dataset instance
@@ -110,6 +204,7 @@ struct fpgm_ctx {
         double base = 2.0;    // round ramp factor
         int64_t cap = 8;      // ramp cap (tests per edge per round); larger rounds come from the budget
         int64_t promote = 0;  // dependent tests after which an edge runs the rest of its walk
+        int lanes = 1;        // edge lanes whose rounds alternate (host work overlaps device work)
     } sched;
     int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
     uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices
@@ -119,17 +214,13 @@ struct fpgm_ctx {
     std::vector<double> crit_host;
     double *d_crit = nullptr;
 
-    DevBuf<DevTest> d_tests;
-    DevBuf<int32_t> d_zpool, d_item_test, d_ws;
+    Slot slot[2];
+    DevBuf<int32_t> d_ws;
     DevBuf<uint32_t> d_done, d_flag;
-    DevBuf<DevResult> d_res;
-    DevBuf<WideItem> d_wide_items;
-    DevBuf<WidePart> d_wide_parts;
-    DevBuf<WideTest> d_wide_tests;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t ev_grp[6] = {};  // per-sink launch timing when tracing
     int trace = 0;               // > 0: one line per counting launch on stderr
+    double t_prep = 0.0, t_wait = 0.0;  // host seconds building launches / waiting for them (trace)
     double k_ms = 0.0, k_bytes = 0.0;
     int64_t k_launches = 0;
 
@@ -182,19 +273,24 @@ fpgm_status ensure_crit(fpgm_ctx *ctx, double alpha)
     return FPGM_OK;
 }
 
-// Runs tests[0..n) on the device; results land in out (host).  flags: FPGM_F_*,
-// tflags: extra kTF* bits for every test.  keep_ws receives the workspace offset
-// of test 0 when kTFKeep is set.
-fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int32_t *zpool, int64_t zlen,
-                      double alpha, uint32_t flags, float band, int32_t tflags, std::vector<DevResult> &out,
-                      int64_t rows_per_chunk_forced = 0, int64_t *keep_ws = nullptr)
+fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S);
+
+// Builds and enqueues one launch sequence for tests [i0, i1) of a call, as many as fit
+// the workspace limit (*i1_out receives i1); nothing waits here.  The slot's pinned
+// buffers stage the descriptors, and its results arrive in slot.h_res once
+// batch_wait returns.  flags: FPGM_F_*, tflags: extra kTF* bits for every test.
+// keep_ws receives the workspace offset of test i0 when kTFKeep is set.
+fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t n, const int32_t *zpool,
+                         int64_t zlen, double alpha, uint32_t flags, float band, int32_t tflags,
+                         int64_t rows_per_chunk_forced, int64_t i0, int64_t *i1_out, int64_t *keep_ws)
 {
-    out.resize((size_t)n);
-    if (n == 0) return FPGM_OK;
-    fpgm_status st = ensure_crit(ctx, alpha);
+    fpgm_status st = batch_wait(ctx, S);  // a batch left pending by an earlier error
+    if (st != FPGM_OK) return st;
+    const double tp0 = now_s();
+    st = ensure_crit(ctx, alpha);
     if (st != FPGM_OK) return st;
     int64_t N = 0;  // longest instance in the batch
-    for (int64_t i = 0; i < n; ++i) N = std::max(N, ctx->inst[tests[i].inst].n_rows);
+    for (int64_t i = i0; i < n; ++i) N = std::max(N, ctx->inst[tests[i].inst].n_rows);
     const int64_t steps = std::max<int64_t>(1, (N + kRowsPerStep - 1) / kRowsPerStep);
 
     // rows per chunk, common to the launch: enough work items to fill every CU
@@ -203,220 +299,268 @@ fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int
     if (rows_per_chunk_forced > 0) {
         rpc = (rows_per_chunk_forced + kRowsPerStep - 1) / kRowsPerStep * kRowsPerStep;
     } else {
-        int64_t cpt = std::max<int64_t>(1, (target_items + n - 1) / n);
+        int64_t cpt = std::max<int64_t>(1, (target_items + (n - i0) - 1) / (n - i0));
         cpt = std::min(cpt, steps);
         rpc = (steps + cpt - 1) / cpt * kRowsPerStep;
     }
     rpc = std::min<int64_t>(rpc, kLaneMaxRowsPerChunk);
 
-    HIP_TRY(ctx->d_zpool.ensure(std::max<int64_t>(zlen, 1), false, ctx->stream), "alloc z pool");
-    if (zlen > 0)
-        HIP_TRY(hipMemcpyAsync(ctx->d_zpool.p, zpool, sizeof(int32_t) * zlen, hipMemcpyHostToDevice, ctx->stream),
-                "upload z pool");
-
-    std::vector<DevTest> dt;
-    std::vector<int32_t> item_test, mode_of;
-    std::vector<WideItem> wide_items;
-    std::vector<WideTest> wide_tests;
-    int64_t i0 = 0;
-    while (i0 < n) {
-        dt.clear();
-        mode_of.clear();
-        int64_t ws_used = 0, n_items_total = 0;
-        int64_t i1 = i0;
-        for (; i1 < n; ++i1) {
-            const HostTest &h = tests[i1];
-            DevTest T{};
-            T.x = h.x;
-            T.y = h.y;
-            T.d = h.d;
-            T.inst = h.inst;
-            const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[h.inst].n_rows + rpc - 1) / rpc);
-            T.z_off = (int32_t)h.z_off;
-            T.cx = ctx->card[h.x];
-            T.cy = ctx->card[h.y];
-            int64_t dimz = 1;
-            for (int j = 0; j < h.d; ++j) {
-                dimz *= ctx->card[zpool[h.z_off + j]];
-                if (dimz > kMaxCells) break;
-            }
-            const int64_t cells = dimz * T.cx * T.cy;
-            if (dimz > kMaxCells || cells >= kMaxCells)
-                return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
-            T.dimz = (int32_t)dimz;
-            T.n_chunks = (int32_t)n_chunks;
-            T.n_slices = 1;
-            T.slice_cells = (int32_t)cells;
-            const int64_t marg = dimz * (T.cx + T.cy + 1);
-            T.flags = tflags;
-            bool fits;
-            int mode;
-            if (cells <= kLaneCells) {
-                mode = kSinkLane;
-                T.flags |= kTFLane;
-                fits = cells + marg <= kSmallCells;
-            } else if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
-                mode = kSinkLaneW;
-                fits = cells + marg <= kSmallCells;
-            } else if (cells <= kLdsCells) {
-                mode = kSinkShared;
-                T.flags |= kTFLdsTable;
-                fits = cells + marg <= kLdsCells;
-            } else {
-                mode = kSinkGlobal;
-                fits = false;
-                // large device-memory tables get their statistic from the grid-wide
-                // finalize kernels (the counted table of a parity read stays in place)
-                if (!(tflags & kTFKeep)) T.flags |= kTFWide;
-                const int64_t cxy = (int64_t)T.cx * T.cy;
-                if (cxy <= kSliceCells && !(ctx->sink_mask & 2)) {
-                    const int64_t sc = (kSliceCells / cxy) * cxy;
-                    const int64_t ns = (cells + sc - 1) / sc;
-                    if (ns <= ctx->max_slices) {
-                        mode = kSinkSlice;
-                        T.n_slices = (int32_t)ns;
-                        T.slice_cells = (int32_t)sc;
-                        T.flags &= ~kTFWide;
-                    }
+    std::vector<DevTest> &dt = S.dt;
+    std::vector<int32_t> &mode_of = S.mode_of;
+    dt.clear();
+    mode_of.clear();
+    int64_t ws_used = 0, n_items_total = 0;
+    int64_t i1 = i0;
+    for (; i1 < n; ++i1) {
+        const HostTest &h = tests[i1];
+        DevTest T{};
+        T.x = h.x;
+        T.y = h.y;
+        T.d = h.d;
+        T.inst = h.inst;
+        const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[h.inst].n_rows + rpc - 1) / rpc);
+        T.z_off = (int32_t)h.z_off;
+        T.cx = ctx->card[h.x];
+        T.cy = ctx->card[h.y];
+        int64_t dimz = 1;
+        for (int j = 0; j < h.d; ++j) {
+            dimz *= ctx->card[zpool[h.z_off + j]];
+            if (dimz > kMaxCells) break;
+        }
+        const int64_t cells = dimz * T.cx * T.cy;
+        if (dimz > kMaxCells || cells >= kMaxCells) return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
+        T.dimz = (int32_t)dimz;
+        T.n_chunks = (int32_t)n_chunks;
+        T.n_slices = 1;
+        T.slice_cells = (int32_t)cells;
+        const int64_t marg = dimz * (T.cx + T.cy + 1);
+        T.flags = tflags;
+        bool fits;
+        int mode;
+        if (cells <= kLaneCells) {
+            mode = kSinkLane;
+            T.flags |= kTFLane;
+            fits = cells + marg <= kSmallCells;
+        } else if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
+            mode = kSinkLaneW;
+            fits = cells + marg <= kSmallCells;
+        } else if (cells <= kLdsCells) {
+            mode = kSinkShared;
+            T.flags |= kTFLdsTable;
+            fits = cells + marg <= kLdsCells;
+        } else {
+            mode = kSinkGlobal;
+            fits = false;
+            // large device-memory tables get their statistic from the grid-wide
+            // finalize kernels (the counted table of a parity read stays in place)
+            if (!(tflags & kTFKeep)) T.flags |= kTFWide;
+            const int64_t cxy = (int64_t)T.cx * T.cy;
+            if (cxy <= kSliceCells && !(ctx->sink_mask & 2)) {
+                const int64_t sc = (kSliceCells / cxy) * cxy;
+                const int64_t ns = (cells + sc - 1) / sc;
+                if (ns <= ctx->max_slices) {
+                    mode = kSinkSlice;
+                    T.n_slices = (int32_t)ns;
+                    T.slice_cells = (int32_t)sc;
+                    T.flags &= ~kTFWide;
                 }
             }
-            const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fits;
-            if (!need_ws) T.flags |= kTFLdsMarg;
-            T.ws_off = -1;
-            if (need_ws) {
-                const int64_t need = (cells + marg + 63) / 64 * 64;
-                if (ws_used + need > kWsMax && i1 > i0) break;
-                if (need > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test workspace exceeds 4 GiB");
-                T.ws_off = ws_used;
-                ws_used += need;
-            }
-            dt.push_back(T);
-            mode_of.push_back(mode);
-            n_items_total += n_chunks * T.n_slices;
-            if (n_items_total > ((int64_t)1 << 30)) {
-                ++i1;
-                break;
-            }
         }
-        const int64_t nb = (int64_t)dt.size();
-        // work items grouped by sink: [lane | shared | global], one launch per group;
-        // within a group, tests keep the caller's order (neighbouring tests share columns)
-        constexpr int kGroups = 5;
-        int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
-        const int order[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
-        item_test.clear();
-        item_test.reserve((size_t)n_items_total);
-        for (int gi = 0; gi < kGroups; ++gi) {
-            grp_begin[gi] = (int64_t)item_test.size();
-            for (int64_t k = 0; k < nb; ++k) {
-                if (mode_of[k] != order[gi]) continue;
-                dt[k].item0 = (int32_t)item_test.size();
-                for (int64_t c = 0; c < (int64_t)dt[k].n_chunks * dt[k].n_slices; ++c) item_test.push_back((int32_t)k);
-            }
-            grp_len[gi] = (int64_t)item_test.size() - grp_begin[gi];
+        const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fits;
+        if (!need_ws) T.flags |= kTFLdsMarg;
+        T.ws_off = -1;
+        if (need_ws) {
+            const int64_t need = (cells + marg + 63) / 64 * 64;
+            if (ws_used + need > kWsMax && i1 > i0) break;
+            if (need > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test workspace exceeds 4 GiB");
+            T.ws_off = ws_used;
+            ws_used += need;
         }
-        wide_items.clear();
-        wide_tests.clear();
+        dt.push_back(T);
+        mode_of.push_back(mode);
+        n_items_total += n_chunks * T.n_slices;
+        if (n_items_total > ((int64_t)1 << 30)) {
+            ++i1;
+            break;
+        }
+    }
+    const int64_t nb = (int64_t)dt.size();
+    // work items grouped by sink, one launch per group; within a group, tests keep the
+    // caller's order (neighbouring tests share columns)
+    int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
+    HIP_TRY(S.h_items.ensure(n_items_total, ctx->stream), "alloc pinned items");
+    int64_t ni = 0;
+    for (int gi = 0; gi < kGroups; ++gi) {
+        grp_begin[gi] = ni;
+        for (int64_t k = 0; k < nb; ++k) {
+            if (mode_of[k] != kGroupSink[gi]) continue;
+            dt[k].item0 = (int32_t)ni;
+            for (int64_t c = 0; c < (int64_t)dt[k].n_chunks * dt[k].n_slices; ++c) S.h_items.p[ni++] = (int32_t)k;
+        }
+        grp_len[gi] = ni - grp_begin[gi];
+    }
+    // device-memory tables: z ranges for the grid-wide finalize
+    int64_t n_wi = 0, n_wt = 0;
+    for (int64_t k = 0; k < nb; ++k)
+        if (dt[k].flags & kTFWide) {
+            n_wi += (dt[k].dimz + kWideZ - 1) / kWideZ;
+            ++n_wt;
+        }
+    if (n_wt > 0) {
+        HIP_TRY(S.h_wide_items.ensure(n_wi, ctx->stream), "alloc pinned finalize items");
+        HIP_TRY(S.h_wide_tests.ensure(n_wt, ctx->stream), "alloc pinned finalize tests");
+        int64_t wi = 0, wt = 0;
         for (int64_t k = 0; k < nb; ++k) {
             if (!(dt[k].flags & kTFWide)) continue;
-            WideTest W{(int32_t)k, (int32_t)wide_items.size(), 0, 0};
+            WideTest W{(int32_t)k, (int32_t)wi, 0, 0};
             for (int64_t z0 = 0; z0 < dt[k].dimz; z0 += kWideZ)
-                wide_items.push_back(
-                    WideItem{(int32_t)k, (int32_t)z0, (int32_t)std::min<int64_t>(z0 + kWideZ, dt[k].dimz), 0});
-            W.nparts = (int32_t)wide_items.size() - W.part0;
-            wide_tests.push_back(W);
+                S.h_wide_items.p[wi++] =
+                    WideItem{(int32_t)k, (int32_t)z0, (int32_t)std::min<int64_t>(z0 + kWideZ, dt[k].dimz), 0};
+            W.nparts = (int32_t)wi - W.part0;
+            S.h_wide_tests.p[wt++] = W;
         }
-        if (!wide_items.empty()) {
-            HIP_TRY(ctx->d_wide_items.ensure((int64_t)wide_items.size(), false, ctx->stream), "alloc finalize items");
-            HIP_TRY(ctx->d_wide_parts.ensure((int64_t)wide_items.size(), false, ctx->stream), "alloc finalize parts");
-            HIP_TRY(ctx->d_wide_tests.ensure((int64_t)wide_tests.size(), false, ctx->stream), "alloc finalize tests");
-            HIP_TRY(hipMemcpyAsync(ctx->d_wide_items.p, wide_items.data(), sizeof(WideItem) * wide_items.size(),
-                                   hipMemcpyHostToDevice, ctx->stream),
-                    "upload finalize items");
-            HIP_TRY(hipMemcpyAsync(ctx->d_wide_tests.p, wide_tests.data(), sizeof(WideTest) * wide_tests.size(),
-                                   hipMemcpyHostToDevice, ctx->stream),
-                    "upload finalize tests");
-        }
-        HIP_TRY(ctx->d_tests.ensure(nb, false, ctx->stream), "alloc tests");
-        HIP_TRY(ctx->d_item_test.ensure((int64_t)item_test.size(), false, ctx->stream), "alloc items");
-        HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
-        HIP_TRY(ctx->d_done.ensure(nb, true, ctx->stream), "alloc counters");
-        HIP_TRY(ctx->d_res.ensure(nb, false, ctx->stream), "alloc results");
-        HIP_TRY(hipMemcpyAsync(ctx->d_tests.p, dt.data(), sizeof(DevTest) * nb, hipMemcpyHostToDevice, ctx->stream),
-                "upload tests");
-        HIP_TRY(hipMemcpyAsync(ctx->d_item_test.p, item_test.data(), sizeof(int32_t) * item_test.size(),
-                               hipMemcpyHostToDevice, ctx->stream),
-                "upload items");
-        LaunchArgs a{};
-        a.inst = ctx->d_inst;
-        a.tests = ctx->d_tests.p;
-        a.zpool = ctx->d_zpool.p;
-        a.card = ctx->d_card;
-        a.item_test = ctx->d_item_test.p;
-        a.rows_per_chunk = rpc;
-        a.ws = ctx->d_ws.p;
-        a.done = ctx->d_done.p;
-        a.crit = ctx->d_crit;
-        a.alpha = alpha;
-        a.z_alpha = ctx->z_alpha;
-        a.band = band > 0.f ? band : 0.02f;
-        a.flags = flags;
-        a.res = ctx->d_res.p;
-        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-        for (int gi = 0; gi < kGroups; ++gi) {
-            if (ctx->trace) HIP_TRY(hipEventRecord(ctx->ev_grp[gi], ctx->stream), "event");
-            if (grp_len[gi] == 0) continue;
-            a.item_begin = grp_begin[gi];
-            a.n_items = grp_len[gi];
-            const int64_t waves_per_cu = (order[gi] == kSinkGlobal) ? 8 : (order[gi] == kSinkLaneW || order[gi] == kSinkSlice) ? 2 : 4;
-            int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * waves_per_cu);
-            grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
-            HIP_TRY(launch_count_stat(a, order[gi], (int)grid, ctx->stream), "launch count_stat");
-            ctx->k_launches += 1;
-            if (order[gi] == kSinkGlobal && !wide_items.empty()) {
-                HIP_TRY(launch_finalize_wide(a, ctx->d_wide_items.p, (int64_t)wide_items.size(), ctx->d_wide_parts.p,
-                                             ctx->d_wide_tests.p, (int64_t)wide_tests.size(), ctx->stream),
-                        "launch finalize");
-                ctx->k_launches += 2;
-            }
-        }
-        if (ctx->trace) HIP_TRY(hipEventRecord(ctx->ev_grp[kGroups], ctx->stream), "event");
-        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
-        HIP_TRY(launch_pvalue(ctx->d_res.p, nb, alpha, ctx->stream), "launch pvalue");
-        HIP_TRY(hipMemcpyAsync(out.data() + i0, ctx->d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost,
+        HIP_TRY(S.d_wide_items.ensure(n_wi, false, ctx->stream), "alloc finalize items");
+        HIP_TRY(S.d_wide_parts.ensure(n_wi, false, ctx->stream), "alloc finalize parts");
+        HIP_TRY(S.d_wide_tests.ensure(n_wt, false, ctx->stream), "alloc finalize tests");
+        HIP_TRY(hipMemcpyAsync(S.d_wide_items.p, S.h_wide_items.p, sizeof(WideItem) * n_wi, hipMemcpyHostToDevice,
                                ctx->stream),
-                "download results");
-        HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
-        ctx->k_ms += ms;
-        for (int64_t k = i0; k < i1; ++k)
-            ctx->k_bytes += (double)(tests[k].d + 2) * (double)ctx->inst[tests[k].inst].n_rows;
-        if (ctx->trace) {
-            static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
-            for (int gi = 0; gi < kGroups; ++gi) {
-                if (grp_len[gi] == 0) continue;
-                int64_t nt = 0, dsum = 0;
-                double csum = 0.0, cmax = 0.0, bytes = 0.0;
-                for (int64_t k = 0; k < nb; ++k) {
-                    if (mode_of[k] != order[gi]) continue;
-                    const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
-                    ++nt;
-                    dsum += dt[k].d;
-                    csum += c;
-                    cmax = std::max(cmax, c);
-                    bytes += (double)(dt[k].d + 2) * (double)ctx->inst[dt[k].inst].n_rows;
-                }
-                float gms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&gms, ctx->ev_grp[gi], ctx->ev_grp[gi + 1]), "event time");
-                fprintf(stderr,
-                        "[fpgm]   launch %-9s tests %7lld items %8lld mean |Z| %.2f cells mean %.0f max %.0f: "
-                        "%8.3f ms, %7.0f GB/s\n",
-                        names[gi], (long long)nt, (long long)grp_len[gi], (double)dsum / nt, csum / nt, cmax, gms,
-                        gms > 0 ? bytes / (gms * 1e6) : 0.0);
-            }
+                "upload finalize items");
+        HIP_TRY(hipMemcpyAsync(S.d_wide_tests.p, S.h_wide_tests.p, sizeof(WideTest) * n_wt, hipMemcpyHostToDevice,
+                               ctx->stream),
+                "upload finalize tests");
+    }
+    HIP_TRY(S.h_tests.ensure(nb, ctx->stream), "alloc pinned tests");
+    std::copy(dt.begin(), dt.end(), S.h_tests.p);
+    HIP_TRY(S.h_zpool.ensure(std::max<int64_t>(zlen, 1), ctx->stream), "alloc pinned z pool");
+    if (zlen > 0) std::copy(zpool, zpool + zlen, S.h_zpool.p);
+    HIP_TRY(S.d_zpool.ensure(std::max<int64_t>(zlen, 1), false, ctx->stream), "alloc z pool");
+    HIP_TRY(S.d_tests.ensure(nb, false, ctx->stream), "alloc tests");
+    HIP_TRY(S.d_item_test.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
+    HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
+    HIP_TRY(ctx->d_done.ensure(std::max<int64_t>(nb, 1024), true, ctx->stream), "alloc counters");
+    HIP_TRY(S.d_res.ensure(nb, false, ctx->stream), "alloc results");
+    HIP_TRY(S.h_res.ensure(nb, ctx->stream), "alloc pinned results");
+    if (zlen > 0)
+        HIP_TRY(hipMemcpyAsync(S.d_zpool.p, S.h_zpool.p, sizeof(int32_t) * zlen, hipMemcpyHostToDevice, ctx->stream),
+                "upload z pool");
+    HIP_TRY(hipMemcpyAsync(S.d_tests.p, S.h_tests.p, sizeof(DevTest) * nb, hipMemcpyHostToDevice, ctx->stream),
+            "upload tests");
+    HIP_TRY(hipMemcpyAsync(S.d_item_test.p, S.h_items.p, sizeof(int32_t) * ni, hipMemcpyHostToDevice, ctx->stream),
+            "upload items");
+    LaunchArgs a{};
+    a.inst = ctx->d_inst;
+    a.tests = S.d_tests.p;
+    a.zpool = S.d_zpool.p;
+    a.card = ctx->d_card;
+    a.item_test = S.d_item_test.p;
+    a.rows_per_chunk = rpc;
+    a.ws = ctx->d_ws.p;
+    a.done = ctx->d_done.p;
+    a.crit = ctx->d_crit;
+    a.alpha = alpha;
+    a.z_alpha = ctx->z_alpha;
+    a.band = band > 0.f ? band : 0.02f;
+    a.flags = flags;
+    a.res = S.d_res.p;
+    ctx->t_prep += now_s() - tp0;
+    HIP_TRY(hipEventRecord(S.ev0, ctx->stream), "event");
+    for (int gi = 0; gi < kGroups; ++gi) {
+        if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[gi], ctx->stream), "event");
+        if (grp_len[gi] == 0) continue;
+        a.item_begin = grp_begin[gi];
+        a.n_items = grp_len[gi];
+        const int sink = kGroupSink[gi];
+        const int64_t waves_per_cu = (sink == kSinkGlobal) ? 8 : (sink == kSinkLaneW || sink == kSinkSlice) ? 2 : 4;
+        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * waves_per_cu);
+        grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+        HIP_TRY(launch_count_stat(a, sink, (int)grid, ctx->stream), "launch count_stat");
+        ctx->k_launches += 1;
+        if (sink == kSinkGlobal && n_wt > 0) {
+            HIP_TRY(launch_finalize_wide(a, S.d_wide_items.p, n_wi, S.d_wide_parts.p, S.d_wide_tests.p, n_wt,
+                                         ctx->stream),
+                    "launch finalize");
+            ctx->k_launches += 2;
         }
-        if (keep_ws) *keep_ws = dt[0].ws_off;
+    }
+    if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[kGroups], ctx->stream), "event");
+    HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
+    HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, ctx->stream), "launch pvalue");
+    HIP_TRY(hipMemcpyAsync(S.h_res.p, S.d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost, ctx->stream),
+            "download results");
+    HIP_TRY(hipEventRecord(S.ev_done, ctx->stream), "event");
+    S.n = nb;
+    S.bytes = 0.0;
+    for (int64_t k = i0; k < i1; ++k) S.bytes += (double)(tests[k].d + 2) * (double)ctx->inst[tests[k].inst].n_rows;
+    if (ctx->trace) {
+        for (int gi = 0; gi < kGroups; ++gi) {
+            Slot::GroupInfo &G = S.grp[gi];
+            G = Slot::GroupInfo{};
+            G.items = grp_len[gi];
+        }
+        for (int64_t k = 0; k < nb; ++k) {
+            int gi = 0;
+            while (kGroupSink[gi] != mode_of[k]) ++gi;
+            Slot::GroupInfo &G = S.grp[gi];
+            const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
+            G.tests += 1;
+            G.dsum += dt[k].d;
+            G.csum += c;
+            G.cmax = std::max(G.cmax, c);
+            G.bytes += (double)(dt[k].d + 2) * (double)ctx->inst[dt[k].inst].n_rows;
+        }
+    }
+    S.pending = true;
+    if (keep_ws) *keep_ws = dt[0].ws_off;
+    *i1_out = i1;
+    return FPGM_OK;
+}
+
+// Waits for the slot's launch sequence; its results are then in S.h_res[0, S.n).
+fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
+{
+    if (!S.pending) return FPGM_OK;
+    const double tw0 = now_s();
+    HIP_TRY(hipEventSynchronize(S.ev_done), "synchronize");
+    ctx->t_wait += now_s() - tw0;
+    S.pending = false;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, S.ev0, S.ev1), "event time");
+    ctx->k_ms += ms;
+    ctx->k_bytes += S.bytes;
+    if (ctx->trace) {
+        static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
+        for (int gi = 0; gi < kGroups; ++gi) {
+            const Slot::GroupInfo &G = S.grp[gi];
+            if (G.items == 0) continue;
+            float gms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&gms, S.ev_grp[gi], S.ev_grp[gi + 1]), "event time");
+            fprintf(stderr,
+                    "[fpgm]   launch %-9s tests %7lld items %8lld mean |Z| %.2f cells mean %.0f max %.0f: "
+                    "%8.3f ms, %7.0f GB/s\n",
+                    names[gi], (long long)G.tests, (long long)G.items, (double)G.dsum / G.tests, G.csum / G.tests,
+                    G.cmax, gms, gms > 0 ? G.bytes / (gms * 1e6) : 0.0);
+        }
+    }
+    return FPGM_OK;
+}
+
+// Runs tests[0..n) on the device and waits; results land in out (host).
+fpgm_status run_batch(fpgm_ctx *ctx, const HostTest *tests, int64_t n, const int32_t *zpool, int64_t zlen,
+                      double alpha, uint32_t flags, float band, int32_t tflags, std::vector<DevResult> &out,
+                      int64_t rows_per_chunk_forced = 0, int64_t *keep_ws = nullptr)
+{
+    out.resize((size_t)n);
+    Slot &S = ctx->slot[0];
+    fpgm_status st = batch_wait(ctx, S);
+    if (st != FPGM_OK) return st;
+    for (int64_t i0 = 0; i0 < n;) {
+        int64_t i1 = i0;
+        st = batch_submit(ctx, S, tests, n, zpool, zlen, alpha, flags, band, tflags, rows_per_chunk_forced, i0, &i1,
+                          i0 == 0 ? keep_ws : nullptr);
+        if (st != FPGM_OK) return st;
+        st = batch_wait(ctx, S);
+        if (st != FPGM_OK) return st;
+        std::copy(S.h_res.p, S.h_res.p + (i1 - i0), out.begin() + i0);
         i0 = i1;
     }
     return FPGM_OK;
@@ -436,10 +580,16 @@ void reset_kernel_stats(fpgm_ctx *ctx)
 // Level 0 of one instance on the MFMA units: C = O^T O, then one thread per pair
 // (pairs in tv order, all of instance `inst`).  Returns false in *used when the
 // one-hot width is too large for a dense C (the caller then counts pair by pair).
+// Results land in the pinned buffer of slot 0; *out points at them.
 fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest> &tv, double alpha, uint32_t flags,
-                            float band, std::vector<DevResult> &res, bool *used)
+                            float band, const DevResult **out, bool *used)
 {
     *used = false;
+    Slot &S = ctx->slot[0];
+    {
+        fpgm_status st = batch_wait(ctx, S);
+        if (st != FPGM_OK) return st;
+    }
     const int n = ctx->n_vars;
     std::vector<int32_t> off(n + 1, 0);
     for (int v = 0; v < n; ++v) off[v + 1] = off[v] + ctx->card[v];
@@ -482,7 +632,8 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     HIP_TRY(ctx->d_tiles.ensure((int64_t)tiles.size(), false, ctx->stream), "alloc gram tiles");
     HIP_TRY(ctx->d_off.ensure(n + 1, false, ctx->stream), "alloc gram offsets");
     HIP_TRY(ctx->d_pairs.ensure(2 * np, false, ctx->stream), "alloc pairs");
-    HIP_TRY(ctx->d_res.ensure(np, false, ctx->stream), "alloc results");
+    HIP_TRY(S.d_res.ensure(np, false, ctx->stream), "alloc results");
+    HIP_TRY(S.h_res.ensure(np, ctx->stream), "alloc pinned results");
     HIP_TRY(hipMemcpyAsync(ctx->d_rowvar.p, rowvar.data(), sizeof(int32_t) * Wp, hipMemcpyHostToDevice, ctx->stream),
             "upload gram rows");
     HIP_TRY(hipMemcpyAsync(ctx->d_rowstate.p, rowstate.data(), Wp, hipMemcpyHostToDevice, ctx->stream),
@@ -519,7 +670,7 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     p.z_alpha = ctx->z_alpha;
     p.band = band > 0.f ? band : 0.02f;
     p.flags = flags;
-    p.res = ctx->d_res.p;
+    p.res = S.d_res.p;
     hipEvent_t e_mid = nullptr;
     HIP_TRY(hipEventCreate(&e_mid), "event");
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
@@ -527,9 +678,9 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     HIP_TRY(hipEventRecord(e_mid, ctx->stream), "event");
     HIP_TRY(launch_pairs_l0(p, ctx->stream), "launch pairs");
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
-    HIP_TRY(launch_pvalue(ctx->d_res.p, np, alpha, ctx->stream), "launch pvalue");
-    res.resize((size_t)np);
-    HIP_TRY(hipMemcpyAsync(res.data(), ctx->d_res.p, sizeof(DevResult) * np, hipMemcpyDeviceToHost, ctx->stream),
+    HIP_TRY(launch_pvalue(S.d_res.p, np, alpha, ctx->stream), "launch pvalue");
+    *out = S.h_res.p;
+    HIP_TRY(hipMemcpyAsync(S.h_res.p, S.d_res.p, sizeof(DevResult) * np, hipMemcpyDeviceToHost, ctx->stream),
             "download results");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
     float ms_g = 0.f, ms_p = 0.f;
@@ -699,13 +850,14 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_ROUND_BASE")) ctx->sched.base = std::max(1.0, std::atof(e));
     if (const char *e = std::getenv("FPGM_ROUND_CAP")) ctx->sched.cap = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_ROUND_PROMOTE")) ctx->sched.promote = std::max<int64_t>(0, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_LANES")) ctx->sched.lanes = std::atoi(e) == 2 ? 2 : 1;
     if (const char *e = std::getenv("FPGM_MAX_SLICES")) ctx->max_slices = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ctx->ev1), "hipEventCreate");
-    for (hipEvent_t &e : ctx->ev_grp) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    for (Slot &sl : ctx->slot) HIP_TRY(sl.init(), "hipEventCreate");
     ctx->n_vars = n_vars;
     ctx->card.assign(card, card + n_vars);
     HIP_TRY(hipMalloc((void **)&ctx->d_card, sizeof(int32_t) * std::max(1, n_vars)), "hipMalloc card");
@@ -831,13 +983,10 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     if (ctx->d_inst) (void)hipFree(ctx->d_inst);
     if (ctx->d_card) (void)hipFree(ctx->d_card);
     if (ctx->d_crit) (void)hipFree(ctx->d_crit);
-    ctx->d_tests.release();
-    ctx->d_zpool.release();
-    ctx->d_item_test.release();
+    for (Slot &sl : ctx->slot) sl.release();
     ctx->d_ws.release();
     ctx->d_done.release();
     ctx->d_flag.release();
-    ctx->d_res.release();
     ctx->d_C.release();
     ctx->d_rowvar.release();
     ctx->d_rowstate.release();
@@ -846,8 +995,6 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     ctx->d_pairs.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    for (hipEvent_t e : ctx->ev_grp)
-        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -985,8 +1132,6 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
     }
     std::vector<DevResult> res;
     std::vector<HostTest> tv;
-    std::vector<int32_t> zpool;
-    std::vector<int64_t> t_edge;  // (instance, edge) key of each test in the round
 
     // ---- level 0: every pair of the complete graph (PCStable.cpp:133-195)
     {
@@ -995,26 +1140,29 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         for (int32_t i = 0; i < n_inst; ++i)
             for (size_t e = 0; e < P[i].ex.size(); ++e) tv.push_back(HostTest{P[i].ex[e], P[i].ey[e], 0, i, 0});
         bool gram = false;
+        const DevResult *r0 = nullptr;
         if (!plain && n_inst == 1 && !(o.flags & FPGM_F_NO_GRAM)) {
             const double ms0 = ctx->g_ms + ctx->g_pair_ms;
-            st = run_level0_gram(ctx, 0, tv, alpha, o.flags, band, res, &gram);
+            st = run_level0_gram(ctx, 0, tv, alpha, o.flags, band, &r0, &gram);
             if (st != FPGM_OK) return st;
             S.kernel_seconds_per_level[0] += (ctx->g_ms + ctx->g_pair_ms - ms0) * 1e-3;
         }
         if (!gram) {
             st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
             if (st != FPGM_OK) return st;
+            r0 = res.data();
         }
+        const DevResult *res0 = r0;
         int64_t k = 0, inb = 0, left = 0;
         for (int32_t i = 0; i < n_inst; ++i) {
             PcInst &I = P[i];
             int64_t w = 0;
             for (size_t e = 0; e < I.ex.size(); ++e, ++k) {
-                inb += res[k].in_band;
-                if (res[k].exact && res[k].df > 0 && res[k].g2_64 > 0.0)
+                inb += res0[k].in_band;
+                if (res0[k].exact && res0[k].df > 0 && res0[k].g2_64 > 0.0)
                     S.max_rel_err_g2_fp32 =
-                        std::max(S.max_rel_err_g2_fp32, std::fabs((double)res[k].g2_32 - res[k].g2_64) / res[k].g2_64);
-                if (res[k].indep) {
+                        std::max(S.max_rel_err_g2_fp32, std::fabs((double)res0[k].g2_32 - res0[k].g2_64) / res0[k].g2_64);
+                if (res0[k].indep) {
                     I.seps.push_back(PcInst::Sep{I.ex[e], I.ey[e], 0, 0, (int64_t)I.sepz.size()});
                     I.adj[(size_t)I.ex[e] * n + I.ey[e]] = 0;
                     I.adj[(size_t)I.ey[e] * n + I.ex[e]] = 0;
@@ -1046,6 +1194,7 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         if (!any) break;
         if (d > kMaxD) return fail(ctx, FPGM_ETOOBIG, "conditioning depth exceeds the device limit");
         const double tl = now_s();
+        const double prep0 = ctx->t_prep, wait0 = ctx->t_wait;
         std::vector<std::pair<int32_t, int64_t>> active;  // (instance, edge)
         for (int32_t i = 0; i < n_inst; ++i) {
             PcInst &I = P[i];
@@ -1078,24 +1227,48 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                 ? o.round_tests
                 : std::min<int64_t>((int64_t)1 << 20,
                                     std::max<int64_t>(8192, (int64_t)(2147483648.0 / std::max(1.0, (d + 2) * avg_rows))));
-        while (!active.empty()) {
-            if (o.max_rounds > 0 && rounds >= o.max_rounds)
+        // With two lanes, the edges take turns: while the device counts one lane's round,
+        // the host takes the other lane's results and builds its next round.  Edges are
+        // independent within a level (frozen adjacency), so the split changes timing and
+        // speculation only, never results.  One lane is the default: at 10^6 rows the
+        // half-size launches of two lanes cost the device more than the overlap saves.
+        // Plain mode runs one lane synchronously.
+        struct Lane {
+            std::vector<std::pair<int32_t, int64_t>> active;  // (instance, edge)
+            std::vector<HostTest> tv;
+            std::vector<int32_t> zpool;
+            std::vector<int64_t> t_edge;  // active index of each test of the round
+            std::vector<DevResult> res;   // results of rounds not run asynchronously
+            const DevResult *r = nullptr;
+            int64_t rounds = 0;
+            int slot = 0;
+            bool in_flight = false, async = false;
+        };
+        const int n_lanes = plain ? 1 : ctx->sched.lanes;
+        Lane lanes[2];
+        for (size_t ai = 0; ai < active.size(); ++ai) lanes[ai % n_lanes].active.push_back(active[ai]);
+        for (int l = 0; l < n_lanes; ++l) lanes[l].slot = l;
+        const int64_t lane_budget = std::max<int64_t>(1, budget / n_lanes);
+        const double kms_level0 = ctx->k_ms;
+
+        auto start = [&](Lane &ln) -> fpgm_status {
+            if (o.max_rounds > 0 && ln.rounds >= o.max_rounds)
                 return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
-            ++rounds;
+            ++ln.rounds;
             const bool plain_round = plain && (o.plain_sample <= 0 || S.plain_tests_per_level[d] < o.plain_sample);
             // tests per edge this round: the budget shared by the active edges, at least a
             // geometric ramp (base^(round-1), capped) so long walks finish in few rounds
-            int64_t W = plain_round ? 1 : std::max<int64_t>(1, budget / (int64_t)active.size());
+            int64_t W = plain_round ? 1 : std::max<int64_t>(1, lane_budget / (int64_t)ln.active.size());
             if (!plain_round) {
-                const double ramp = std::pow(ctx->sched.base, (double)(rounds - 1));
+                const double ramp = std::pow(ctx->sched.base, (double)(ln.rounds - 1));
                 W = std::max<int64_t>(W, (int64_t)std::min<double>(ramp, (double)ctx->sched.cap));
             }
-            tv.clear();
-            zpool.clear();
-            t_edge.clear();
-            for (size_t ai = 0; ai < active.size(); ++ai) {
-                PcInst &I = P[active[ai].first];
-                const int64_t e = active[ai].second;
+            ln.tv.clear();
+            ln.zpool.clear();
+            ln.t_edge.clear();
+            for (size_t ai = 0; ai < ln.active.size(); ++ai) {
+                PcInst &I = P[ln.active[ai].first];
+                const int64_t e = ln.active[ai].second;
                 EdgeWalk &w = I.walks[e];
                 int32_t *c = &I.choice[(size_t)e * d];
                 // an edge found dependent on its first `promote` sets rarely ends early:
@@ -1103,39 +1276,101 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                 int64_t We = W;
                 if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
                 for (int64_t k = 0; k < We && w.side < 2; ++k) {
-                    tv.push_back(HostTest{w.x, w.y, d, active[ai].first, (int64_t)zpool.size()});
-                    for (int i = 0; i < d; ++i) zpool.push_back(I.F.at(w.u, w.skip, c[i]));
-                    t_edge.push_back((int64_t)ai);
+                    ln.tv.push_back(HostTest{w.x, w.y, d, ln.active[ai].first, (int64_t)ln.zpool.size()});
+                    for (int i = 0; i < d; ++i) ln.zpool.push_back(I.F.at(w.u, w.skip, c[i]));
+                    ln.t_edge.push_back((int64_t)ai);
                     if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
                 }
             }
-            st = run_tests(ctx, plain_round, o, d, tv, zpool.data(), (int64_t)zpool.size(), alpha, band, res, S);
-            if (st != FPGM_OK) return st;
-            executed += (int64_t)tv.size();
+            executed += (int64_t)ln.tv.size();
+            ln.in_flight = true;
+            ln.async = false;
+            const int64_t nt = (int64_t)ln.tv.size(), nz = (int64_t)ln.zpool.size();
+            if (plain) {
+                fpgm_status st2 = run_tests(ctx, plain_round, o, d, ln.tv, ln.zpool.data(), nz, alpha, band, ln.res, S);
+                ln.r = ln.res.data();
+                return st2;
+            }
+            Slot &sl = ctx->slot[ln.slot];
+            int64_t i1 = 0;
+            fpgm_status st2 =
+                batch_submit(ctx, sl, ln.tv.data(), nt, ln.zpool.data(), nz, alpha, o.flags, band, 0, 0, 0, &i1, nullptr);
+            if (st2 != FPGM_OK) return st2;
+            if (i1 == nt) {
+                ln.async = true;
+                return FPGM_OK;
+            }
+            // more than one launch sequence (workspace limit): finish the round here
+            ln.res.resize((size_t)nt);
+            for (int64_t i0 = 0;;) {
+                st2 = batch_wait(ctx, sl);
+                if (st2 != FPGM_OK) return st2;
+                std::copy(sl.h_res.p, sl.h_res.p + (i1 - i0), ln.res.begin() + i0);
+                i0 = i1;
+                if (i0 >= nt) break;
+                st2 = batch_submit(ctx, sl, ln.tv.data(), nt, ln.zpool.data(), nz, alpha, o.flags, band, 0, 0, i0, &i1,
+                                   nullptr);
+                if (st2 != FPGM_OK) return st2;
+            }
+            ln.r = ln.res.data();
+            return FPGM_OK;
+        };
+
+        auto finish = [&](Lane &ln) -> fpgm_status {
+            if (ln.async) {
+                fpgm_status st2 = batch_wait(ctx, ctx->slot[ln.slot]);
+                if (st2 != FPGM_OK) return st2;
+                ln.r = ctx->slot[ln.slot].h_res.p;
+            }
+            const DevResult *r = ln.r;
             std::vector<std::pair<int32_t, int64_t>> still;
             size_t k = 0;
-            for (size_t ai = 0; ai < active.size(); ++ai) {
-                PcInst &I = P[active[ai].first];
-                EdgeWalk &w = I.walks[active[ai].second];
+            for (size_t ai = 0; ai < ln.active.size(); ++ai) {
+                PcInst &I = P[ln.active[ai].first];
+                EdgeWalk &w = I.walks[ln.active[ai].second];
                 bool done = false;
-                for (; k < tv.size() && t_edge[k] == (int64_t)ai; ++k) {
-                    inb += res[k].in_band;
-                    if (res[k].exact && res[k].df > 0 && res[k].g2_64 > 0.0)
-                        S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32,
-                                                         std::fabs((double)res[k].g2_32 - res[k].g2_64) / res[k].g2_64);
+                for (; k < ln.tv.size() && ln.t_edge[k] == (int64_t)ai; ++k) {
+                    inb += r[k].in_band;
+                    if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
+                        S.max_rel_err_g2_fp32 =
+                            std::max(S.max_rel_err_g2_fp32, std::fabs((double)r[k].g2_32 - r[k].g2_64) / r[k].g2_64);
                     if (done) continue;
                     w.tests += 1;
-                    if (res[k].indep) {
+                    if (r[k].indep) {
                         done = true;
                         w.removed = 1;
                         w.zsep = (int64_t)I.zlevel.size();
-                        for (int i = 0; i < d; ++i) I.zlevel.push_back(zpool[tv[k].z_off + i]);
+                        for (int i = 0; i < d; ++i) I.zlevel.push_back(ln.zpool[ln.tv[k].z_off + i]);
                     }
                 }
-                if (!done && w.side < 2) still.push_back(active[ai]);
+                if (!done && w.side < 2) still.push_back(ln.active[ai]);
             }
-            active.swap(still);
+            ln.active.swap(still);
+            ln.in_flight = false;
+            return FPGM_OK;
+        };
+
+        for (int l = 0; l < n_lanes; ++l)
+            if (!lanes[l].active.empty()) {
+                st = start(lanes[l]);
+                if (st != FPGM_OK) return st;
+            }
+        for (bool any = true; any;) {
+            any = false;
+            for (int l = 0; l < n_lanes; ++l) {
+                Lane &ln = lanes[l];
+                if (!ln.in_flight) continue;
+                any = true;
+                st = finish(ln);
+                if (st != FPGM_OK) return st;
+                if (!ln.active.empty()) {
+                    st = start(ln);
+                    if (st != FPGM_OK) return st;
+                }
+            }
         }
+        rounds = std::max(lanes[0].rounds, lanes[1].rounds);
+        if (!plain) S.kernel_seconds_per_level[d] += (ctx->k_ms - kms_level0) * 1e-3;
         int64_t tests = 0, left = 0;
         for (PcInst &I : P) {
             if (!I.cont) {
@@ -1179,6 +1414,9 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         S.seconds_per_level[d] = now_s() - tl;
         S.n_levels = d + 1;
         log_level(o, d, tests, left, now_s() - t_start);
+        if (ctx->trace)
+            fprintf(stderr, "[fpgm]   level %d host: %.4f s building launches, %.4f s launching and waiting, "
+                    "%.4f s level total\n", d, ctx->t_prep - prep0, ctx->t_wait - wait0, now_s() - tl);
     }
     S.ci_seconds = now_s() - t_start;
     for (int l = 0; l < S.n_levels; ++l) S.num_ci_test += S.tests_per_level[l];

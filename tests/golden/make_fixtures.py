@@ -7,7 +7,11 @@ Run in the build container only (it reads /root/reference and oracle/_ref):
 Outputs (data only: inputs and expected outputs):
   alarm_s5000.npz        ALARM sample (dataset/alarm/alarm_s5000), coded by the reference's
                          own CSV loader (Dataset::LoadCSVTrainingData, first-appearance coding)
+  alarm_s5000.csv.gz     the ALARM training file itself (dataset/alarm/alarm_s5000), gzip'd,
+                         input of the host loader tests
   alarm_s5000_struct.json  the 44 adjacencies of dataset/alarm/alarm_s5000_struct
+  alarm_s5000_struct.txt the reference's saved CPDAG (dataset/alarm/alarm_s5000_struct) verbatim,
+                         expected output of orientation + structure file
   ref_choices.json       ChoiceGenerator series for small (a, b)
   ref_counts.npz         Counts2D / Counts3D tables and marginals for sampled tests
   ref_counts_group.npz   Counts3DGroup tables for sampled groups
@@ -15,6 +19,7 @@ Outputs (data only: inputs and expected outputs):
 from __future__ import annotations
 
 import ctypes as C
+import gzip
 import json
 import os
 import re
@@ -82,6 +87,14 @@ def main():
     assert np.array_equal(cols[:, : nr.value], arr.T.astype(np.uint8)), "CSV coding restatement disagrees"
     np.savez_compressed(os.path.join(HERE, "alarm_s5000.npz"), cols=cols[:, : nr.value], card=card,
                         names=np.array(names))
+
+    with open(csv, "rb") as f, open(os.path.join(HERE, "alarm_s5000.csv.gz"), "wb") as raw:
+        with gzip.GzipFile(fileobj=raw, mode="wb", mtime=0) as g:
+            g.write(f.read())
+    with open(os.path.join(REF, "dataset", "alarm", "alarm_s5000_struct"), "rb") as f:
+        struct_text = f.read()
+    with open(os.path.join(HERE, "alarm_s5000_struct.txt"), "wb") as f:
+        f.write(struct_text)
 
     edges = []
     with open(os.path.join(REF, "dataset", "alarm", "alarm_s5000_struct")) as f:

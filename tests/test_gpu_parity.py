@@ -119,6 +119,19 @@ def test_skeleton_round_size_does_not_change_results():
     assert sum(c["in_band_per_level"]) <= sum(c["executed_per_level"])
 
 
+def test_skeleton_two_edge_lanes_same_results(monkeypatch):
+    # FPGM_LANES=2 alternates the rounds of two edge lanes; results must not change
+    s = configs.get(2).sample(20_000)
+    with _ctx(s) as ctx:
+        a = ctx.pc_skeleton(0.01, 5)
+    monkeypatch.setenv("FPGM_LANES", "2")
+    with _ctx(s) as ctx:
+        b = ctx.pc_skeleton(0.01, 5)
+        c = ctx.pc_skeleton(0.01, 5, round_tests=64)
+    _assert_same(a, b)
+    _assert_same(a, c)
+
+
 def test_skeleton_depth_and_alpha():
     w = configs.get(2)
     s = w.sample(20_000)
