@@ -700,16 +700,32 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
             __syncthreads();
             tab = small;
         } else if (kMode == kSinkShared) {
+            // As many histogram copies as fit: `sub` copies per wave, lane l counting into
+            // copy l % sub of its wave, so skewed data collides on one address in at most
+            // 64 / sub lanes of a wave; copies are offset by 4 banks from one another, so
+            // lanes of different copies hitting the same cell use different banks.  Tables
+            // too large for one copy per wave share copies between waves.
             const int c32 = (int)cells;
-            const int copies = (c32 <= kLdsCells / 4) ? 4 : (c32 <= kLdsCells / 2) ? 2 : 1;
-            for (int i = tid; i < copies * c32; i += kWG) lds[i] = 0;
+            const int stride = ((c32 + 63) & ~63) + 4;
+            int sub = 1;
+            while (sub < 16 && 4 * (2 * sub) * stride <= kLdsCells) sub *= 2;
+            int copies, my;
+            if (4 * stride <= kLdsCells) {
+                copies = 4 * sub;
+                my = wave * sub + (lane & (sub - 1));
+            } else {
+                copies = (2 * stride <= kLdsCells) ? 2 : 1;
+                my = wave % copies;
+            }
+            const int span = (copies > 1) ? copies * stride : c32;
+            for (int i = tid; i < span; i += kWG) lds[i] = 0;
             __syncthreads();
-            count_rows<kSinkShared>(I, T, sh, r_begin, r_end, lds + (wave % copies) * c32);
+            count_rows<kSinkShared>(I, T, sh, r_begin, r_end, lds + (copies > 1 ? my * stride : 0));
             __syncthreads();
             if (copies > 1) {
                 for (int i = tid; i < c32; i += kWG) {
                     int32_t s = lds[i];
-                    for (int k = 1; k < copies; ++k) s += lds[k * c32 + i];
+                    for (int k = 1; k < copies; ++k) s += lds[k * stride + i];
                     lds[i] = s;
                 }
                 __syncthreads();
