@@ -30,7 +30,7 @@ EXPORTS = [
     "fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ctx_destroy", "fpgm_last_error", "fpgm_status_str",
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats",
-    "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats",
+    "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
 ]
 
 
@@ -102,9 +102,10 @@ def load(path: str = LIB_PATH):
     L.fpgm_ctx_create_multi.argtypes = [vp, vp, vp, i32, i32, vp, C.POINTER(vp)]
     L.fpgm_last_gram_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
     L.fpgm_pc_skeleton_multi.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, vp, vp, C.POINTER(Stats)]
+    L.fpgm_mle_counts.argtypes = [vp, i32, vp, vp, vp, vp, i64]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
-                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats"):
+                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -283,6 +284,25 @@ class Context:
                 "plain_tests_per_level": [int(v) for v in st.plain_tests_per_level[:L]],
                 "plain_seconds_per_level": [float(v) for v in st.plain_seconds_per_level[:L]],
                 "kernel_seconds_per_level": [float(v) for v in st.kernel_seconds_per_level[:L]]}
+
+    def mle_counts(self, parents):
+        """Maximum-likelihood tables N(AB) of nodes 0..len(parents)-1 with the given parent
+        lists (sorted here): (count_off int64[n+1], counts int32[total])."""
+        n = len(parents)
+        off = np.zeros(n + 1, dtype=np.int32)
+        flat = []
+        for i, ps in enumerate(parents):
+            flat.extend(sorted(int(p) for p in ps))
+            off[i + 1] = len(flat)
+        flat = np.ascontiguousarray(np.asarray(flat if flat else [0], dtype=np.int32))
+        count_off = np.zeros(n + 1, dtype=np.int64)
+        self._check(self._L.fpgm_mle_counts(self.h, n, off.ctypes.data, flat.ctypes.data, count_off.ctypes.data,
+                                            None, 0))
+        total = int(count_off[n])
+        counts = np.zeros(max(total, 1), dtype=np.int32)
+        self._check(self._L.fpgm_mle_counts(self.h, n, off.ctypes.data, flat.ctypes.data, count_off.ctypes.data,
+                                            counts.ctypes.data, total))
+        return count_off, counts[:total]
 
     def gram_stats(self):
         ms, ops, pms, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()

@@ -20,6 +20,7 @@
 #include "chisq.h"
 #include "fpgm_ci.h"
 #include "gram_kernels.h"
+#include "mle_kernels.h"
 
 #include <algorithm>
 #include <chrono>
@@ -229,6 +230,10 @@ struct fpgm_ctx {
     DevBuf<uint8_t> d_rowstate;
     double g_ms = 0.0, g_ops = 0.0, g_pair_ms = 0.0;
     int64_t g_launches = 0;
+
+    // maximum-likelihood counts
+    DevBuf<MleNode> d_mle_nodes;
+    DevBuf<int32_t> d_mle_rel, d_mle_items, d_mle_out;
 
     std::vector<fpgm_sepset> seps;
     std::vector<int32_t> sep_z;
@@ -657,6 +662,7 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     g.k_splits = (int32_t)splits;
     g.k_per_split = kps;
     g.C = ctx->d_C.p;
+    g.max_card = *std::max_element(ctx->card.begin(), ctx->card.end());
     PairArgs p{};
     p.C = ctx->d_C.p;
     p.Wp = (int32_t)Wp;
@@ -993,6 +999,10 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     ctx->d_tiles.release();
     ctx->d_off.release();
     ctx->d_pairs.release();
+    ctx->d_mle_nodes.release();
+    ctx->d_mle_rel.release();
+    ctx->d_mle_items.release();
+    ctx->d_mle_out.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1070,6 +1080,89 @@ fpgm_status fpgm_ci_counts(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z
     for (int j = 0; j < d; ++j) dimz *= ctx->card[z[j]];
     const int64_t used = nc + dimz * (ctx->card[x] + ctx->card[y] + 1);
     HIP_TRY(hipMemsetAsync(ctx->d_ws.p + off, 0, sizeof(int32_t) * used, ctx->stream), "clear table");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_mle_counts(fpgm_ctx *ctx, int32_t n_nodes, const int32_t *par_off, const int32_t *parents,
+                            int64_t *count_off, int32_t *counts, int64_t cap)
+{
+    if (!ctx || !par_off || !count_off) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->inst.size() != 1) return fail(ctx, FPGM_EINVAL, "maximum-likelihood counts need a one-instance context");
+    if (n_nodes < 1 || n_nodes > ctx->n_vars) return fail(ctx, FPGM_EINVAL, "n_nodes must lie in [1, n_vars]");
+    if (par_off[0] != 0) return fail(ctx, FPGM_EINVAL, "par_off[0] must be 0");
+    std::vector<MleNode> nodes((size_t)n_nodes);
+    std::vector<int32_t> rel;
+    int64_t total = 0;
+    for (int32_t i = 0; i < n_nodes; ++i) {
+        const int32_t k = par_off[i + 1] - par_off[i];
+        if (k < 0 || k + 1 > kMleMaxRel || (k > 0 && !parents))
+            return fail(ctx, FPGM_EINVAL, "bad parent list length");
+        MleNode &N = nodes[i];
+        N.n_rel = k + 1;
+        N.rel_off = (int32_t)rel.size();
+        rel.push_back(i);
+        int64_t size = ctx->card[i];
+        for (int32_t j = 0; j < k; ++j) {
+            const int32_t p = parents[par_off[i] + j];
+            if (p < 0 || p >= ctx->n_vars || p == i) return fail(ctx, FPGM_EINVAL, "bad parent index");
+            if (j > 0 && p <= parents[par_off[i] + j - 1])
+                return fail(ctx, FPGM_EINVAL, "parents must be in ascending order");
+            rel.push_back(p);
+            size *= ctx->card[p];
+            if (size >= kMaxCells) return fail(ctx, FPGM_ETOOBIG, "probability table exceeds 2^31 cells");
+        }
+        N.size = (int32_t)size;
+        N.out_off = total;
+        count_off[i] = total;
+        total += size;
+    }
+    count_off[n_nodes] = total;
+    if (!counts) return FPGM_OK;  // size query
+    if (cap < total) return fail(ctx, FPGM_EINVAL, "counts capacity too small");
+    const InstHost &I = ctx->inst[0];
+    const int64_t steps = std::max<int64_t>(1, (I.n_rows + kRowsPerStep - 1) / kRowsPerStep);
+    // rows per chunk: enough work items for every CU, at least 16 steps per item
+    const int64_t cpn = std::max<int64_t>(1, std::min<int64_t>(steps / 16, ((int64_t)ctx->n_cu * 8 + n_nodes - 1) / n_nodes));
+    const int64_t rpc = (steps + cpn - 1) / cpn * kRowsPerStep;
+    const int64_t n_chunks = std::max<int64_t>(1, (I.n_rows + rpc - 1) / rpc);
+    std::vector<int32_t> items;
+    items.reserve((size_t)(n_chunks * n_nodes));
+    for (int32_t i = 0; i < n_nodes; ++i) {
+        nodes[i].item0 = (int32_t)items.size();
+        nodes[i].n_chunks = (int32_t)n_chunks;
+        for (int64_t c = 0; c < n_chunks; ++c) items.push_back(i);
+    }
+    HIP_TRY(ctx->d_mle_nodes.ensure(n_nodes, false, ctx->stream), "alloc mle nodes");
+    HIP_TRY(ctx->d_mle_rel.ensure((int64_t)rel.size(), false, ctx->stream), "alloc mle variables");
+    HIP_TRY(ctx->d_mle_items.ensure((int64_t)items.size(), false, ctx->stream), "alloc mle items");
+    HIP_TRY(ctx->d_mle_out.ensure(std::max<int64_t>(total, 1), false, ctx->stream), "alloc mle tables");
+    HIP_TRY(hipMemcpyAsync(ctx->d_mle_nodes.p, nodes.data(), sizeof(MleNode) * n_nodes, hipMemcpyHostToDevice,
+                           ctx->stream),
+            "upload mle nodes");
+    HIP_TRY(hipMemcpyAsync(ctx->d_mle_rel.p, rel.data(), sizeof(int32_t) * rel.size(), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "upload mle variables");
+    HIP_TRY(hipMemcpyAsync(ctx->d_mle_items.p, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "upload mle items");
+    HIP_TRY(hipMemsetAsync(ctx->d_mle_out.p, 0, sizeof(int32_t) * total, ctx->stream), "clear mle tables");
+    MleArgs a{};
+    a.cols = I.cols;
+    a.stride = I.stride;
+    a.n_rows = I.n_rows;
+    a.nodes = ctx->d_mle_nodes.p;
+    a.rel = ctx->d_mle_rel.p;
+    a.card = ctx->d_card;
+    a.item_node = ctx->d_mle_items.p;
+    a.n_items = (int64_t)items.size();
+    a.rows_per_chunk = rpc;
+    a.out = ctx->d_mle_out.p;
+    const int grid = (int)std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * 4);
+    HIP_TRY(launch_mle_counts(a, grid, ctx->stream), "launch mle counts");
+    HIP_TRY(hipMemcpyAsync(counts, ctx->d_mle_out.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost, ctx->stream),
+            "download mle tables");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
     return FPGM_OK;
 }

@@ -22,6 +22,7 @@ struct GramArgs {
     int32_t k_splits;          // row splits per tile pair
     int64_t k_per_split;       // rows per split, multiple of 32
     int32_t *C;                // [Wp][Wp] int32, zero before the launch
+    int32_t max_card;          // largest cardinality (<= 8: one-hot bytes by table lookup)
 };
 
 struct PairArgs {

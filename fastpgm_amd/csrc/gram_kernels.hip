@@ -5,7 +5,8 @@
 // block of C = O^T O: n_{vw}[a][b] = C[o_v + a][o_w + b] (the Counts2D table,
 // CellTable.cpp:430-441).  C is computed with v_mfma_i32_32x32x32_i8 (int8 x int8 ->
 // int32, exact) over 128 x 128 tiles of the upper triangle; the one-hot operands are
-// formed in registers from 16-byte loads of the uint8 columns.  A second kernel reads
+// formed in registers from 16-byte loads of the uint8 columns (one v_perm_b32 table
+// lookup per 4 rows when every cardinality is <= 8).  A second kernel reads
 // each pair's block and evaluates df / G^2 / decision like the counting kernels.
 //
 // MFMA operand layout (checked on the device): lane l supplies row (or column) l % 32
@@ -30,16 +31,6 @@ __device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
     return (~(y | t) & 0x80808080u) >> 7;
 }
 
-__device__ __forceinline__ v4i onehot16(const uint4 &q, uint32_t pat, uint4 mask)
-{
-    v4i r;
-    r[0] = (int)(eq_bytes(q.x, pat) & mask.x);
-    r[1] = (int)(eq_bytes(q.y, pat) & mask.y);
-    r[2] = (int)(eq_bytes(q.z, pat) & mask.z);
-    r[3] = (int)(eq_bytes(q.w, pat) & mask.w);
-    return r;
-}
-
 __device__ __forceinline__ uint32_t byte_mask_word(int64_t valid, int word)
 {
     // bytes [4*word, 4*word+4) of a 16-byte group, valid if their index < valid
@@ -49,6 +40,56 @@ __device__ __forceinline__ uint32_t byte_mask_word(int64_t valid, int word)
     return 0xFFFFFFFFu >> (8 * (4 - (int)(valid - lo)));
 }
 
+// One-hot operand of one row/column of O for 16 rows.  Every cardinality <= 8
+// (kPerm): the column bytes are states 0..7 and index the 8-byte table (lo, hi) whose
+// byte `state` is 1, so v_perm_b32 yields the 16 one-hot bytes in 4 instructions.
+// Otherwise a SWAR byte comparison against state * 0x01010101 (lo).
+template <bool kPerm>
+__device__ __forceinline__ v4i onehot(const uint4 &q, uint32_t lo, uint32_t hi)
+{
+    v4i r;
+    if (kPerm) {
+        r[0] = (int)__builtin_amdgcn_perm(hi, lo, q.x);
+        r[1] = (int)__builtin_amdgcn_perm(hi, lo, q.y);
+        r[2] = (int)__builtin_amdgcn_perm(hi, lo, q.z);
+        r[3] = (int)__builtin_amdgcn_perm(hi, lo, q.w);
+    } else {
+        r[0] = (int)eq_bytes(q.x, lo);
+        r[1] = (int)eq_bytes(q.y, lo);
+        r[2] = (int)eq_bytes(q.z, lo);
+        r[3] = (int)eq_bytes(q.w, lo);
+    }
+    return r;
+}
+
+__device__ __forceinline__ v4i and4(v4i a, const uint4 &m)
+{
+    a[0] &= (int)m.x;
+    a[1] &= (int)m.y;
+    a[2] &= (int)m.z;
+    a[3] &= (int)m.w;
+    return a;
+}
+
+// Lookup words of one-hot row i: (lo, hi) for the v_perm_b32 form, (pattern, -) for
+// the byte comparison.  Padding rows (state 255) never match in either form.
+template <bool kPerm>
+__device__ __forceinline__ void onehot_lut(uint8_t state, uint32_t &lo, uint32_t &hi)
+{
+    if (kPerm) {
+        lo = (state < 4) ? 1u << (8 * state) : 0u;
+        hi = (state >= 4 && state < 8) ? 1u << (8 * (state - 4)) : 0u;
+    } else {
+        lo = (uint32_t)state * 0x01010101u;
+        hi = 0u;
+    }
+}
+
+struct Ops4 {
+    uint4 a0, a1, b0, b1;
+};
+
+template <bool kPerm>
 __global__ __launch_bounds__(256) void k_gram_l0(GramArgs g)
 {
     const int item = blockIdx.x;
@@ -58,33 +99,66 @@ __global__ __launch_bounds__(256) void k_gram_l0(GramArgs g)
     const int h = lane >> 5, l32 = lane & 31;
     const int ib = ti * kGramTile + (wave >> 1) * 64 + l32;
     const int jb = tj * kGramTile + (wave & 1) * 64 + l32;
-    const uint8_t *pa0 = g.cols + (int64_t)g.row_var[ib] * g.stride;
-    const uint8_t *pa1 = g.cols + (int64_t)g.row_var[ib + 32] * g.stride;
-    const uint8_t *pb0 = g.cols + (int64_t)g.row_var[jb] * g.stride;
-    const uint8_t *pb1 = g.cols + (int64_t)g.row_var[jb + 32] * g.stride;
-    const uint32_t sa0 = (uint32_t)g.row_state[ib] * 0x01010101u, sa1 = (uint32_t)g.row_state[ib + 32] * 0x01010101u;
-    const uint32_t sb0 = (uint32_t)g.row_state[jb] * 0x01010101u, sb1 = (uint32_t)g.row_state[jb + 32] * 0x01010101u;
+    const uint8_t *pa0 = g.cols + (int64_t)g.row_var[ib] * g.stride + h * 16;
+    const uint8_t *pa1 = g.cols + (int64_t)g.row_var[ib + 32] * g.stride + h * 16;
+    const uint8_t *pb0 = g.cols + (int64_t)g.row_var[jb] * g.stride + h * 16;
+    const uint8_t *pb1 = g.cols + (int64_t)g.row_var[jb + 32] * g.stride + h * 16;
+    uint32_t la0, ha0, la1, ha1, lb0, hb0, lb1, hb1;
+    onehot_lut<kPerm>(g.row_state[ib], la0, ha0);
+    onehot_lut<kPerm>(g.row_state[ib + 32], la1, ha1);
+    onehot_lut<kPerm>(g.row_state[jb], lb0, hb0);
+    onehot_lut<kPerm>(g.row_state[jb + 32], lb1, hb1);
 
     v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
     const int64_t r_begin = (int64_t)ks * g.k_per_split;
     const int64_t r_end = (r_begin + g.k_per_split < g.n_rows) ? r_begin + g.k_per_split : g.n_rows;
-    const uint4 full = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-    for (int64_t r0 = r_begin; r0 < r_end; r0 += 32) {
-        const int64_t rr = r0 + h * 16;
-        uint4 m = full;
-        const int64_t valid = r_end - rr;  // rows of this lane's 16-byte group that count
-        if (valid < 16) {
-            m.x = byte_mask_word(valid, 0);
-            m.y = byte_mask_word(valid, 1);
-            m.z = byte_mask_word(valid, 2);
-            m.w = byte_mask_word(valid, 3);
+    // full 32-row steps: no masks; the loads of the next step are issued before the
+    // current step's one-hot formation and MFMAs (two register buffers in turn)
+    const int64_t full_end = r_begin + (r_end - r_begin) / 32 * 32;
+    auto load = [&](int64_t r0, Ops4 &o) {
+        o.a0 = *reinterpret_cast<const uint4 *>(pa0 + r0);
+        o.a1 = *reinterpret_cast<const uint4 *>(pa1 + r0);
+        o.b0 = *reinterpret_cast<const uint4 *>(pb0 + r0);
+        o.b1 = *reinterpret_cast<const uint4 *>(pb1 + r0);
+    };
+    auto step = [&](const Ops4 &o) {
+        const v4i a0 = onehot<kPerm>(o.a0, la0, ha0), a1 = onehot<kPerm>(o.a1, la1, ha1);
+        const v4i b0 = onehot<kPerm>(o.b0, lb0, hb0), b1 = onehot<kPerm>(o.b1, lb1, hb1);
+        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+    };
+    if (r_begin < full_end) {
+        // the scheduling barriers keep the next step's loads ahead of this step's
+        // arithmetic (left alone, the compiler sinks them to their first use)
+        Ops4 o0, o1;
+        load(r_begin, o0);
+        int64_t r0 = r_begin;
+        for (; r0 + 32 < full_end; r0 += 64) {
+            load(r0 + 32, o1);
+            __builtin_amdgcn_sched_barrier(0);
+            step(o0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (r0 + 64 < full_end) load(r0 + 64, o0);
+            __builtin_amdgcn_sched_barrier(0);
+            step(o1);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        const uint4 qa0 = *reinterpret_cast<const uint4 *>(pa0 + rr);
-        const uint4 qa1 = *reinterpret_cast<const uint4 *>(pa1 + rr);
-        const uint4 qb0 = *reinterpret_cast<const uint4 *>(pb0 + rr);
-        const uint4 qb1 = *reinterpret_cast<const uint4 *>(pb1 + rr);
-        const v4i a0 = onehot16(qa0, sa0, m), a1 = onehot16(qa1, sa1, m);
-        const v4i b0 = onehot16(qb0, sb0, m), b1 = onehot16(qb1, sb1, m);
+        if (r0 < full_end) step(o0);
+    }
+    if (full_end < r_end) {
+        // last partial step: bytes past r_end are masked out (their values are arbitrary)
+        const int64_t valid = r_end - (full_end + h * 16);
+        uint4 m;
+        m.x = byte_mask_word(valid, 0);
+        m.y = byte_mask_word(valid, 1);
+        m.z = byte_mask_word(valid, 2);
+        m.w = byte_mask_word(valid, 3);
+        Ops4 o;
+        load(full_end, o);
+        const v4i a0 = and4(onehot<kPerm>(o.a0, la0, ha0), m), a1 = and4(onehot<kPerm>(o.a1, la1, ha1), m);
+        const v4i b0 = onehot<kPerm>(o.b0, lb0, hb0), b1 = onehot<kPerm>(o.b1, lb1, hb1);
         c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
         c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
         c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
@@ -220,7 +294,8 @@ __global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
 hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gram_l0, dim3(n_items), dim3(256), 0, s, g);
+    if (g.max_card <= 8) hipLaunchKernelGGL(k_gram_l0<true>, dim3(n_items), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL(k_gram_l0<false>, dim3(n_items), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 

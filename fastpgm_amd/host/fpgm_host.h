@@ -9,6 +9,11 @@
 //                       save_struct, struct_file, verbose)            PCStable.cpp:25-82
 //   Dataset::LoadCSVTrainingData / LoadLIBSVMTrainingData            Dataset.cpp:389-503, 55-162
 //   Network edge operations, Edge                                     Network.cpp:169-360, Edge.cpp:7-33
+//   Network::OrderEdge / FindCompelled / IsDAG / GetEdge               Network.cpp:95-129, 547-699
+//   CustomNetwork::LoadBIFFile (structure and domains)                 CustomNetwork.cpp:52-230
+//   BNSLComparison(true_graph, learned_graph).GetSHD()                BNSLComparison.cpp:7-73
+//   ParameterLearning(net).LearnParamsKnowStructCompData(dts, alpha, save_param,
+//                     param_file, verbose), SaveBNParameter           ParameterLearning.cpp:10-155
 // Where the reference calls exit(1) (unreadable file, duplicate header names, alpha
 // outside [0, 1], a random DAG completion that closes a cycle), these classes throw
 // fpgm::Error under the same conditions.
@@ -60,12 +65,17 @@ public:
 };
 
 enum EndPoint { TAIL, ARROW };
+// Chickering's edge labels (Edge.h:13-31); an edge FindCompelled never labels keeps COMPELLED
+// (the reference leaves the field uninitialised)
+enum Label { COMPELLED, REVERSIBLE };
 
 // One edge (Edge.h:15-45): undirected edges are stored with node1 < node2, directed
 // edges p -> c as (p, c, TAIL, ARROW).
 struct Edge {
     int node1 = 0, node2 = 0;
     EndPoint end_point1 = TAIL, end_point2 = TAIL;
+    bool is_ordered = false;
+    Label label = COMPELLED;
     Edge() = default;
     Edge(int a, int b);                            // undirected
     Edge(int a, int b, EndPoint e1, EndPoint e2);  // as given
@@ -80,7 +90,17 @@ struct Edge {
 // neighbours in ascending order (the reference's map<int, map<int, double>>).
 using Adjacencies = std::vector<std::vector<int>>;
 
-// The graph part of the reference Network (Network.h) that PC-stable uses.
+// Conditional probability table of one node (PotentialTableBase.h): variables = the node,
+// then its parents ascending; entry index = sum config[k] * cum_levels[k].
+struct PotentialTable {
+    int num_variables = 0;
+    std::vector<int> vec_related_variables, var_dims, cum_levels;
+    int table_size = 0;
+    std::vector<double> potentials;
+};
+
+// The graph part of the reference Network (Network.h) that PC-stable, SHD and parameter
+// learning use.
 class Network {
 public:
     int num_nodes = 0;
@@ -88,10 +108,13 @@ public:
     bool pure_discrete = true;
     std::vector<std::string> node_names;
     std::vector<int> node_cards;
+    std::vector<std::map<std::string, int>> possible_values_ids;  // value name -> state, per node
     std::vector<Edge> vec_edges;
     std::vector<std::set<int>> set_parent_indexes, set_children_indexes;
+    std::vector<PotentialTable> pts;  // one per node after parameter learning
 
     explicit Network(bool pure_disc = true) : pure_discrete(pure_disc) {}
+    virtual ~Network() = default;
 
     void GenerateUndirectedCompleteGraph();
     // adds p -> c unless it closes a directed cycle; false (and no edge) when it would
@@ -107,10 +130,56 @@ public:
     bool ContainCircle() const;
     std::vector<int> GetTopoOrd();
 
-private:
+    // position in vec_edges of the first edge joining a and b in either direction, or -1
+    int GetEdge(int a, int b) const;
+    // position of the directed edge a -> b, or -1
+    int GetDirectedEdge(int a, int b) const;
+    // edges of a DAG in Chickering's order (Network.cpp:547-584)
+    std::vector<Edge> OrderEdge();
+    // labels every edge COMPELLED or REVERSIBLE (Network.cpp:590-684)
+    void FindCompelled(std::vector<Edge> &edge_order);
+    // true when the first stored edge is directed (Network.cpp:692-698)
+    bool IsDAG() const;
+    std::string GetValueNameByIndex(int node, int index) const;
+
+protected:
     std::vector<int> topo_ord;
     void SetParentChild(int p, int c);
     void RemoveParentChild(int p, int c);
+};
+
+// A network read from a file (CustomNetwork.h).  LoadBIFFile keeps what structure
+// comparison and parameter learning read: nodes in declaration order with their names,
+// domain sizes and value names, and one arc per parent in each probability block's order.
+class CustomNetwork : public Network {
+public:
+    CustomNetwork() : Network(true) {}
+    void LoadBIFFile(const std::string &path);
+};
+
+// Structural Hamming distance between a true DAG and a learned graph (BNSLComparison.h).
+// A learned CPDAG is compared with the true DAG's CPDAG: the true graph's REVERSIBLE
+// edges are replaced by undirected ones, in place, as the reference does.
+class BNSLComparison {
+public:
+    Network *true_graph;
+    Network *learned_graph;
+    BNSLComparison(Network *t, Network *l) : true_graph(t), learned_graph(l) {}
+    int GetSHD();
+    int GetSHDOneEdge(int index1, int index2);
+};
+
+// Maximum-likelihood parameters of a known structure (ParameterLearning.h); the counts
+// N(AB) of every node come from the device (fpgm_mle_counts).
+class ParameterLearning {
+public:
+    Network *network;
+    double seconds = 0.0;  // "parameter_learning" timer
+    explicit ParameterLearning(Network *net) : network(net) {}
+    void LearnParamsKnowStructCompData(const Dataset *dts, int alpha, bool save_param, const std::string &param_file,
+                                       int verbose);
+    void SaveBNParameter(const std::string &param_file) const;
+    std::string FormatBNParameter() const;
 };
 
 // PC-stable structure learning (PCStable.h:24-62).

@@ -48,6 +48,9 @@ void PCStable::AssignNodeInformation(const Dataset *dts)
     network->num_nodes = dts->num_vars;
     network->node_names = dts->vec_var_names;
     network->node_cards = dts->num_of_possible_values_of_disc_vars;
+    network->possible_values_ids = dts->vars_possible_values_ids;
+    network->possible_values_ids.resize(dts->num_vars);
+    network->pts.clear();
     network->vec_edges.clear();
     network->num_edges = 0;
     network->set_parent_indexes.assign(dts->num_vars, {});
@@ -334,6 +337,8 @@ void PCStable::AddRootNode(const std::vector<int> &sub_roots)
     N.set_children_indexes.emplace_back();
     N.node_names.push_back("ROOT");
     N.node_cards.push_back(2);
+    N.possible_values_ids.resize(r);
+    N.possible_values_ids.push_back({{"0", 0}, {"1", 1}});
     for (int c : sub_roots) {
         N.set_children_indexes[r].insert(c);
         N.set_parent_indexes[c].insert(r);

@@ -1,6 +1,7 @@
-// Command-line driver of the host mirror, the flow of BNSL_PCStable
-// (fastpgm_api.cpp:18-45): load a training set, learn the structure with PC-stable
-// (step 1 on the device), optionally save the structure file.
+// Command-line driver of the host mirror, the flows of BNSL_PCStable and BNL_PCStable
+// (fastpgm_api.cpp:18-75): load a training set, learn the structure with PC-stable
+// (step 1 on the device), optionally save the structure file, compare it with a
+// reference BIF network (SHD) and learn the parameters (counts on the device).
 //
 //   fpgm_pcstable [options] TRAIN_FILE
 //     -alpha A         significance level (0.05)
@@ -17,6 +18,9 @@
 //                      then the uint8 columns (n_rows bytes each), then the names, one a line
 //     --load-only      load (and dump) the training set, learn nothing
 //     --stats          print one JSON line: num_ci_test, edges, step-1 seconds
+//     --ref-bif F      print "SHD = k" against the BIF network F (BNSLComparison)
+//     --param [F]      learn the parameters (Laplace alpha 1) and save them to F
+//                      (reference naming: TRAIN_FILE + "_param")
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -33,7 +37,7 @@ static void usage()
     std::fprintf(stderr,
                  "usage: fpgm_pcstable [-alpha A] [-depth D] [-g N] [-t N] [-o FILE] [-v N] [--libsvm] "
                  "[--int-values] [--dag] [--add-root] [--skeleton-in FILE] [--dump-data FILE] [--load-only] [--stats] "
-                 "TRAIN_FILE\n");
+                 "[--ref-bif FILE] [--param FILE] TRAIN_FILE\n");
 }
 
 static void read_skeleton(const std::string &path, int n_expect, std::vector<uint8_t> &adj,
@@ -88,7 +92,7 @@ int main(int argc, char **argv)
     double alpha = 0.05;
     int depth = 1000, gs = 1, threads = 1, verbose = 0;
     bool libsvm = false, int_values = false, dag = false, add_root = false, stats = false, load_only = false;
-    std::string out, skel, dump, train;
+    std::string out, skel, dump, train, ref_bif, param_file;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -112,6 +116,8 @@ int main(int argc, char **argv)
         else if (a == "--dump-data") dump = next();
         else if (a == "--stats") stats = true;
         else if (a == "--load-only") load_only = true;
+        else if (a == "--ref-bif") ref_bif = next();
+        else if (a == "--param") param_file = next();
         else if (a == "-h" || a == "--help") {
             usage();
             return 0;
@@ -149,6 +155,16 @@ int main(int argc, char **argv)
         if (stats)
             std::printf("{\"num_ci_test\": %d, \"edges\": %d, \"step1_seconds\": %.6f, \"nodes\": %d}\n",
                         pc.num_ci_test, net.num_edges, pc.step1_seconds, net.num_nodes);
+        if (!ref_bif.empty()) {
+            CustomNetwork r_net;
+            r_net.LoadBIFFile(ref_bif);
+            BNSLComparison comp(&r_net, &net);
+            std::cout << "SHD = " << comp.GetSHD() << std::endl;
+        }
+        if (!param_file.empty()) {
+            ParameterLearning pl(&net);
+            pl.LearnParamsKnowStructCompData(&D, 1, true, param_file, verbose);
+        }
     } catch (const std::exception &e) {
         std::fprintf(stderr, "fpgm_pcstable: %s\n", e.what());
         return 1;

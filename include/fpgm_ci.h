@@ -182,6 +182,21 @@ fpgm_status fpgm_sepset_count(const fpgm_ctx *ctx, int64_t *n_records, int64_t *
 fpgm_status fpgm_sepset_get(const fpgm_ctx *ctx, fpgm_sepset *records, int64_t cap_records,
                             int32_t *z, int64_t cap_z);
 
+/*
+ * Maximum-likelihood counts of a known structure: replaces the counting loops of
+ * ParameterLearning::LearnParamsKnowStructCompData (ParameterLearning.cpp:24-69).  Node i
+ * (= variable i, i < n_nodes) has the parents parents[par_off[i] .. par_off[i+1]) in
+ * ascending id order (the reference keeps them in a std::set).  Its table N(AB) holds
+ * card(i) * prod card(parents) int32 counts over every row, indexed like the reference's
+ * PotentialTableBase (PotentialTableBase.cpp:102-130): value(i) most significant, then
+ * the parents in the given order; N(B) (ParameterLearning.cpp:34,64-66) is its sum over
+ * value(i).  count_off (n_nodes + 1 entries) receives the table offsets in counts; with
+ * counts == NULL only count_off is filled (size query).  cap: capacity of counts.
+ * The reference's virtual ROOT parent (random 0/1 per row) has no device counterpart.
+ */
+fpgm_status fpgm_mle_counts(fpgm_ctx *ctx, int32_t n_nodes, const int32_t *par_off, const int32_t *parents,
+                            int64_t *count_off, int32_t *counts, int64_t cap);
+
 /* chi-square helpers of the library's own FP64 CDF (the reference's stats::pchisq,
  * called at IndependenceTest.cpp:146,268,355), exposed for checks. */
 double fpgm_pchisq(double x, int32_t df);

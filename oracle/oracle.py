@@ -72,6 +72,9 @@ def lib():
         L.orc_level_sample.restype = C.c_int64
         L.orc_level_sample.argtypes = [C.POINTER(OrcData), C.c_double, C.c_int, C.c_void_p, C.c_int64, C.c_int,
                                        C.POINTER(C.c_int64)]
+        L.orc_mle.restype = C.c_int64
+        L.orc_mle.argtypes = [C.POINTER(OrcData), C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                              C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -201,3 +204,29 @@ def level_sample(data: Data, alpha: float, d: int, adj_before: np.ndarray, max_e
     t = lib().orc_level_sample(data.ref, float(alpha), int(d), a.ctypes.data, int(max_edges), int(n_threads),
                                C.byref(rem))
     return int(t), int(rem.value)
+
+
+def parent_lists(parents):
+    """CSR form (par_off, flat) of per-node ascending parent lists."""
+    off = np.zeros(len(parents) + 1, dtype=np.int32)
+    flat = []
+    for i, ps in enumerate(parents):
+        ps = sorted(int(p) for p in ps)
+        flat.extend(ps)
+        off[i + 1] = len(flat)
+    return off, np.ascontiguousarray(np.asarray(flat if flat else [0], dtype=np.int32))
+
+
+def mle(data: Data, parents, alpha: int = 1):
+    """Maximum-likelihood tables and Laplace-smoothed potentials of nodes 0..len(parents)-1
+    (ParameterLearning.cpp:24-82): (count_off, counts, potentials)."""
+    off, flat = parent_lists(parents)
+    n = len(parents)
+    count_off = np.zeros(n + 1, dtype=np.int64)
+    total = lib().orc_mle(data.ref, n, off.ctypes.data, flat.ctypes.data, int(alpha), count_off.ctypes.data,
+                          None, None)
+    counts = np.zeros(max(total, 1), dtype=np.int32)
+    pots = np.zeros(max(total, 1), dtype=np.float64)
+    lib().orc_mle(data.ref, n, off.ctypes.data, flat.ctypes.data, int(alpha), count_off.ctypes.data,
+                  counts.ctypes.data, pots.ctypes.data)
+    return count_off, counts[:total], pots[:total]

@@ -602,3 +602,47 @@ int64_t orc_level_sample(const orc_data *D, double alpha, int d, const uint8_t *
     if (removed) *removed = rem;
     return tests;
 }
+
+/* ------------------------------------------------------------------ MLE counts */
+
+/*
+ * ParameterLearning::LearnParamsKnowStructCompData (ParameterLearning.cpp:24-82), the
+ * counting and the Laplace-smoothed estimate, for nodes without the virtual ROOT parent:
+ * N(AB)[index] over every row, N(B)[index % cum_levels[0]] with cum_levels[0] = the
+ * parents' table size (ParameterLearning.cpp:57-68), potential = (N(AB) + alpha) /
+ * (N(B) + alpha * c(node)) (:72-81).  A node without parents has N(B) = number of rows
+ * (:35-40, :75-76).
+ */
+int64_t orc_mle(const orc_data *D, int n_nodes, const int32_t *par_off, const int32_t *parents, int alpha,
+                int64_t *count_off, int32_t *counts, double *potentials)
+{
+    int64_t total = 0;
+    for (int i = 0; i < n_nodes; ++i) {
+        int64_t size = D->card[i];
+        for (int32_t j = par_off[i]; j < par_off[i + 1]; ++j) size *= D->card[parents[j]];
+        count_off[i] = total;
+        total += size;
+    }
+    count_off[n_nodes] = total;
+    if (!counts) return total;
+    for (int i = 0; i < n_nodes; ++i) {
+        int32_t *t = counts + count_off[i];
+        const int64_t size = count_off[i + 1] - count_off[i];
+        const int64_t cum0 = size / D->card[i];  /* cum_levels[0] */
+        memset(t, 0, (size_t)size * sizeof(int32_t));
+        for (int64_t r = 0; r < D->n_rows; ++r) {
+            int64_t idx = D->cols[(int64_t)i * D->stride + r];
+            for (int32_t j = par_off[i]; j < par_off[i + 1]; ++j)
+                idx = idx * D->card[parents[j]] + D->cols[(int64_t)parents[j] * D->stride + r];
+            t[idx]++;
+        }
+        if (!potentials) continue;
+        int32_t *nb = (int32_t *)calloc((size_t)cum0, sizeof(int32_t));
+        for (int64_t k = 0; k < size; ++k) nb[k % cum0] += t[k];
+        for (int64_t k = 0; k < size; ++k)
+            potentials[count_off[i] + k] =
+                ((double)t[k] + alpha) / ((double)nb[k % cum0] + alpha * D->card[i]);
+        free(nb);
+    }
+    return total;
+}

@@ -13,6 +13,8 @@
  *   enumeration src/fastpgm/ChoiceGenerator.cpp:14-85
  *   skeleton    src/fastpgm/structure_learning/PCStable.cpp:84-232,265-626
  *   edges       src/fastpgm/Network.cpp:270-283, src/fastpgm/Edge.cpp:17-28
+ *   MLE counts  src/fastpgm/parameter_learning/ParameterLearning.cpp:24-82
+ *               (table index: src/fastpgm/PotentialTableBase.cpp:102-130)
  *
  * Data layout: uint8 column-major, column v occupies cols[v*stride .. v*stride+n_rows).
  * Value coding is the identity (value k of variable v is state k, k < card[v]).
@@ -86,6 +88,18 @@ int64_t orc_level0_sample(const orc_data *D, double alpha, int64_t n_pairs, int 
 /* Bounded timing sample of level d over the first max_edges edges of adj_before. */
 int64_t orc_level_sample(const orc_data *D, double alpha, int d, const uint8_t *adj_before, int64_t max_edges,
                          int n_threads, int64_t *removed);
+
+/*
+ * Maximum-likelihood tables of a known structure (ParameterLearning.cpp:24-69): node i
+ * (i < n_nodes) with parents parents[par_off[i]..par_off[i+1]) (ascending ids); table
+ * index value(i) * prod c(parents) + mixed-radix parent configuration, first parent most
+ * significant (PotentialTableBase.cpp:102-130).  count_off (n_nodes+1) receives the
+ * table offsets; counts (may be NULL for a size query) the tables back to back.
+ * potentials (may be NULL) receives ((double)N(AB) + alpha) / ((double)N(B) + alpha * c(i))
+ * per entry (ParameterLearning.cpp:72-81).  Returns the total number of cells.
+ */
+int64_t orc_mle(const orc_data *D, int n_nodes, const int32_t *par_off, const int32_t *parents, int alpha,
+                int64_t *count_off, int32_t *counts, double *potentials);
 
 #ifdef __cplusplus
 }

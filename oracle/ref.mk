@@ -16,7 +16,9 @@ SRCS = $(REF)/src/fastpgm/structure_learning/CellTable.cpp \
        $(REF)/src/fastpgm/ContinuousNode.cpp \
        $(REF)/src/fastpgm/Edge.cpp \
        $(REF)/src/fastpgm/common.cpp \
-       $(REF)/src/fastpgm/PotentialTableBase.cpp
+       $(REF)/src/fastpgm/PotentialTableBase.cpp \
+       $(REF)/src/fastpgm/parameter_learning/ParameterLearning.cpp \
+       $(REF)/src/fastpgm/structure_learning/BNSLComparison.cpp
 
 all: $(OUT)/libfpgm_ref.so
 

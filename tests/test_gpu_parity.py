@@ -273,6 +273,26 @@ def test_level0_gram_matches_counting_kernel(n_rows):
     _assert_same(b, o)
 
 
+@pytest.mark.parametrize("card_hi", [8, 12])
+def test_level0_gram_card_paths(card_hi):
+    # cardinalities <= 8 form the one-hot operands by byte-table lookup, larger ones by
+    # byte comparison; both level-0 results equal the counting kernel's and the oracle's.
+    # 5003 rows: a partial last 32-row step in every row split
+    s = datagen.random_sample(77 + card_hi, 40, 5003, 2, card_hi)
+    # a few dependent columns so that level 0 keeps some edges
+    s.cols[1, :5003] = s.cols[0, :5003] % s.card[1]
+    s.cols[3, :5003] = (s.cols[2, :5003] + s.cols[5, :5003]) % s.card[3]
+    D = O.Data.of(s)
+    o = O.pc_skeleton(D, 0.05, 1, n_threads=8)
+    with _ctx(s) as ctx:
+        a = ctx.pc_skeleton(0.05, 1)
+        gs = ctx.gram_stats()
+        b = ctx.pc_skeleton(0.05, 1, flags=_lib.FPGM_F_NO_GRAM)
+    assert gs["launches"] == 1
+    _assert_same(a, o)
+    _assert_same(b, o)
+
+
 def test_mid_tables_many_steps_per_chunk():
     # 300 003 rows and thousands of tests: every chunk spans tens of 4096-row steps, so the
     # 8-bit per-lane counters (tables of 65..256 cells) go through several flushes
