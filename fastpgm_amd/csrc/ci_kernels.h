@@ -31,7 +31,7 @@ struct DevTest {
     int32_t inst;            // dataset instance
     int32_t n_slices;        // z slices of a sliced table (1 otherwise)
     int32_t slice_cells;     // cells per slice (whole z planes)
-    int32_t pad0;
+    int32_t edge;            // skeleton edge of the level (skip checks), -1 for none
     int32_t cx, cy;
     int32_t dimz;            // product of the Z cardinalities
     int32_t n_chunks;        // row chunks of this test (work items)
@@ -70,7 +70,7 @@ struct WideTest {
 struct DevResult {
     float g2_32;      // FP32 compensated G^2
     float abs_sum;    // sum of |terms|, feeds the FP32 error bound
-    int32_t df;
+    int32_t df;       // kSkippedDf: not run, its edge found an independent test in an earlier round
     uint8_t indep, in_band, exact, need_p;
     double g2_64;     // FP64 G^2 (exact/in-band tests)
     double p64;       // FP64 p-value (exact/in-band tests), NaN otherwise
@@ -81,9 +81,10 @@ struct LaunchArgs {
     const DevTest *tests;
     const int32_t *zpool;
     const int32_t *card;
-    const int32_t *item_test;
+    const uint64_t *items;   // work items: test << 32 | (slice * n_chunks + chunk)
     int64_t item_begin;      // this launch's items: [item_begin, item_begin + n_items)
-    int64_t n_items;
+    int64_t n_items;         // (an upper bound when n_items_dev is set)
+    const uint32_t *n_items_dev;  // item count written on the device (compacted rounds), or nullptr
     int64_t rows_per_chunk;
     int32_t *ws;
     uint32_t *done;
@@ -93,7 +94,19 @@ struct LaunchArgs {
     float band;
     uint32_t flags;          // FPGM_F_* of the call
     DevResult *res;
+    // speculative rounds: edge_done[e] = first round in which a test of edge e was found
+    // independent (INT32_MAX while none); a test of this launch (round `round`) whose edge
+    // was done in an earlier round is skipped.  edge_done == nullptr: no skipping.
+    int32_t *edge_done;
+    int32_t round;
+    // work queue of this launch: kXcds counters, zero at launch; workgroups of XCD x take
+    // items of slab x in turn, then of the other slabs
+    uint32_t *queue;
 };
+
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+constexpr int32_t kSkippedDf = -1;
 
 // Counting sinks (one kernel each; a batch launches each sink over its items).
 constexpr int kSinkShared = 0;  // LDS histogram shared by waves (copies per wave when small)
@@ -104,10 +117,28 @@ constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) pe
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 
+// Compaction of a round's work items before its counting launches: items of tests whose
+// edge was done in an earlier round are dropped (their result is marked skipped), the rest
+// keep their order within each 256-item tile; count[g] receives group g's live items.
+struct CompactArgs {
+    const uint64_t *in;
+    uint64_t *out;
+    const DevTest *tests;
+    const int32_t *edge_done;
+    int32_t round;
+    DevResult *res;
+    int64_t begin[5];  // per sink group: first item (in and out use the same group regions)
+    int64_t len[5];
+    uint32_t *count;   // [5], zero before the launch
+};
+
 hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s);
+hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s);
 hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
                                 const WideTest *wt, int64_t n_wide, hipStream_t s);
-hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s);
+// p-values of the rechecked tests; with edge_done, an independent one marks its edge (round).
+hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests, int32_t *edge_done,
+                         int32_t round, hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);
 

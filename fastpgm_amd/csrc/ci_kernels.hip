@@ -505,6 +505,33 @@ __device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, f
     return need64;
 }
 
+// A test decided independent marks its edge with the launch's round: later rounds skip
+// the edge's tests, tests of the same round never skip one another.
+__device__ __forceinline__ void mark_edge(const LaunchArgs &a, const DevTest &T, const DevResult &R)
+{
+    if (a.edge_done && T.edge >= 0 && R.indep && !R.need_p) atomicMin(&a.edge_done[T.edge], a.round);
+}
+
+__device__ __forceinline__ bool skipped(const LaunchArgs &a, const DevTest &T)
+{
+    return a.edge_done && T.edge >= 0 && a.edge_done[T.edge] < a.round;
+}
+
+__device__ __forceinline__ DevResult skipped_result()
+{
+    DevResult R;
+    R.g2_32 = 0.f;
+    R.abs_sum = 0.f;
+    R.df = kSkippedDf;
+    R.indep = 0;
+    R.in_band = 0;
+    R.exact = 0;
+    R.need_p = 0;
+    R.g2_64 = NAN;
+    R.p64 = NAN;
+    return R;
+}
+
 // Statistic of test t from its full table (LDS or global) with marginal scratch
 // marg = [ni: dimz*cx | nj: dimz*cy | nk: dimz].  Tables with many z slices are
 // processed one slice per thread in a single pass; small ones cell-parallel.
@@ -629,6 +656,7 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
             R.need_p = (sh.df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
         }
     }
+    if (tid == 0) mark_edge(a, T, a.res[t]);
     __syncthreads();
 }
 
@@ -647,19 +675,45 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
     __shared__ Shared sh;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
-    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so give
-    // each XCD a contiguous slab of work items (neighbouring tests share columns).
-    const int64_t nx = 8;
-    const int64_t xcd = blockIdx.x % nx, local = blockIdx.x / nx, per = gridDim.x / nx;
-    const int64_t slab = (a.n_items + nx - 1) / nx;
-    const int64_t lo = a.item_begin + xcd * slab;
-    const int64_t hi = (xcd * slab + slab < a.n_items) ? lo + slab : a.item_begin + a.n_items;
-
-    for (int64_t item = lo + local; item < hi; item += per) {
-        const int t = a.item_test[item];
+    // XCD-aware dynamic order: workgroups are dispatched round-robin over the XCDs, so the
+    // items are cut into one contiguous slab per XCD (neighbouring tests share columns
+    // through that XCD's L2) and the workgroups of an XCD take its slab's items from an
+    // atomic counter, then help the other slabs.  Item costs differ (table sinks, |Z|,
+    // tests skipped because their edge is done), so a static split leaves the launch
+    // waiting for its most loaded workgroup.
+    const int home = blockIdx.x % kXcds;
+    const int64_t n_items = a.n_items_dev ? (int64_t)*a.n_items_dev : a.n_items;
+    const int64_t slab = (n_items + kXcds - 1) / kXcds;
+    __shared__ int64_t s_item[2];
+    int tried = 0;  // slabs exhausted so far (thread 0's state)
+    auto fetch = [&]() -> int64_t {
+        while (tried < kXcds) {
+            const int x = (home + tried) % kXcds;
+            const int64_t lo = (int64_t)x * slab;
+            const int64_t hi = (lo + slab < n_items) ? lo + slab : n_items;
+            if (lo < hi) {
+                const int64_t i = lo + (int64_t)atomicAdd(&a.queue[x], 1u);
+                if (i < hi) return a.item_begin + i;
+            }
+            ++tried;
+        }
+        return -1;
+    };
+    if (tid == 0) s_item[0] = fetch();
+    for (int it = 0;; ++it) {
+        __syncthreads();  // s_item[it & 1] written, s_item[(it + 1) & 1] read by everyone
+        const int64_t item = s_item[it & 1];
+        if (item < 0) break;
+        if (tid == 0) s_item[(it + 1) & 1] = fetch();
+        const uint64_t pk = a.items[item];
+        const int t = (int)(pk >> 32);
+        const int64_t local = (int64_t)(uint32_t)pk;  // slice * n_chunks + chunk
         const DevTest T = a.tests[t];
+        if (skipped(a, T)) {  // uniform over the workgroup: the flag cannot drop below round meanwhile
+            if (tid == 0 && local == 0) a.res[t] = skipped_result();
+            continue;
+        }
         const DevInst I = a.inst[T.inst];
-        const int64_t local = item - T.item0;  // slice * n_chunks + chunk
         const int64_t slice = (kMode == kSinkSlice) ? local / T.n_chunks : 0;
         const int64_t chunk = local - slice * T.n_chunks;
         const int64_t r_begin = chunk * a.rows_per_chunk;
@@ -786,6 +840,40 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
     }
 }
 
+// Round compaction (one 256-item tile of one sink group per workgroup): a wave ballot of
+// the live items, their ranks within the tile, one atomic reservation per tile.
+__global__ __launch_bounds__(kWG) void k_compact(CompactArgs c)
+{
+    __shared__ uint32_t s_wave[kWG / 64];
+    __shared__ uint32_t s_base;
+    const int g = blockIdx.y, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t i = (int64_t)blockIdx.x * kWG + tid;
+    bool live = false;
+    uint64_t pk = 0;
+    if (i < c.len[g]) {
+        pk = c.in[c.begin[g] + i];
+        const int t = (int)(pk >> 32);
+        const int32_t e = c.tests[t].edge;
+        live = !(e >= 0 && c.edge_done[e] < c.round);
+        if (!live && (uint32_t)pk == 0u) c.res[t] = skipped_result();
+    }
+    const uint64_t mask = __ballot(live);
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kWG / 64; ++w) {
+            const uint32_t v = s_wave[w];
+            s_wave[w] = total;
+            total += v;
+        }
+        s_base = total ? atomicAdd(&c.count[g], total) : 0u;
+    }
+    __syncthreads();
+    if (live) c.out[c.begin[g] + s_base + s_wave[wave] + rank] = pk;
+}
+
 // Grid-wide finalize, part 1: the z slices [z0, z1) of one test's device-memory table,
 // one slice per thread.  Marginals go to the test's scratch behind the table; FP32 and
 // FP64 terms (same order within a slice as IndependenceTest.cpp:112-137) and df are
@@ -797,6 +885,7 @@ __global__ __launch_bounds__(kWG) void k_fin_wide_part(LaunchArgs a, const WideI
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const WideItem it = items[blockIdx.x];
     const DevTest T = a.tests[it.t];
+    if (skipped(a, T)) return;  // the table was never counted: it is still zero
     const int cx = T.cx, cy = T.cy, cxy = cx * cy;
     int32_t *tab = a.ws + T.ws_off;
     int32_t *ni = tab + (int64_t)T.dimz * cxy;
@@ -857,6 +946,10 @@ __global__ __launch_bounds__(kWG) void k_fin_wide_reduce(LaunchArgs a, const Wid
     if (k >= n_wide) return;
     const WideTest W = wt[k];
     const DevTest T = a.tests[W.t];
+    if (skipped(a, T)) {
+        a.res[W.t] = skipped_result();
+        return;
+    }
     int df = 0;
     float S = 0.f, C = 0.f, AS = 0.f;
     double G = 0.0;
@@ -876,6 +969,7 @@ __global__ __launch_bounds__(kWG) void k_fin_wide_reduce(LaunchArgs a, const Wid
         R.need_p = (df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
     }
     a.res[W.t] = R;
+    mark_edge(a, T, R);
 }
 
 __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows,
@@ -899,7 +993,8 @@ __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64
 
 // FP64 p-value of the tests the statistic kernel rechecked (one thread per test):
 // p = 1 - pchisq(G^2, df), independent iff p > alpha (IndependenceTest.cpp:146-154).
-__global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, double alpha)
+__global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests,
+                                                 int32_t *edge_done, int32_t round)
 {
     const int64_t t = (int64_t)blockIdx.x * kWG + threadIdx.x;
     if (t >= n) return;
@@ -909,6 +1004,10 @@ __global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, doubl
     R.p64 = p;
     R.indep = (p > alpha) ? 1 : 0;
     R.need_p = 0;
+    if (edge_done && tests && R.indep) {
+        const int32_t e = tests[t].edge;
+        if (e >= 0) atomicMin(&edge_done[e], round);
+    }
 }
 
 }  // namespace
@@ -926,6 +1025,13 @@ hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_
     return hipGetLastError();
 }
 
+hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s)
+{
+    if (max_len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((max_len + kWG - 1) / kWG), 5), dim3(kWG), 0, s, c);
+    return hipGetLastError();
+}
+
 hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
                                 const WideTest *wt, int64_t n_wide, hipStream_t s)
 {
@@ -938,11 +1044,12 @@ hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int6
     return hipGetLastError();
 }
 
-hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, hipStream_t s)
+hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests, int32_t *edge_done,
+                         int32_t round, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int64_t blocks = (n + kWG - 1) / kWG;
-    hipLaunchKernelGGL(k_pvalue, dim3((unsigned)blocks), dim3(kWG), 0, s, res, n, alpha);
+    hipLaunchKernelGGL(k_pvalue, dim3((unsigned)blocks), dim3(kWG), 0, s, res, n, alpha, tests, edge_done, round);
     return hipGetLastError();
 }
 

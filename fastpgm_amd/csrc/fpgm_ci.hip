@@ -30,6 +30,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+
+#include <omp.h>
 #include <string>
 #include <vector>
 
@@ -41,6 +43,7 @@ constexpr int64_t kMaxCells = (int64_t)1 << 31;   // dense-table limit (the refe
 constexpr int64_t kWsInit = (int64_t)1 << 24;     // workspace, int32 entries
 constexpr int64_t kWsMax = (int64_t)1 << 30;      // 4 GiB of workspace at most per launch
 constexpr int kMaxD = kMaxZLds;
+constexpr int64_t kParallelTests = 16384;          // rounds this large are built on several host threads
 
 template <typename T>
 struct DevBuf {
@@ -122,18 +125,21 @@ constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSl
 // slots launch on the context's one stream and every launch leaves them zero.
 struct Slot {
     DevBuf<DevTest> d_tests;
-    DevBuf<int32_t> d_zpool, d_item_test;
+    DevBuf<int32_t> d_zpool;
+    DevBuf<uint64_t> d_items, d_items_live;  // work items as built, and after compaction
     DevBuf<DevResult> d_res;
+    DevBuf<uint32_t> d_queue;  // work-queue counters (kGroups x kXcds) and live-item counts (kGroups)
     DevBuf<WideItem> d_wide_items;
     DevBuf<WidePart> d_wide_parts;
     DevBuf<WideTest> d_wide_tests;
     HostBuf<DevTest> h_tests;
-    HostBuf<int32_t> h_items, h_zpool;
+    HostBuf<int32_t> h_zpool;
+    HostBuf<uint64_t> h_items;  // test << 32 | (slice * n_chunks + chunk)
     HostBuf<DevResult> h_res;
     HostBuf<WideItem> h_wide_items;
     HostBuf<WideTest> h_wide_tests;
-    std::vector<DevTest> dt;
     std::vector<int32_t> mode_of;
+    std::vector<int64_t> need;  // workspace int32 entries of each test of the batch being built
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
     hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing
     bool pending = false;
@@ -157,8 +163,10 @@ struct Slot {
     {
         d_tests.release();
         d_zpool.release();
-        d_item_test.release();
+        d_items.release();
+        d_items_live.release();
         d_res.release();
+        d_queue.release();
         d_wide_items.release();
         d_wide_parts.release();
         d_wide_tests.release();
@@ -179,6 +187,7 @@ struct HostTest {
     int32_t x, y, d;
     int32_t inst;   // dataset instance
     int64_t z_off;  // into the host z pool
+    int32_t edge = -1;  // index of the test's edge in the level (device skip checks), -1 for none
 };
 
 struct InstHost {
@@ -201,11 +210,15 @@ struct fpgm_ctx {
     std::string err;
     std::mutex mu;
     int n_cu = 256;
+    int host_threads = 1;  // host threads for building large rounds
     struct {
         double base = 2.0;    // round ramp factor
         int64_t cap = 8;      // ramp cap (tests per edge per round); larger rounds come from the budget
         int64_t promote = 0;  // dependent tests after which an edge runs the rest of its walk
-        int lanes = 1;        // edge lanes whose rounds alternate (host work overlaps device work)
+        double round_bytes = 2147483648.0;  // algorithmic bytes a round aims at (tests = bytes / ((d+2) N))
+        int64_t items_per_cu = 16;          // work items a launch aims at per CU (row chunks per test)
+        int64_t wpc[5] = {4, 2, 4, 2, 8};   // grid workgroups per CU: lane, lane-wide, shared, slice, device
+        int pipeline = 1;                    // build round r+1 while the device counts round r
     } sched;
     int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
     uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices
@@ -217,6 +230,10 @@ struct fpgm_ctx {
 
     Slot slot[2];
     DevBuf<int32_t> d_ws;
+    // speculative rounds of a level: round number of the launch being submitted (0 = no
+    // skip checks) and, per edge of the level, the first round that found it independent
+    int32_t skip_round = 0;
+    DevBuf<int32_t> d_edge_done;
     DevBuf<uint32_t> d_done, d_flag;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -226,8 +243,11 @@ struct fpgm_ctx {
     int64_t k_launches = 0;
 
     // level-0 Gram path
-    DevBuf<int32_t> d_C, d_rowvar, d_tiles, d_off, d_pairs;
-    DevBuf<uint8_t> d_rowstate;
+    DevBuf<int32_t> d_C, d_rowvar, d_tiles, d_off;
+    DevBuf<uint8_t> d_rowstate, d_l0_flags;
+    DevBuf<PairSummary> d_l0_sum;
+    HostBuf<uint8_t> h_l0_flags;
+    HostBuf<PairSummary> h_l0_sum;
     double g_ms = 0.0, g_ops = 0.0, g_pair_ms = 0.0;
     int64_t g_launches = 0;
 
@@ -299,7 +319,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     const int64_t steps = std::max<int64_t>(1, (N + kRowsPerStep - 1) / kRowsPerStep);
 
     // rows per chunk, common to the launch: enough work items to fill every CU
-    const int64_t target_items = (int64_t)ctx->n_cu * 16;
+    const int64_t target_items = (int64_t)ctx->n_cu * ctx->sched.items_per_cu;
     int64_t rpc;
     if (rows_per_chunk_forced > 0) {
         rpc = (rows_per_chunk_forced + kRowsPerStep - 1) / kRowsPerStep * kRowsPerStep;
@@ -310,19 +330,28 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     }
     rpc = std::min<int64_t>(rpc, kLaneMaxRowsPerChunk);
 
-    std::vector<DevTest> &dt = S.dt;
-    std::vector<int32_t> &mode_of = S.mode_of;
-    dt.clear();
-    mode_of.clear();
-    int64_t ws_used = 0, n_items_total = 0;
-    int64_t i1 = i0;
-    for (; i1 < n; ++i1) {
-        const HostTest &h = tests[i1];
+    // descriptors go straight into the slot's pinned buffer.  Per test, in parallel: its
+    // descriptor, sink group (gidx), work items and workspace need; then, in order, the
+    // workspace offsets (a launch sequence ends where the workspace limit is reached) and
+    // each test's first work item within its group.
+    const int64_t m = n - i0;
+    HIP_TRY(S.h_tests.ensure(m, ctx->stream), "alloc pinned tests");
+    DevTest *dt = S.h_tests.p;
+    std::vector<int32_t> &gidx = S.mode_of;
+    std::vector<int64_t> &need = S.need;
+    gidx.resize((size_t)m);
+    need.resize((size_t)m);
+    int too_big = 0;
+    const int nth = (m >= kParallelTests) ? ctx->host_threads : 1;
+#pragma omp parallel for num_threads(nth) schedule(static) reduction(| : too_big)
+    for (int64_t k = 0; k < m; ++k) {
+        const HostTest &h = tests[i0 + k];
         DevTest T{};
         T.x = h.x;
         T.y = h.y;
         T.d = h.d;
         T.inst = h.inst;
+        T.edge = h.edge;
         const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[h.inst].n_rows + rpc - 1) / rpc);
         T.z_off = (int32_t)h.z_off;
         T.cx = ctx->card[h.x];
@@ -333,7 +362,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             if (dimz > kMaxCells) break;
         }
         const int64_t cells = dimz * T.cx * T.cy;
-        if (dimz > kMaxCells || cells >= kMaxCells) return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
+        if (dimz > kMaxCells || cells >= kMaxCells) {
+            too_big = 1;
+            continue;
+        }
         T.dimz = (int32_t)dimz;
         T.n_chunks = (int32_t)n_chunks;
         T.n_slices = 1;
@@ -341,20 +373,20 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         const int64_t marg = dimz * (T.cx + T.cy + 1);
         T.flags = tflags;
         bool fits;
-        int mode;
+        int gi;
         if (cells <= kLaneCells) {
-            mode = kSinkLane;
+            gi = 0;  // kSinkLane
             T.flags |= kTFLane;
             fits = cells + marg <= kSmallCells;
         } else if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
-            mode = kSinkLaneW;
+            gi = 1;  // kSinkLaneW
             fits = cells + marg <= kSmallCells;
         } else if (cells <= kLdsCells) {
-            mode = kSinkShared;
+            gi = 2;  // kSinkShared
             T.flags |= kTFLdsTable;
             fits = cells + marg <= kLdsCells;
         } else {
-            mode = kSinkGlobal;
+            gi = 4;  // kSinkGlobal
             fits = false;
             // large device-memory tables get their statistic from the grid-wide
             // finalize kernels (the counted table of a parity read stays in place)
@@ -364,7 +396,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                 const int64_t sc = (kSliceCells / cxy) * cxy;
                 const int64_t ns = (cells + sc - 1) / sc;
                 if (ns <= ctx->max_slices) {
-                    mode = kSinkSlice;
+                    gi = 3;  // kSinkSlice
                     T.n_slices = (int32_t)ns;
                     T.slice_cells = (int32_t)sc;
                     T.flags &= ~kTFWide;
@@ -374,36 +406,49 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fits;
         if (!need_ws) T.flags |= kTFLdsMarg;
         T.ws_off = -1;
-        if (need_ws) {
-            const int64_t need = (cells + marg + 63) / 64 * 64;
-            if (ws_used + need > kWsMax && i1 > i0) break;
-            if (need > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test workspace exceeds 4 GiB");
+        need[k] = need_ws ? (cells + marg + 63) / 64 * 64 : 0;
+        gidx[k] = gi;
+        dt[k] = T;
+    }
+    if (too_big) return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
+    int64_t ws_used = 0, n_items_total = 0, nb = 0;
+    int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
+    S.bytes = 0.0;
+    for (; nb < m; ++nb) {
+        DevTest &T = dt[nb];
+        if (need[nb] > 0) {
+            if (ws_used + need[nb] > kWsMax && nb > 0) break;
+            if (need[nb] > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test workspace exceeds 4 GiB");
             T.ws_off = ws_used;
-            ws_used += need;
+            ws_used += need[nb];
         }
-        dt.push_back(T);
-        mode_of.push_back(mode);
-        n_items_total += n_chunks * T.n_slices;
+        const int64_t cnt = (int64_t)T.n_chunks * T.n_slices;
+        T.item0 = (int32_t)grp_len[gidx[nb]];  // within the group; shifted by the group start below
+        grp_len[gidx[nb]] += cnt;
+        n_items_total += cnt;
+        S.bytes += (double)(T.d + 2) * (double)ctx->inst[T.inst].n_rows;
         if (n_items_total > ((int64_t)1 << 30)) {
-            ++i1;
+            ++nb;
             break;
         }
     }
-    const int64_t nb = (int64_t)dt.size();
+    const int64_t i1 = i0 + nb;
     // work items grouped by sink, one launch per group; within a group, tests keep the
     // caller's order (neighbouring tests share columns)
-    int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
-    HIP_TRY(S.h_items.ensure(n_items_total, ctx->stream), "alloc pinned items");
-    int64_t ni = 0;
+    HIP_TRY(S.h_items.ensure(std::max<int64_t>(n_items_total, 1), ctx->stream), "alloc pinned items");
+    int64_t acc = 0;
     for (int gi = 0; gi < kGroups; ++gi) {
-        grp_begin[gi] = ni;
-        for (int64_t k = 0; k < nb; ++k) {
-            if (mode_of[k] != kGroupSink[gi]) continue;
-            dt[k].item0 = (int32_t)ni;
-            for (int64_t c = 0; c < (int64_t)dt[k].n_chunks * dt[k].n_slices; ++c) S.h_items.p[ni++] = (int32_t)k;
-        }
-        grp_len[gi] = ni - grp_begin[gi];
+        grp_begin[gi] = acc;
+        acc += grp_len[gi];
     }
+    const int nth2 = (nb >= kParallelTests) ? ctx->host_threads : 1;
+#pragma omp parallel for num_threads(nth2) schedule(static)
+    for (int64_t k = 0; k < nb; ++k) {
+        dt[k].item0 += (int32_t)grp_begin[gidx[k]];
+        const int64_t cnt = (int64_t)dt[k].n_chunks * dt[k].n_slices;
+        for (int64_t j = 0; j < cnt; ++j) S.h_items.p[dt[k].item0 + j] = ((uint64_t)k << 32) | (uint64_t)j;
+    }
+    const int64_t ni = n_items_total;
     // device-memory tables: z ranges for the grid-wide finalize
     int64_t n_wi = 0, n_wt = 0;
     for (int64_t k = 0; k < nb; ++k)
@@ -434,30 +479,32 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                                ctx->stream),
                 "upload finalize tests");
     }
-    HIP_TRY(S.h_tests.ensure(nb, ctx->stream), "alloc pinned tests");
-    std::copy(dt.begin(), dt.end(), S.h_tests.p);
     HIP_TRY(S.h_zpool.ensure(std::max<int64_t>(zlen, 1), ctx->stream), "alloc pinned z pool");
     if (zlen > 0) std::copy(zpool, zpool + zlen, S.h_zpool.p);
     HIP_TRY(S.d_zpool.ensure(std::max<int64_t>(zlen, 1), false, ctx->stream), "alloc z pool");
     HIP_TRY(S.d_tests.ensure(nb, false, ctx->stream), "alloc tests");
-    HIP_TRY(S.d_item_test.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
+    HIP_TRY(S.d_items.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
+    if (ctx->skip_round > 0) HIP_TRY(S.d_items_live.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
     HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
     HIP_TRY(ctx->d_done.ensure(std::max<int64_t>(nb, 1024), true, ctx->stream), "alloc counters");
     HIP_TRY(S.d_res.ensure(nb, false, ctx->stream), "alloc results");
+    HIP_TRY(S.d_queue.ensure(kGroups * (kXcds + 1), false, ctx->stream), "alloc work queues");
+    HIP_TRY(hipMemsetAsync(S.d_queue.p, 0, sizeof(uint32_t) * kGroups * (kXcds + 1), ctx->stream),
+            "clear work queues");
     HIP_TRY(S.h_res.ensure(nb, ctx->stream), "alloc pinned results");
     if (zlen > 0)
         HIP_TRY(hipMemcpyAsync(S.d_zpool.p, S.h_zpool.p, sizeof(int32_t) * zlen, hipMemcpyHostToDevice, ctx->stream),
                 "upload z pool");
     HIP_TRY(hipMemcpyAsync(S.d_tests.p, S.h_tests.p, sizeof(DevTest) * nb, hipMemcpyHostToDevice, ctx->stream),
             "upload tests");
-    HIP_TRY(hipMemcpyAsync(S.d_item_test.p, S.h_items.p, sizeof(int32_t) * ni, hipMemcpyHostToDevice, ctx->stream),
+    HIP_TRY(hipMemcpyAsync(S.d_items.p, S.h_items.p, sizeof(uint64_t) * ni, hipMemcpyHostToDevice, ctx->stream),
             "upload items");
     LaunchArgs a{};
     a.inst = ctx->d_inst;
     a.tests = S.d_tests.p;
     a.zpool = S.d_zpool.p;
     a.card = ctx->d_card;
-    a.item_test = S.d_item_test.p;
+    a.items = S.d_items.p;
     a.rows_per_chunk = rpc;
     a.ws = ctx->d_ws.p;
     a.done = ctx->d_done.p;
@@ -467,16 +514,40 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     a.band = band > 0.f ? band : 0.02f;
     a.flags = flags;
     a.res = S.d_res.p;
+    a.edge_done = ctx->skip_round > 0 ? ctx->d_edge_done.p : nullptr;
+    a.round = ctx->skip_round;
     ctx->t_prep += now_s() - tp0;
     HIP_TRY(hipEventRecord(S.ev0, ctx->stream), "event");
+    uint32_t *live_count = S.d_queue.p + kGroups * kXcds;
+    if (a.edge_done) {
+        // a speculative round: drop the items of tests whose edge an earlier round decided
+        CompactArgs c{};
+        c.in = S.d_items.p;
+        c.out = S.d_items_live.p;
+        c.tests = S.d_tests.p;
+        c.edge_done = a.edge_done;
+        c.round = a.round;
+        c.res = S.d_res.p;
+        int64_t max_len = 0;
+        for (int gi = 0; gi < kGroups; ++gi) {
+            c.begin[gi] = grp_begin[gi];
+            c.len[gi] = grp_len[gi];
+            max_len = std::max(max_len, grp_len[gi]);
+        }
+        c.count = live_count;
+        HIP_TRY(launch_compact(c, max_len, ctx->stream), "launch compact");
+        a.items = S.d_items_live.p;
+    }
     for (int gi = 0; gi < kGroups; ++gi) {
         if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[gi], ctx->stream), "event");
         if (grp_len[gi] == 0) continue;
         a.item_begin = grp_begin[gi];
         a.n_items = grp_len[gi];
+        a.queue = S.d_queue.p + gi * kXcds;
+        a.n_items_dev = a.edge_done ? live_count + gi : nullptr;
         const int sink = kGroupSink[gi];
-        const int64_t waves_per_cu = (sink == kSinkGlobal) ? 8 : (sink == kSinkLaneW || sink == kSinkSlice) ? 2 : 4;
-        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * waves_per_cu);
+        const int64_t wg_per_cu = ctx->sched.wpc[gi];
+        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wg_per_cu);
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
         HIP_TRY(launch_count_stat(a, sink, (int)grid, ctx->stream), "launch count_stat");
         ctx->k_launches += 1;
@@ -489,13 +560,11 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     }
     if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[kGroups], ctx->stream), "event");
     HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
-    HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, ctx->stream), "launch pvalue");
+    HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, S.d_tests.p, a.edge_done, a.round, ctx->stream), "launch pvalue");
     HIP_TRY(hipMemcpyAsync(S.h_res.p, S.d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost, ctx->stream),
             "download results");
     HIP_TRY(hipEventRecord(S.ev_done, ctx->stream), "event");
     S.n = nb;
-    S.bytes = 0.0;
-    for (int64_t k = i0; k < i1; ++k) S.bytes += (double)(tests[k].d + 2) * (double)ctx->inst[tests[k].inst].n_rows;
     if (ctx->trace) {
         for (int gi = 0; gi < kGroups; ++gi) {
             Slot::GroupInfo &G = S.grp[gi];
@@ -503,9 +572,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             G.items = grp_len[gi];
         }
         for (int64_t k = 0; k < nb; ++k) {
-            int gi = 0;
-            while (kGroupSink[gi] != mode_of[k]) ++gi;
-            Slot::GroupInfo &G = S.grp[gi];
+            Slot::GroupInfo &G = S.grp[gidx[k]];
             const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
             G.tests += 1;
             G.dsum += dt[k].d;
@@ -582,12 +649,12 @@ void reset_kernel_stats(fpgm_ctx *ctx)
     ctx->g_launches = 0;
 }
 
-// Level 0 of one instance on the MFMA units: C = O^T O, then one thread per pair
-// (pairs in tv order, all of instance `inst`).  Returns false in *used when the
-// one-hot width is too large for a dense C (the caller then counts pair by pair).
-// Results land in the pinned buffer of slot 0; *out points at them.
-fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest> &tv, double alpha, uint32_t flags,
-                            float band, const DevResult **out, bool *used)
+// Level 0 of one instance on the MFMA units: C = O^T O, then one thread per pair, every
+// pair x < y in lexicographic order.  Returns false in *used when the one-hot width is too
+// large for a dense C (the caller then counts pair by pair).  Results: one flag byte per
+// pair (bit 0 independent, bit 1 in the recheck band) in ctx->h_l0_flags, and the summary
+// in ctx->h_l0_sum.
+fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flags, float band, bool *used)
 {
     *used = false;
     Slot &S = ctx->slot[0];
@@ -625,20 +692,17 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     const int64_t splits = std::min<int64_t>(ksteps, std::max<int64_t>(1, (target + n_tp - 1) / n_tp));
     const int64_t kps = (ksteps + splits - 1) / splits * 32;
     const int64_t n_items = n_tp * splits;
-    const int64_t np = (int64_t)tv.size();
-    std::vector<int32_t> pairs((size_t)2 * np);
-    for (int64_t k = 0; k < np; ++k) {
-        pairs[k] = tv[k].x;
-        pairs[np + k] = tv[k].y;
-    }
+    const int64_t np = (int64_t)n * (n - 1) / 2;
     HIP_TRY(ctx->d_C.ensure(Wp * Wp, false, ctx->stream), "alloc gram");
     HIP_TRY(ctx->d_rowvar.ensure(Wp, false, ctx->stream), "alloc gram rows");
     HIP_TRY(ctx->d_rowstate.ensure(Wp, false, ctx->stream), "alloc gram rows");
     HIP_TRY(ctx->d_tiles.ensure((int64_t)tiles.size(), false, ctx->stream), "alloc gram tiles");
     HIP_TRY(ctx->d_off.ensure(n + 1, false, ctx->stream), "alloc gram offsets");
-    HIP_TRY(ctx->d_pairs.ensure(2 * np, false, ctx->stream), "alloc pairs");
     HIP_TRY(S.d_res.ensure(np, false, ctx->stream), "alloc results");
-    HIP_TRY(S.h_res.ensure(np, ctx->stream), "alloc pinned results");
+    HIP_TRY(ctx->d_l0_flags.ensure(np, false, ctx->stream), "alloc level-0 flags");
+    HIP_TRY(ctx->h_l0_flags.ensure(np, ctx->stream), "alloc pinned level-0 flags");
+    HIP_TRY(ctx->d_l0_sum.ensure(1, false, ctx->stream), "alloc level-0 summary");
+    HIP_TRY(ctx->h_l0_sum.ensure(1, ctx->stream), "alloc pinned level-0 summary");
     HIP_TRY(hipMemcpyAsync(ctx->d_rowvar.p, rowvar.data(), sizeof(int32_t) * Wp, hipMemcpyHostToDevice, ctx->stream),
             "upload gram rows");
     HIP_TRY(hipMemcpyAsync(ctx->d_rowstate.p, rowstate.data(), Wp, hipMemcpyHostToDevice, ctx->stream),
@@ -648,8 +712,6 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
             "upload gram tiles");
     HIP_TRY(hipMemcpyAsync(ctx->d_off.p, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream),
             "upload offsets");
-    HIP_TRY(hipMemcpyAsync(ctx->d_pairs.p, pairs.data(), sizeof(int32_t) * 2 * np, hipMemcpyHostToDevice, ctx->stream),
-            "upload pairs");
     HIP_TRY(hipMemsetAsync(ctx->d_C.p, 0, sizeof(int32_t) * Wp * Wp, ctx->stream), "clear gram");
     GramArgs g{};
     g.cols = I.cols;
@@ -668,8 +730,9 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     p.Wp = (int32_t)Wp;
     p.off = ctx->d_off.p;
     p.card = ctx->d_card;
-    p.px = ctx->d_pairs.p;
-    p.py = ctx->d_pairs.p + np;
+    p.px = nullptr;
+    p.py = nullptr;
+    p.n_vars = n;
     p.n_pairs = np;
     p.crit = ctx->d_crit;
     p.alpha = alpha;
@@ -684,10 +747,13 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, const std::vector<HostTest>
     HIP_TRY(hipEventRecord(e_mid, ctx->stream), "event");
     HIP_TRY(launch_pairs_l0(p, ctx->stream), "launch pairs");
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
-    HIP_TRY(launch_pvalue(S.d_res.p, np, alpha, ctx->stream), "launch pvalue");
-    *out = S.h_res.p;
-    HIP_TRY(hipMemcpyAsync(S.h_res.p, S.d_res.p, sizeof(DevResult) * np, hipMemcpyDeviceToHost, ctx->stream),
-            "download results");
+    HIP_TRY(launch_pvalue(S.d_res.p, np, alpha, nullptr, nullptr, 0, ctx->stream), "launch pvalue");
+    HIP_TRY(hipMemsetAsync(ctx->d_l0_sum.p, 0, sizeof(PairSummary), ctx->stream), "clear level-0 summary");
+    HIP_TRY(launch_pack_l0(S.d_res.p, np, ctx->d_l0_flags.p, ctx->d_l0_sum.p, ctx->stream), "launch pack");
+    HIP_TRY(hipMemcpyAsync(ctx->h_l0_flags.p, ctx->d_l0_flags.p, (size_t)np, hipMemcpyDeviceToHost, ctx->stream),
+            "download level-0 flags");
+    HIP_TRY(hipMemcpyAsync(ctx->h_l0_sum.p, ctx->d_l0_sum.p, sizeof(PairSummary), hipMemcpyDeviceToHost, ctx->stream),
+            "download level-0 summary");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
     float ms_g = 0.f, ms_p = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms_g, ctx->ev0, e_mid), "event time");
@@ -853,11 +919,20 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         return fail(ctx, FPGM_EDEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
     ctx->dev = dev;
     ctx->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    ctx->host_threads = std::max(1, std::min(8, omp_get_max_threads()));
     if (const char *e = std::getenv("FPGM_ROUND_BASE")) ctx->sched.base = std::max(1.0, std::atof(e));
     if (const char *e = std::getenv("FPGM_ROUND_CAP")) ctx->sched.cap = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_ROUND_PROMOTE")) ctx->sched.promote = std::max<int64_t>(0, std::atoll(e));
-    if (const char *e = std::getenv("FPGM_LANES")) ctx->sched.lanes = std::atoi(e) == 2 ? 2 : 1;
     if (const char *e = std::getenv("FPGM_MAX_SLICES")) ctx->max_slices = std::max<int64_t>(1, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_PIPELINE")) ctx->sched.pipeline = std::atoi(e) != 0;
+    if (const char *e = std::getenv("FPGM_ROUND_BYTES")) ctx->sched.round_bytes = std::max(1e6, std::atof(e));
+    if (const char *e = std::getenv("FPGM_ITEMS_PER_CU"))
+        ctx->sched.items_per_cu = std::max<int64_t>(1, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_WPC")) {  // five comma-separated values, in sink-group order
+        int64_t v[kGroups];
+        if (std::sscanf(e, "%ld,%ld,%ld,%ld,%ld", &v[0], &v[1], &v[2], &v[3], &v[4]) == kGroups)
+            for (int g = 0; g < kGroups; ++g) ctx->sched.wpc[g] = std::min<int64_t>(64, std::max<int64_t>(1, v[g]));
+    }
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -998,11 +1073,15 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     ctx->d_rowstate.release();
     ctx->d_tiles.release();
     ctx->d_off.release();
-    ctx->d_pairs.release();
+    ctx->d_l0_flags.release();
+    ctx->d_l0_sum.release();
+    ctx->h_l0_flags.release();
+    ctx->h_l0_sum.release();
     ctx->d_mle_nodes.release();
     ctx->d_mle_rel.release();
     ctx->d_mle_items.release();
     ctx->d_mle_out.release();
+    ctx->d_edge_done.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1208,16 +1287,8 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
 
     const double t_start = now_s();
     for (PcInst &I : P) {
-        I.adj.assign((size_t)n * n, 0);
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j) I.adj[(size_t)i * n + j] = (i != j);
         I.ex.clear();
         I.ey.clear();
-        for (int i = 0; i < n; ++i)
-            for (int j = i + 1; j < n; ++j) {
-                I.ex.push_back(i);
-                I.ey.push_back(j);
-            }
         I.seps.clear();
         I.sepz.clear();
         I.num_ci_test = 0;
@@ -1226,58 +1297,76 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
     std::vector<DevResult> res;
     std::vector<HostTest> tv;
 
-    // ---- level 0: every pair of the complete graph (PCStable.cpp:133-195)
+    // ---- level 0: every pair of the complete graph (PCStable.cpp:133-195), pairs x < y in
+    // the complete graph's lexicographic edge order (Network.cpp:270-283)
     {
         const double tl = now_s();
-        tv.clear();
-        for (int32_t i = 0; i < n_inst; ++i)
-            for (size_t e = 0; e < P[i].ex.size(); ++e) tv.push_back(HostTest{P[i].ex[e], P[i].ey[e], 0, i, 0});
+        const int64_t np = (int64_t)n * (n - 1) / 2;
         bool gram = false;
-        const DevResult *r0 = nullptr;
         if (!plain && n_inst == 1 && !(o.flags & FPGM_F_NO_GRAM)) {
             const double ms0 = ctx->g_ms + ctx->g_pair_ms;
-            st = run_level0_gram(ctx, 0, tv, alpha, o.flags, band, &r0, &gram);
+            st = run_level0_gram(ctx, 0, alpha, o.flags, band, &gram);
             if (st != FPGM_OK) return st;
             S.kernel_seconds_per_level[0] += (ctx->g_ms + ctx->g_pair_ms - ms0) * 1e-3;
         }
-        if (!gram) {
+        int64_t inb = 0, left = 0;
+        if (gram) {
+            PcInst &I = P[0];
+            const uint8_t *f = ctx->h_l0_flags.p;
+            int64_t k = 0;
+            for (int x = 0; x < n; ++x)
+                for (int y = x + 1; y < n; ++y, ++k) {
+                    if (f[k] & 1) {
+                        I.seps.push_back(PcInst::Sep{x, y, 0, 0, (int64_t)I.sepz.size()});
+                    } else {
+                        I.ex.push_back(x);
+                        I.ey.push_back(y);
+                    }
+                }
+            inb = ctx->h_l0_sum.p->in_band;
+            float err;
+            std::memcpy(&err, &ctx->h_l0_sum.p->max_rel_err_bits, sizeof(float));
+            S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, (double)err);
+            I.num_ci_test += np;
+            left = (int64_t)I.ex.size();
+        } else {
+            tv.clear();
+            for (int32_t i = 0; i < n_inst; ++i)
+                for (int x = 0; x < n; ++x)
+                    for (int y = x + 1; y < n; ++y) tv.push_back(HostTest{x, y, 0, i, 0});
             st = run_tests(ctx, plain, o, 0, tv, nullptr, 0, alpha, band, res, S);
             if (st != FPGM_OK) return st;
-            r0 = res.data();
-        }
-        const DevResult *res0 = r0;
-        int64_t k = 0, inb = 0, left = 0;
-        for (int32_t i = 0; i < n_inst; ++i) {
-            PcInst &I = P[i];
-            int64_t w = 0;
-            for (size_t e = 0; e < I.ex.size(); ++e, ++k) {
-                inb += res0[k].in_band;
-                if (res0[k].exact && res0[k].df > 0 && res0[k].g2_64 > 0.0)
-                    S.max_rel_err_g2_fp32 =
-                        std::max(S.max_rel_err_g2_fp32, std::fabs((double)res0[k].g2_32 - res0[k].g2_64) / res0[k].g2_64);
-                if (res0[k].indep) {
-                    I.seps.push_back(PcInst::Sep{I.ex[e], I.ey[e], 0, 0, (int64_t)I.sepz.size()});
-                    I.adj[(size_t)I.ex[e] * n + I.ey[e]] = 0;
-                    I.adj[(size_t)I.ey[e] * n + I.ex[e]] = 0;
-                } else {
-                    I.ex[w] = I.ex[e];
-                    I.ey[w] = I.ey[e];
-                    ++w;
-                }
+            const DevResult *r = res.data();
+            int64_t k = 0;
+            for (int32_t i = 0; i < n_inst; ++i) {
+                PcInst &I = P[i];
+                for (int x = 0; x < n; ++x)
+                    for (int y = x + 1; y < n; ++y, ++k) {
+                        inb += r[k].in_band;
+                        if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
+                            S.max_rel_err_g2_fp32 =
+                                std::max(S.max_rel_err_g2_fp32, std::fabs((double)r[k].g2_32 - r[k].g2_64) / r[k].g2_64);
+                        if (r[k].indep) {
+                            I.seps.push_back(PcInst::Sep{x, y, 0, 0, (int64_t)I.sepz.size()});
+                        } else {
+                            I.ex.push_back(x);
+                            I.ey.push_back(y);
+                        }
+                    }
+                I.num_ci_test += np;
+                left += (int64_t)I.ex.size();
             }
-            I.num_ci_test += (int64_t)I.ex.size();
-            I.ex.resize(w);
-            I.ey.resize(w);
-            left += w;
         }
-        S.tests_per_level[0] = (int64_t)tv.size();
-        S.executed_per_level[0] = (int64_t)tv.size();
+        S.tests_per_level[0] = np * n_inst;
+        S.executed_per_level[0] = np * n_inst;
         S.in_band_per_level[0] = inb;
         S.rounds_per_level[0] = 1;
         S.edges_after_level[0] = left;
         S.seconds_per_level[0] = now_s() - tl;
         S.n_levels = 1;
-        log_level(o, 0, (int64_t)tv.size(), left, now_s() - t_start);
+        log_level(o, 0, np * n_inst, left, now_s() - t_start);
+        // level 1 follows unconditionally (PCStable.cpp:209-231: FreeDegree is checked after
+        // each level d >= 1 only)
     }
 
     // ---- levels d >= 1 (PCStable.cpp:209-231, SearchAtDepth :265-388)
@@ -1292,15 +1381,24 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         for (int32_t i = 0; i < n_inst; ++i) {
             PcInst &I = P[i];
             if (!I.cont) continue;
-            // frozen copy of the adjacency (PCStable.cpp:272)
-            I.F.off.assign(n + 1, 0);
-            I.F.nb.clear();
-            for (int a = 0; a < n; ++a) {
-                for (int b = 0; b < n; ++b)
-                    if (I.adj[(size_t)a * n + b]) I.F.nb.push_back(b);
-                I.F.off[a + 1] = (int32_t)I.F.nb.size();
-            }
+            // frozen copy of the adjacency (PCStable.cpp:272), per node ascending, from the
+            // surviving edges in lexicographic order: node u receives its neighbours v < u
+            // (edges (v, u), ascending v) before its neighbours v > u (edges (u, v))
             const int64_t E = (int64_t)I.ex.size();
+            I.F.off.assign(n + 1, 0);
+            for (int64_t e = 0; e < E; ++e) {
+                ++I.F.off[I.ex[e] + 1];
+                ++I.F.off[I.ey[e] + 1];
+            }
+            for (int a = 0; a < n; ++a) I.F.off[a + 1] += I.F.off[a];
+            I.F.nb.assign((size_t)(2 * E), 0);
+            {
+                std::vector<int32_t> cur(I.F.off.begin(), I.F.off.end() - 1);
+                for (int64_t e = 0; e < E; ++e) {
+                    I.F.nb[cur[I.ex[e]]++] = I.ey[e];
+                    I.F.nb[cur[I.ey[e]]++] = I.ex[e];
+                }
+            }
             I.walks.assign((size_t)E, EdgeWalk{});
             I.choice.assign((size_t)E * d, 0);
             I.zlevel.clear();
@@ -1319,110 +1417,181 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
             o.round_tests > 0
                 ? o.round_tests
                 : std::min<int64_t>((int64_t)1 << 20,
-                                    std::max<int64_t>(8192, (int64_t)(2147483648.0 / std::max(1.0, (d + 2) * avg_rows))));
-        // With two lanes, the edges take turns: while the device counts one lane's round,
-        // the host takes the other lane's results and builds its next round.  Edges are
-        // independent within a level (frozen adjacency), so the split changes timing and
-        // speculation only, never results.  One lane is the default: at 10^6 rows the
-        // half-size launches of two lanes cost the device more than the overlap saves.
-        // Plain mode runs one lane synchronously.
-        struct Lane {
-            std::vector<std::pair<int32_t, int64_t>> active;  // (instance, edge)
+                                    std::max<int64_t>(8192, (int64_t)(ctx->sched.round_bytes /
+                                                                      std::max(1.0, (d + 2) * avg_rows))));
+        const double kms_level0 = ctx->k_ms;
+        // Edge g of the level is active[g].  done_round[g]: the first round in which a test of
+        // edge g came out independent (INT32_MAX while none) -- the host's copy of the
+        // device's d_edge_done, filled in round order from the results.
+        const int64_t n_act = (int64_t)active.size();
+        std::vector<int32_t> done_round((size_t)n_act, INT32_MAX);
+        std::vector<int64_t> live((size_t)n_act);  // edges with tests left, not known to be done
+        for (int64_t g = 0; g < n_act; ++g) live[g] = g;
+        if (!plain && n_act > 0) {
+            HIP_TRY(ctx->d_edge_done.ensure(n_act, false, ctx->stream), "alloc edge flags");
+            // 0x7f7f7f7f: above every round number
+            HIP_TRY(hipMemsetAsync(ctx->d_edge_done.p, 0x7f, sizeof(int32_t) * n_act, ctx->stream), "clear edge flags");
+        }
+        struct Round {
+            int32_t id = 0;
+            int slot = 0;
             std::vector<HostTest> tv;
             std::vector<int32_t> zpool;
-            std::vector<int64_t> t_edge;  // active index of each test of the round
-            std::vector<DevResult> res;   // results of rounds not run asynchronously
+            std::vector<DevResult> res;  // results of rounds run synchronously
             const DevResult *r = nullptr;
-            int64_t rounds = 0;
-            int slot = 0;
-            bool in_flight = false, async = false;
+            bool pending = false, async = false;
         };
-        const int n_lanes = plain ? 1 : ctx->sched.lanes;
-        Lane lanes[2];
-        for (size_t ai = 0; ai < active.size(); ++ai) lanes[ai % n_lanes].active.push_back(active[ai]);
-        for (int l = 0; l < n_lanes; ++l) lanes[l].slot = l;
-        const int64_t lane_budget = std::max<int64_t>(1, budget / n_lanes);
-        const double kms_level0 = ctx->k_ms;
+        Round rd[2];
+        rd[1].slot = 1;
+        int32_t next_id = 1;
+        struct Gen {  // one host thread's share of a round
+            std::vector<HostTest> tv;
+            std::vector<int32_t> z;
+            std::vector<int64_t> still;
+        };
+        std::vector<Gen> gen;
 
-        auto start = [&](Lane &ln) -> fpgm_status {
-            if (o.max_rounds > 0 && ln.rounds >= o.max_rounds)
+        // Builds the next round from the live edges -- every live edge's next W tests in
+        // enumeration order; the walks advance past them -- and submits it.  Batched rounds
+        // run asynchronously: the host builds round r+1 while the device counts round r, and
+        // the device skips round r+1's tests of edges that round r (or earlier) found
+        // independent.  A test's outcome depends only on (x, y, Z) and the data, and tests of
+        // one round never skip one another, so each edge's first independent test in
+        // enumeration order -- its sepset -- is the sequential one.
+        double t_build = 0.0, t_process = 0.0;
+        auto build = [&](Round &R) -> fpgm_status {
+            const double tb0 = now_s();
+            struct BuildTimer {
+                double t0, &acc;
+                ~BuildTimer() { acc += now_s() - t0; }
+            } build_timer{tb0, t_build};
+            if (o.max_rounds > 0 && rounds >= o.max_rounds)
                 return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
-            ++ln.rounds;
+            ++rounds;
+            R.id = next_id++;
             const bool plain_round = plain && (o.plain_sample <= 0 || S.plain_tests_per_level[d] < o.plain_sample);
-            // tests per edge this round: the budget shared by the active edges, at least a
+            // tests per edge this round: the budget shared by the live edges, at least a
             // geometric ramp (base^(round-1), capped) so long walks finish in few rounds
-            int64_t W = plain_round ? 1 : std::max<int64_t>(1, lane_budget / (int64_t)ln.active.size());
+            int64_t W = plain_round ? 1 : std::max<int64_t>(1, budget / (int64_t)live.size());
             if (!plain_round) {
-                const double ramp = std::pow(ctx->sched.base, (double)(ln.rounds - 1));
+                const double ramp = std::pow(ctx->sched.base, (double)(R.id - 1));
                 W = std::max<int64_t>(W, (int64_t)std::min<double>(ramp, (double)ctx->sched.cap));
             }
-            ln.tv.clear();
-            ln.zpool.clear();
-            ln.t_edge.clear();
-            for (size_t ai = 0; ai < ln.active.size(); ++ai) {
-                PcInst &I = P[ln.active[ai].first];
-                const int64_t e = ln.active[ai].second;
-                EdgeWalk &w = I.walks[e];
-                int32_t *c = &I.choice[(size_t)e * d];
-                // an edge found dependent on its first `promote` sets rarely ends early:
-                // give it the rest of its walk at once
-                int64_t We = W;
-                if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
-                for (int64_t k = 0; k < We && w.side < 2; ++k) {
-                    ln.tv.push_back(HostTest{w.x, w.y, d, ln.active[ai].first, (int64_t)ln.zpool.size()});
-                    for (int i = 0; i < d; ++i) ln.zpool.push_back(I.F.at(w.u, w.skip, c[i]));
-                    ln.t_edge.push_back((int64_t)ai);
-                    if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
+            // every live edge's tests, in live order; large rounds are generated in
+            // contiguous chunks of live edges on several host threads, then concatenated
+            const int64_t L = (int64_t)live.size();
+            const int nth = (L * W >= kParallelTests) ? ctx->host_threads : 1;
+            gen.resize((size_t)nth);
+#pragma omp parallel for num_threads(nth) schedule(static)
+            for (int ch = 0; ch < nth; ++ch) {
+                Gen &G = gen[ch];
+                G.tv.clear();
+                G.z.clear();
+                G.still.clear();
+                for (int64_t li = L * ch / nth; li < L * (ch + 1) / nth; ++li) {
+                    const int64_t g = live[li];
+                    PcInst &I = P[active[g].first];
+                    const int64_t e = active[g].second;
+                    EdgeWalk &w = I.walks[e];
+                    int32_t *c = &I.choice[(size_t)e * d];
+                    // an edge found dependent on its first `promote` sets rarely ends early:
+                    // give it the rest of its walk at once
+                    int64_t We = W;
+                    if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
+                    for (int64_t k = 0; k < We && w.side < 2; ++k) {
+                        G.tv.push_back(HostTest{w.x, w.y, d, active[g].first, (int64_t)G.z.size(), (int32_t)g});
+                        for (int i = 0; i < d; ++i) G.z.push_back(I.F.at(w.u, w.skip, c[i]));
+                        if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
+                    }
+                    if (w.side < 2) G.still.push_back(g);
                 }
             }
-            executed += (int64_t)ln.tv.size();
-            ln.in_flight = true;
-            ln.async = false;
-            const int64_t nt = (int64_t)ln.tv.size(), nz = (int64_t)ln.zpool.size();
-            if (plain) {
-                fpgm_status st2 = run_tests(ctx, plain_round, o, d, ln.tv, ln.zpool.data(), nz, alpha, band, ln.res, S);
-                ln.r = ln.res.data();
+            R.tv.clear();
+            R.zpool.clear();
+            live.clear();
+            for (int ch = 0; ch < nth; ++ch) {
+                const Gen &G = gen[ch];
+                const int64_t zb = (int64_t)R.zpool.size();
+                R.zpool.insert(R.zpool.end(), G.z.begin(), G.z.end());
+                const size_t t0 = R.tv.size();
+                R.tv.insert(R.tv.end(), G.tv.begin(), G.tv.end());
+                for (size_t k = t0; k < R.tv.size(); ++k) R.tv[k].z_off += zb;
+                live.insert(live.end(), G.still.begin(), G.still.end());
+            }
+            R.pending = true;
+            R.async = false;
+            const int64_t nt = (int64_t)R.tv.size(), nz = (int64_t)R.zpool.size();
+            if (plain) {  // synchronous: every earlier round is processed, nothing to skip
+                fpgm_status st2 = run_tests(ctx, plain_round, o, d, R.tv, R.zpool.data(), nz, alpha, band, R.res, S);
+                R.r = R.res.data();
                 return st2;
             }
-            Slot &sl = ctx->slot[ln.slot];
+            struct SkipScope {  // the round number goes into the launch arguments
+                fpgm_ctx *c;
+                ~SkipScope() { c->skip_round = 0; }
+            } skip_scope{ctx};
+            ctx->skip_round = R.id;
+            Slot &sl = ctx->slot[R.slot];
             int64_t i1 = 0;
             fpgm_status st2 =
-                batch_submit(ctx, sl, ln.tv.data(), nt, ln.zpool.data(), nz, alpha, o.flags, band, 0, 0, 0, &i1, nullptr);
+                batch_submit(ctx, sl, R.tv.data(), nt, R.zpool.data(), nz, alpha, o.flags, band, 0, 0, 0, &i1, nullptr);
             if (st2 != FPGM_OK) return st2;
             if (i1 == nt) {
-                ln.async = true;
+                R.async = true;
                 return FPGM_OK;
             }
             // more than one launch sequence (workspace limit): finish the round here
-            ln.res.resize((size_t)nt);
+            R.res.resize((size_t)nt);
             for (int64_t i0 = 0;;) {
                 st2 = batch_wait(ctx, sl);
                 if (st2 != FPGM_OK) return st2;
-                std::copy(sl.h_res.p, sl.h_res.p + (i1 - i0), ln.res.begin() + i0);
+                std::copy(sl.h_res.p, sl.h_res.p + (i1 - i0), R.res.begin() + i0);
                 i0 = i1;
                 if (i0 >= nt) break;
-                st2 = batch_submit(ctx, sl, ln.tv.data(), nt, ln.zpool.data(), nz, alpha, o.flags, band, 0, 0, i0, &i1,
+                st2 = batch_submit(ctx, sl, R.tv.data(), nt, R.zpool.data(), nz, alpha, o.flags, band, 0, 0, i0, &i1,
                                    nullptr);
                 if (st2 != FPGM_OK) return st2;
             }
-            ln.r = ln.res.data();
+            R.r = R.res.data();
             return FPGM_OK;
         };
 
-        auto finish = [&](Lane &ln) -> fpgm_status {
-            if (ln.async) {
-                fpgm_status st2 = batch_wait(ctx, ctx->slot[ln.slot]);
+        // Takes a round's results in enumeration order: per edge, the tests up to its first
+        // independent one count (reference-equivalent); later ones are speculative; the tests
+        // of an edge done in an earlier round were skipped by the device and count nowhere.
+        auto process = [&](Round &R) -> fpgm_status {
+            struct ProcTimer {
+                double t0, &acc;
+                ~ProcTimer() { acc += now_s() - t0; }
+            } proc_timer{now_s(), t_process};
+            if (R.async) {
+                fpgm_status st2 = batch_wait(ctx, ctx->slot[R.slot]);
                 if (st2 != FPGM_OK) return st2;
-                ln.r = ctx->slot[ln.slot].h_res.p;
+                R.r = ctx->slot[R.slot].h_res.p;
             }
-            const DevResult *r = ln.r;
-            std::vector<std::pair<int32_t, int64_t>> still;
-            size_t k = 0;
-            for (size_t ai = 0; ai < ln.active.size(); ++ai) {
-                PcInst &I = P[ln.active[ai].first];
-                EdgeWalk &w = I.walks[ln.active[ai].second];
+            const DevResult *r = R.r;
+            double skipped_bytes = 0.0;
+            int64_t host_skipped = 0, dev_skipped = 0;
+            bool any_done = false;
+            for (size_t k = 0; k < R.tv.size();) {
+                const int32_t g = R.tv[k].edge;
+                size_t k1 = k;
+                while (k1 < R.tv.size() && R.tv[k1].edge == g) ++k1;
+                PcInst &I = P[active[g].first];
+                EdgeWalk &w = I.walks[active[g].second];
+                if (done_round[g] < R.id) {
+                    skipped_bytes += (double)(k1 - k) * (d + 2) * (double)ctx->inst[active[g].first].n_rows;
+                    if (ctx->trace)
+                        for (size_t j = k; j < k1; ++j) dev_skipped += (r[j].df == kSkippedDf);
+                    host_skipped += (int64_t)(k1 - k);
+                    k = k1;
+                    continue;
+                }
                 bool done = false;
-                for (; k < ln.tv.size() && ln.t_edge[k] == (int64_t)ai; ++k) {
+                for (; k < k1; ++k) {
+                    if (r[k].df == kSkippedDf)
+                        return fail(ctx, FPGM_EDEVICE, "internal: the device skipped a test of a live edge");
+                    ++executed;
                     inb += r[k].in_band;
                     if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
                         S.max_rel_err_g2_fp32 =
@@ -1431,38 +1600,46 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                     w.tests += 1;
                     if (r[k].indep) {
                         done = true;
+                        any_done = true;
+                        done_round[g] = R.id;
                         w.removed = 1;
                         w.zsep = (int64_t)I.zlevel.size();
-                        for (int i = 0; i < d; ++i) I.zlevel.push_back(ln.zpool[ln.tv[k].z_off + i]);
+                        for (int i = 0; i < d; ++i) I.zlevel.push_back(R.zpool[R.tv[k].z_off + i]);
                     }
                 }
-                if (!done && w.side < 2) still.push_back(ln.active[ai]);
             }
-            ln.active.swap(still);
-            ln.in_flight = false;
+            ctx->k_bytes -= skipped_bytes;  // batch_wait counted every submitted test
+            if (ctx->trace)
+                fprintf(stderr, "[fpgm]   round %d: %zu tests, %lld skipped (device skipped %lld)\n", R.id, R.tv.size(),
+                        (long long)host_skipped, (long long)dev_skipped);
+            if (any_done)
+                live.erase(std::remove_if(live.begin(), live.end(),
+                                          [&](int64_t g) { return done_round[g] != INT32_MAX; }),
+                           live.end());
+            R.pending = false;
             return FPGM_OK;
         };
 
-        for (int l = 0; l < n_lanes; ++l)
-            if (!lanes[l].active.empty()) {
-                st = start(lanes[l]);
-                if (st != FPGM_OK) return st;
-            }
-        for (bool any = true; any;) {
-            any = false;
-            for (int l = 0; l < n_lanes; ++l) {
-                Lane &ln = lanes[l];
-                if (!ln.in_flight) continue;
-                any = true;
-                st = finish(ln);
-                if (st != FPGM_OK) return st;
-                if (!ln.active.empty()) {
-                    st = start(ln);
-                    if (st != FPGM_OK) return st;
-                }
-            }
+        int cur = 0;
+        if (!live.empty()) {
+            st = build(rd[0]);
+            if (st != FPGM_OK) return st;
         }
-        rounds = std::max(lanes[0].rounds, lanes[1].rounds);
+        while (rd[cur].pending) {
+            Round &A = rd[cur], &B = rd[cur ^ 1];
+            const bool ahead = !plain && ctx->sched.pipeline;
+            if (ahead && !live.empty()) {  // round r+1 goes to the device behind round r
+                st = build(B);
+                if (st != FPGM_OK) return st;
+            }
+            st = process(A);
+            if (st != FPGM_OK) return st;
+            if (!ahead && !live.empty()) {
+                st = build(B);
+                if (st != FPGM_OK) return st;
+            }
+            cur ^= 1;
+        }
         if (!plain) S.kernel_seconds_per_level[d] += (ctx->k_ms - kms_level0) * 1e-3;
         int64_t tests = 0, left = 0;
         for (PcInst &I : P) {
@@ -1477,8 +1654,6 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                 if (w.removed) {
                     I.seps.push_back(PcInst::Sep{w.x, w.y, d, d, (int64_t)I.sepz.size()});
                     for (int i = 0; i < d; ++i) I.sepz.push_back(I.zlevel[w.zsep + i]);
-                    I.adj[(size_t)w.x * n + w.y] = 0;
-                    I.adj[(size_t)w.y * n + w.x] = 0;
                 } else {
                     I.ex[wkeep] = w.x;
                     I.ey[wkeep] = w.y;
@@ -1491,12 +1666,12 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
             tests += lt;
             left += wkeep;
             // FreeDegree (PCStable.cpp:620-626): continue while max degree - 1 > d
-            int maxdeg = 0;
-            for (int a = 0; a < n; ++a) {
-                int deg = 0;
-                for (int b = 0; b < n; ++b) deg += I.adj[(size_t)a * n + b];
-                maxdeg = std::max(maxdeg, deg);
+            std::vector<int32_t> deg((size_t)n, 0);
+            for (size_t e = 0; e < I.ex.size(); ++e) {
+                ++deg[I.ex[e]];
+                ++deg[I.ey[e]];
             }
+            const int maxdeg = n > 0 ? *std::max_element(deg.begin(), deg.end()) : 0;
             I.cont = (maxdeg - 1 > d);
         }
         S.tests_per_level[d] = tests;
@@ -1508,11 +1683,19 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         S.n_levels = d + 1;
         log_level(o, d, tests, left, now_s() - t_start);
         if (ctx->trace)
-            fprintf(stderr, "[fpgm]   level %d host: %.4f s building launches, %.4f s launching and waiting, "
-                    "%.4f s level total\n", d, ctx->t_prep - prep0, ctx->t_wait - wait0, now_s() - tl);
+            fprintf(stderr, "[fpgm]   level %d host: %.4f s making rounds (%.4f s descriptors), %.4f s taking "
+                    "results (%.4f s waiting), %.4f s level total\n", d, t_build, ctx->t_prep - prep0, t_process,
+                    ctx->t_wait - wait0, now_s() - tl);
     }
     S.ci_seconds = now_s() - t_start;
     for (int l = 0; l < S.n_levels; ++l) S.num_ci_test += S.tests_per_level[l];
+    for (PcInst &I : P) {  // the skeleton as an adjacency matrix (after the timed region)
+        I.adj.assign((size_t)n * n, 0);
+        for (size_t e = 0; e < I.ex.size(); ++e) {
+            I.adj[(size_t)I.ex[e] * n + I.ey[e]] = 1;
+            I.adj[(size_t)I.ey[e] * n + I.ex[e]] = 1;
+        }
+    }
     return FPGM_OK;
 }
 

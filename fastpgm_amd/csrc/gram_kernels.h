@@ -30,7 +30,8 @@ struct PairArgs {
     int32_t Wp;
     const int32_t *off;        // one-hot offset of each variable
     const int32_t *card;
-    const int32_t *px, *py;    // pairs x < y
+    const int32_t *px, *py;    // pairs x < y; nullptr: all pairs of n_vars in lexicographic order
+    int32_t n_vars;
     int64_t n_pairs;
     const double *crit;
     double alpha, z_alpha;
@@ -39,7 +40,16 @@ struct PairArgs {
     DevResult *res;
 };
 
+// Level-0 results in one byte per pair (bit 0 independent, bit 1 in the recheck band) and
+// the summary the host keeps: the largest relative FP32 error of G^2 against FP64 over the
+// rechecked pairs (as float bits) and the number of in-band pairs.
+struct PairSummary {
+    uint32_t max_rel_err_bits;
+    uint32_t in_band;
+};
+
 hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s);
 hipError_t launch_pairs_l0(const PairArgs &p, hipStream_t s);
+hipError_t launch_pack_l0(const DevResult *res, int64_t n, uint8_t *flags, PairSummary *sum, hipStream_t s);
 
 }  // namespace fpgm

@@ -207,7 +207,21 @@ __global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
 {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= p.n_pairs) return;
-    const int x = p.px[t], y = p.py[t];
+    int x, y;
+    if (p.px) {
+        x = p.px[t];
+        y = p.py[t];
+    } else {
+        // t-th pair (x, y), x < y, in lexicographic order: row x starts at x (2n - x - 1) / 2
+        const int64_t n = p.n_vars;
+        const double b = 2.0 * (double)n - 1.0;
+        int64_t xx = (int64_t)((b - sqrt(fmax(b * b - 8.0 * (double)t, 0.0))) * 0.5);
+        auto row0 = [&](int64_t r) { return r * (2 * n - r - 1) / 2; };
+        while (xx > 0 && row0(xx) > t) --xx;
+        while (row0(xx + 1) <= t) ++xx;
+        x = (int)xx;
+        y = (int)(t - row0(xx) + xx + 1);
+    }
     const int cx = p.card[x], cy = p.card[y];
     const int32_t *base = p.C + (int64_t)p.off[x] * p.Wp + p.off[y];
     int32_t nj[kPairMaxCard];
@@ -289,7 +303,50 @@ __global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
     p.res[t] = R;
 }
 
+__global__ __launch_bounds__(256) void k_pack_l0(const DevResult *res, int64_t n, uint8_t *flags, PairSummary *sum)
+{
+    __shared__ float s_err[4];
+    __shared__ uint32_t s_inb[4];
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float err = 0.f;
+    uint32_t inb = 0;
+    if (t < n) {
+        const DevResult R = res[t];
+        flags[t] = (uint8_t)((R.indep ? 1 : 0) | (R.in_band ? 2 : 0));
+        inb = R.in_band ? 1u : 0u;
+        if (R.exact && R.df > 0 && R.g2_64 > 0.0) err = (float)(fabs((double)R.g2_32 - R.g2_64) / R.g2_64);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        err = fmaxf(err, __shfl_xor(err, o, 64));
+        inb += __shfl_xor(inb, o, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_err[wave] = err;
+        s_inb[wave] = inb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float e = s_err[0];
+        uint32_t c = s_inb[0];
+        for (int w = 1; w < 4; ++w) {
+            e = fmaxf(e, s_err[w]);
+            c += s_inb[w];
+        }
+        // non-negative floats order like their bit patterns
+        if (e > 0.f) atomicMax(&sum->max_rel_err_bits, __float_as_uint(e));
+        if (c) atomicAdd(&sum->in_band, c);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_pack_l0(const DevResult *res, int64_t n, uint8_t *flags, PairSummary *sum, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_l0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, res, n, flags, sum);
+    return hipGetLastError();
+}
 
 hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s)
 {

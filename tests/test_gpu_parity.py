@@ -119,17 +119,26 @@ def test_skeleton_round_size_does_not_change_results():
     assert sum(c["in_band_per_level"]) <= sum(c["executed_per_level"])
 
 
-def test_skeleton_two_edge_lanes_same_results(monkeypatch):
-    # FPGM_LANES=2 alternates the rounds of two edge lanes; results must not change
-    s = configs.get(2).sample(20_000)
+@pytest.mark.parametrize("key,alpha,depth,rows", [(2, 0.01, 5, 20_000), (3, 0.05, 4, 8_000)])
+def test_skeleton_pipelined_rounds_same_results(monkeypatch, key, alpha, depth, rows):
+    # FPGM_PIPELINE=1 builds round r+1 while round r runs; the device drops round r+1's
+    # tests of edges round r decided (compaction kernel).  Skeleton, sepsets and counted
+    # tests must equal the synchronous schedule's and the oracle's; small rounds
+    # (round_tests) make many rounds, so that many tests are dropped.
+    w = configs.get(key)
+    s = w.sample(rows)
+    o = O.pc_skeleton(O.Data.of(s), alpha, depth, n_threads=8)
+    monkeypatch.setenv("FPGM_PIPELINE", "0")
     with _ctx(s) as ctx:
-        a = ctx.pc_skeleton(0.01, 5)
-    monkeypatch.setenv("FPGM_LANES", "2")
+        a = ctx.pc_skeleton(alpha, depth)
+    monkeypatch.setenv("FPGM_PIPELINE", "1")
     with _ctx(s) as ctx:
-        b = ctx.pc_skeleton(0.01, 5)
-        c = ctx.pc_skeleton(0.01, 5, round_tests=64)
-    _assert_same(a, b)
-    _assert_same(a, c)
+        b = ctx.pc_skeleton(alpha, depth)
+        c = ctx.pc_skeleton(alpha, depth, round_tests=64)
+        d = ctx.pc_skeleton(alpha, depth, flags=_lib.FPGM_F_EXACT)
+    for g in (a, b, c, d):
+        _assert_same(g, o)
+    assert sum(c["rounds_per_level"]) > sum(b["rounds_per_level"])
 
 
 def test_skeleton_depth_and_alpha():
