@@ -162,6 +162,11 @@ def main():
                          "--pmc CSVs and DIR/run.json (the profiled run's output line); no GPU use")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes and round schedule (no fpgm_autotune before warm-up)")
+    ap.add_argument("--tune-file", default="", metavar="JSON",
+                    help="apply per-level tuning records saved by --save-tune instead of tuning (profiled runs)")
+    ap.add_argument("--save-tune", default="", metavar="JSON", help="write the autotuner's per-level records")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc)
@@ -180,6 +185,26 @@ def main():
     log(f"workload {w.name}: {sample.n_vars} vars x {sample.n_rows} rows generated in "
         f"{time.perf_counter() - t0:.1f} s")
     ctx = _lib.Context.of(sample)   # uploads the columns: not timed
+
+    # launch shapes and round schedule: the autotuner (after two untuned reference stages),
+    # saved records, or the defaults
+    tune, untuned, extra_stages = None, None, 0
+    if args.tune_file:
+        with open(args.tune_file) as f:
+            tune = json.load(f)
+        ctx.tune_set(tune["levels"])
+        log(f"tuning records from {args.tune_file}")
+    elif not args.no_autotune:
+        g = ctx.pc_skeleton(w.alpha, w.depth)
+        untuned = min(g["ci_seconds"], ctx.pc_skeleton(w.alpha, w.depth)["ci_seconds"])
+        extra_stages = 2
+        log(f"untuned stage: {g['num_ci_test']} tests in {untuned:.3f} s")
+        tune = ctx.autotune(w.alpha, w.depth)
+        log(f"autotune: {tune['seconds']:.1f} s, per level default -> tuned ms: " +
+            ", ".join(f"{r['level']}: {r['default_ms']:.1f} -> {r['tuned_ms']:.1f}" for r in tune["levels"]))
+        if args.save_tune:
+            with open(args.save_tune, "w") as f:
+                json.dump({"workload": w.name, **tune}, f, indent=1)
 
     for i in range(args.warmup):
         g = ctx.pc_skeleton(w.alpha, w.depth)
@@ -226,13 +251,16 @@ def main():
                    "depth": w.depth, "dag_edges": w.n_edges, "max_in_degree": w.max_in,
                    "card_law": list(w.card_law), "dirichlet": w.conc, "data_sha256": sample.sha256()[:16]},
         "pc_stable_step1_seconds": elapsed / args.steps,
-        "stages_in_run": args.warmup + args.steps + 1,
+        "stages_in_run": args.warmup + args.steps + 1 + extra_stages,
         "num_ci_test": g["num_ci_test"],
         "tests_per_level": g["tests_per_level"],
         "executed_per_level": g["executed_per_level"],
         "in_band_per_level": g["in_band_per_level"],
         "edges_after_level": g["edges_after_level"],
         "max_rel_err_g2_fp32_vs_fp64": g["max_rel_err_g2_fp32"],
+        "untuned_stage_ms": untuned * 1e3 if untuned else None,
+        "autotune": ({"seconds": tune.get("seconds"), "source": args.tune_file or "fpgm_autotune",
+                      "levels": tune["levels"]} if tune else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_count_stat", "kernel_ms_per_step": kernel_ms / args.steps,

@@ -31,6 +31,7 @@ EXPORTS = [
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats",
     "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
+    "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear",
 ]
 
 
@@ -64,6 +65,27 @@ class Stats(C.Structure):
                 ("n_levels", C.c_int32), ("pad", C.c_int32),
                 ("n_sepsets", C.c_int64), ("sep_z_total", C.c_int64), ("ci_seconds", C.c_double),
                 ("max_rel_err_g2_fp32", C.c_double)]
+
+
+class LevelTune(C.Structure):
+    _fields_ = [("level", C.c_int32), ("pipeline", C.c_int32), ("items_per_cu", C.c_int64),
+                ("wg_per_cu", C.c_int64 * 5), ("max_slices", C.c_int64), ("round_cap", C.c_int64),
+                ("default_ms", C.c_double), ("tuned_ms", C.c_double)]
+
+    def as_dict(self):
+        return {"level": self.level, "pipeline": self.pipeline, "items_per_cu": self.items_per_cu,
+                "wg_per_cu": list(self.wg_per_cu), "max_slices": self.max_slices, "round_cap": self.round_cap,
+                "default_ms": self.default_ms, "tuned_ms": self.tuned_ms}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        t.level, t.pipeline, t.items_per_cu = int(d["level"]), int(d["pipeline"]), int(d["items_per_cu"])
+        for g in range(5):
+            t.wg_per_cu[g] = int(d["wg_per_cu"][g])
+        t.max_slices, t.round_cap = int(d["max_slices"]), int(d["round_cap"])
+        t.default_ms, t.tuned_ms = float(d.get("default_ms", 0.0)), float(d.get("tuned_ms", 0.0))
+        return t
 
 
 class Sepset(C.Structure):
@@ -103,9 +125,13 @@ def load(path: str = LIB_PATH):
     L.fpgm_last_gram_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
     L.fpgm_pc_skeleton_multi.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, vp, vp, C.POINTER(Stats)]
     L.fpgm_mle_counts.argtypes = [vp, i32, vp, vp, vp, vp, i64]
+    L.fpgm_autotune.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, i32, C.POINTER(i32), C.POINTER(f64)]
+    L.fpgm_tune_set.argtypes = [vp, vp, i32]
+    L.fpgm_tune_clear.argtypes = [vp]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
-                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts"):
+                 "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
+                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -303,6 +329,24 @@ class Context:
         self._check(self._L.fpgm_mle_counts(self.h, n, off.ctypes.data, flat.ctypes.data, count_off.ctypes.data,
                                             counts.ctypes.data, total))
         return count_off, counts[:total]
+
+    def autotune(self, alpha, depth=1000, flags=0):
+        """Per-level launch shapes and round schedule chosen by timing on the device; kept in
+        the context for later stages.  Returns {"seconds", "levels": [per-level records]}."""
+        o = PcOpts(mode=FPGM_MODE_BATCHED, n_threads=1, group_size=1, flags=flags)
+        recs = (LevelTune * 64)()
+        n = C.c_int32()
+        sec = C.c_double()
+        self._check(self._L.fpgm_autotune(self.h, float(alpha), int(depth), C.byref(o), C.cast(recs, C.c_void_p), 64,
+                                          C.byref(n), C.byref(sec)))
+        return {"seconds": sec.value, "levels": [recs[i].as_dict() for i in range(n.value) if recs[i].level >= 0]}
+
+    def tune_set(self, levels):
+        recs = (LevelTune * max(1, len(levels)))(*[LevelTune.from_dict(d) for d in levels])
+        self._check(self._L.fpgm_tune_set(self.h, C.cast(recs, C.c_void_p), len(levels)))
+
+    def tune_clear(self):
+        self._check(self._L.fpgm_tune_clear(self.h))
 
     def gram_stats(self):
         ms, ops, pms, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()

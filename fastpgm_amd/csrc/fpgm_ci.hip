@@ -219,7 +219,12 @@ struct fpgm_ctx {
         int64_t items_per_cu = 16;          // work items a launch aims at per CU (row chunks per test)
         int64_t wpc[5] = {4, 2, 4, 2, 8};   // grid workgroups per CU: lane, lane-wide, shared, slice, device
         int pipeline = 1;                    // build round r+1 while the device counts round r
+        int64_t gram_items_per_cu = 8;       // level-0 Gram work items (tile pairs x row splits) per CU
     } sched;
+    // per-level scheduling and launch shapes chosen by fpgm_autotune (index = level); a
+    // candidate being timed by the tuner overrides them
+    std::vector<fpgm_level_tune> tune;
+    const fpgm_level_tune *tune_try = nullptr;
     int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
     uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices
 
@@ -688,7 +693,7 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
         }
     const int64_t n_tp = (int64_t)tiles.size() / 2;
     const int64_t ksteps = std::max<int64_t>(1, (I.n_rows + 31) / 32);
-    const int64_t target = (int64_t)ctx->n_cu * 8;
+    const int64_t target = (int64_t)ctx->n_cu * ctx->sched.gram_items_per_cu;
     const int64_t splits = std::min<int64_t>(ksteps, std::max<int64_t>(1, (target + n_tp - 1) / n_tp));
     const int64_t kps = (ksteps + splits - 1) / splits * 32;
     const int64_t n_items = n_tp * splits;
@@ -1256,8 +1261,9 @@ struct PcInst {
         int32_t x, y, level, d;
         int64_t zoff;
     };
-    std::vector<Sep> seps;
+    std::vector<Sep> seps;          // level 0's removals, then each later level's, each run ascending
     std::vector<int32_t> sepz;
+    size_t n_sep0 = 0;              // level 0's removals (the first run of seps)
     int64_t num_ci_test = 0;
     bool cont = true;               // continues to the next level
     Frozen F;
@@ -1265,10 +1271,361 @@ struct PcInst {
     std::vector<int32_t> choice, zlevel;
 };
 
+void apply_tune(fpgm_ctx *ctx, const fpgm_level_tune &t)
+{
+    if (t.items_per_cu > 0) ctx->sched.items_per_cu = t.items_per_cu;
+    for (int g = 0; g < kGroups; ++g)
+        if (t.wg_per_cu[g] > 0) ctx->sched.wpc[g] = t.wg_per_cu[g];
+    if (t.max_slices > 0) ctx->max_slices = t.max_slices;
+    if (t.round_cap > 0) ctx->sched.cap = t.round_cap;
+    if (t.pipeline >= 0) ctx->sched.pipeline = t.pipeline;
+}
+
+// Level d >= 1 of every instance still running (SearchAtDepth, PCStable.cpp:265-388): the
+// adjacency frozen from the surviving edges, every edge's walk over its conditioning sets
+// in speculative rounds, then the removals and FreeDegree.
+fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o, std::vector<PcInst> &P,
+                      fpgm_stats &S, double t_start)
+{
+    struct SchedScope {  // the level's tuned settings, the defaults again afterwards
+        fpgm_ctx *c;
+        decltype(fpgm_ctx::sched) sched;
+        int64_t max_slices;
+        ~SchedScope()
+        {
+            c->sched = sched;
+            c->max_slices = max_slices;
+        }
+    } sched_scope{ctx, ctx->sched, ctx->max_slices};
+    const fpgm_level_tune *lt = ctx->tune_try;
+    if (!lt && d < (int)ctx->tune.size() && ctx->tune[d].level == d) lt = &ctx->tune[d];
+    if (lt) apply_tune(ctx, *lt);
+    const float band = o.band > 0.f ? o.band : 0.02f;
+    const bool plain = (o.mode == FPGM_MODE_PLAIN);
+    const int n = ctx->n_vars;
+    const int32_t n_inst = (int32_t)P.size();
+    double avg_rows = 0.0;
+    for (const InstHost &I : ctx->inst) avg_rows += (double)I.n_rows / (double)ctx->inst.size();
+    fpgm_status st = FPGM_OK;
+    if (d > kMaxD) return fail(ctx, FPGM_ETOOBIG, "conditioning depth exceeds the device limit");
+    const double tl = now_s();
+    const double prep0 = ctx->t_prep, wait0 = ctx->t_wait;
+    std::vector<std::pair<int32_t, int64_t>> active;  // (instance, edge)
+    for (int32_t i = 0; i < n_inst; ++i) {
+        PcInst &I = P[i];
+        if (!I.cont) continue;
+        // frozen copy of the adjacency (PCStable.cpp:272), per node ascending, from the
+        // surviving edges in lexicographic order: node u receives its neighbours v < u
+        // (edges (v, u), ascending v) before its neighbours v > u (edges (u, v))
+        const int64_t E = (int64_t)I.ex.size();
+        I.F.off.assign(n + 1, 0);
+        for (int64_t e = 0; e < E; ++e) {
+            ++I.F.off[I.ex[e] + 1];
+            ++I.F.off[I.ey[e] + 1];
+        }
+        for (int a = 0; a < n; ++a) I.F.off[a + 1] += I.F.off[a];
+        I.F.nb.assign((size_t)(2 * E), 0);
+        {
+            std::vector<int32_t> cur(I.F.off.begin(), I.F.off.end() - 1);
+            for (int64_t e = 0; e < E; ++e) {
+                I.F.nb[cur[I.ex[e]]++] = I.ey[e];
+                I.F.nb[cur[I.ey[e]]++] = I.ex[e];
+            }
+        }
+        I.walks.assign((size_t)E, EdgeWalk{});
+        I.choice.assign((size_t)E * d, 0);
+        I.zlevel.clear();
+        for (int64_t e = 0; e < E; ++e) {
+            EdgeWalk &w = I.walks[e];
+            w.x = I.ex[e];
+            w.y = I.ey[e];
+            w.tests = 0;
+            w.removed = 0;
+            w.zsep = -1;
+            if (start_side(w, I.F, d, 0, &I.choice[(size_t)e * d])) active.emplace_back(i, e);
+        }
+    }
+    int64_t executed = 0, inb = 0, rounds = 0;
+    const int64_t budget =
+        o.round_tests > 0
+            ? o.round_tests
+            : std::min<int64_t>((int64_t)1 << 20,
+                                std::max<int64_t>(8192, (int64_t)(ctx->sched.round_bytes /
+                                                                  std::max(1.0, (d + 2) * avg_rows))));
+    const double kms_level0 = ctx->k_ms;
+    // Edge g of the level is active[g].  done_round[g]: the first round in which a test of
+    // edge g came out independent (INT32_MAX while none) -- the host's copy of the
+    // device's d_edge_done, filled in round order from the results.
+    const int64_t n_act = (int64_t)active.size();
+    std::vector<int32_t> done_round((size_t)n_act, INT32_MAX);
+    std::vector<int64_t> live((size_t)n_act);  // edges with tests left, not known to be done
+    for (int64_t g = 0; g < n_act; ++g) live[g] = g;
+    if (!plain && n_act > 0) {
+        HIP_TRY(ctx->d_edge_done.ensure(n_act, false, ctx->stream), "alloc edge flags");
+        // 0x7f7f7f7f: above every round number
+        HIP_TRY(hipMemsetAsync(ctx->d_edge_done.p, 0x7f, sizeof(int32_t) * n_act, ctx->stream), "clear edge flags");
+    }
+    struct Round {
+        int32_t id = 0;
+        int slot = 0;
+        std::vector<HostTest> tv;
+        std::vector<int32_t> zpool;
+        std::vector<DevResult> res;  // results of rounds run synchronously
+        const DevResult *r = nullptr;
+        bool pending = false, async = false;
+    };
+    Round rd[2];
+    rd[1].slot = 1;
+    int32_t next_id = 1;
+    struct Gen {  // one host thread's share of a round
+        std::vector<HostTest> tv;
+        std::vector<int32_t> z;
+        std::vector<int64_t> still;
+    };
+    std::vector<Gen> gen;
+
+    // Builds the next round from the live edges -- every live edge's next W tests in
+    // enumeration order; the walks advance past them -- and submits it.  Batched rounds
+    // run asynchronously: the host builds round r+1 while the device counts round r, and
+    // the device skips round r+1's tests of edges that round r (or earlier) found
+    // independent.  A test's outcome depends only on (x, y, Z) and the data, and tests of
+    // one round never skip one another, so each edge's first independent test in
+    // enumeration order -- its sepset -- is the sequential one.
+    double t_build = 0.0, t_process = 0.0;
+    auto build = [&](Round &R) -> fpgm_status {
+        const double tb0 = now_s();
+        struct BuildTimer {
+            double t0, &acc;
+            ~BuildTimer() { acc += now_s() - t0; }
+        } build_timer{tb0, t_build};
+        if (o.max_rounds > 0 && rounds >= o.max_rounds)
+            return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
+        ++rounds;
+        R.id = next_id++;
+        const bool plain_round = plain && (o.plain_sample <= 0 || S.plain_tests_per_level[d] < o.plain_sample);
+        // tests per edge this round: the budget shared by the live edges, at least a
+        // geometric ramp (base^(round-1), capped) so long walks finish in few rounds
+        int64_t W = plain_round ? 1 : std::max<int64_t>(1, budget / (int64_t)live.size());
+        if (!plain_round) {
+            const double ramp = std::pow(ctx->sched.base, (double)(R.id - 1));
+            W = std::max<int64_t>(W, (int64_t)std::min<double>(ramp, (double)ctx->sched.cap));
+        }
+        // every live edge's tests, in live order; large rounds are generated in
+        // contiguous chunks of live edges on several host threads, then concatenated
+        const int64_t L = (int64_t)live.size();
+        const int nth = (L * W >= kParallelTests) ? ctx->host_threads : 1;
+        gen.resize((size_t)nth);
+#pragma omp parallel for num_threads(nth) schedule(static)
+        for (int ch = 0; ch < nth; ++ch) {
+            Gen &G = gen[ch];
+            G.tv.clear();
+            G.z.clear();
+            G.still.clear();
+            for (int64_t li = L * ch / nth; li < L * (ch + 1) / nth; ++li) {
+                const int64_t g = live[li];
+                PcInst &I = P[active[g].first];
+                const int64_t e = active[g].second;
+                EdgeWalk &w = I.walks[e];
+                int32_t *c = &I.choice[(size_t)e * d];
+                // an edge found dependent on its first `promote` sets rarely ends early:
+                // give it the rest of its walk at once
+                int64_t We = W;
+                if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
+                for (int64_t k = 0; k < We && w.side < 2; ++k) {
+                    G.tv.push_back(HostTest{w.x, w.y, d, active[g].first, (int64_t)G.z.size(), (int32_t)g});
+                    for (int i = 0; i < d; ++i) G.z.push_back(I.F.at(w.u, w.skip, c[i]));
+                    if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
+                }
+                if (w.side < 2) G.still.push_back(g);
+            }
+        }
+        R.tv.clear();
+        R.zpool.clear();
+        live.clear();
+        for (int ch = 0; ch < nth; ++ch) {
+            const Gen &G = gen[ch];
+            const int64_t zb = (int64_t)R.zpool.size();
+            R.zpool.insert(R.zpool.end(), G.z.begin(), G.z.end());
+            const size_t t0 = R.tv.size();
+            R.tv.insert(R.tv.end(), G.tv.begin(), G.tv.end());
+            for (size_t k = t0; k < R.tv.size(); ++k) R.tv[k].z_off += zb;
+            live.insert(live.end(), G.still.begin(), G.still.end());
+        }
+        R.pending = true;
+        R.async = false;
+        const int64_t nt = (int64_t)R.tv.size(), nz = (int64_t)R.zpool.size();
+        if (plain) {  // synchronous: every earlier round is processed, nothing to skip
+            fpgm_status st2 = run_tests(ctx, plain_round, o, d, R.tv, R.zpool.data(), nz, alpha, band, R.res, S);
+            R.r = R.res.data();
+            return st2;
+        }
+        struct SkipScope {  // the round number goes into the launch arguments
+            fpgm_ctx *c;
+            ~SkipScope() { c->skip_round = 0; }
+        } skip_scope{ctx};
+        ctx->skip_round = R.id;
+        Slot &sl = ctx->slot[R.slot];
+        int64_t i1 = 0;
+        fpgm_status st2 =
+            batch_submit(ctx, sl, R.tv.data(), nt, R.zpool.data(), nz, alpha, o.flags, band, 0, 0, 0, &i1, nullptr);
+        if (st2 != FPGM_OK) return st2;
+        if (i1 == nt) {
+            R.async = true;
+            return FPGM_OK;
+        }
+        // more than one launch sequence (workspace limit): finish the round here
+        R.res.resize((size_t)nt);
+        for (int64_t i0 = 0;;) {
+            st2 = batch_wait(ctx, sl);
+            if (st2 != FPGM_OK) return st2;
+            std::copy(sl.h_res.p, sl.h_res.p + (i1 - i0), R.res.begin() + i0);
+            i0 = i1;
+            if (i0 >= nt) break;
+            st2 = batch_submit(ctx, sl, R.tv.data(), nt, R.zpool.data(), nz, alpha, o.flags, band, 0, 0, i0, &i1,
+                               nullptr);
+            if (st2 != FPGM_OK) return st2;
+        }
+        R.r = R.res.data();
+        return FPGM_OK;
+    };
+
+    // Takes a round's results in enumeration order: per edge, the tests up to its first
+    // independent one count (reference-equivalent); later ones are speculative; the tests
+    // of an edge done in an earlier round were skipped by the device and count nowhere.
+    auto process = [&](Round &R) -> fpgm_status {
+        struct ProcTimer {
+            double t0, &acc;
+            ~ProcTimer() { acc += now_s() - t0; }
+        } proc_timer{now_s(), t_process};
+        if (R.async) {
+            fpgm_status st2 = batch_wait(ctx, ctx->slot[R.slot]);
+            if (st2 != FPGM_OK) return st2;
+            R.r = ctx->slot[R.slot].h_res.p;
+        }
+        const DevResult *r = R.r;
+        double skipped_bytes = 0.0;
+        int64_t host_skipped = 0, dev_skipped = 0;
+        bool any_done = false;
+        for (size_t k = 0; k < R.tv.size();) {
+            const int32_t g = R.tv[k].edge;
+            size_t k1 = k;
+            while (k1 < R.tv.size() && R.tv[k1].edge == g) ++k1;
+            PcInst &I = P[active[g].first];
+            EdgeWalk &w = I.walks[active[g].second];
+            if (done_round[g] < R.id) {
+                skipped_bytes += (double)(k1 - k) * (d + 2) * (double)ctx->inst[active[g].first].n_rows;
+                if (ctx->trace)
+                    for (size_t j = k; j < k1; ++j) dev_skipped += (r[j].df == kSkippedDf);
+                host_skipped += (int64_t)(k1 - k);
+                k = k1;
+                continue;
+            }
+            bool done = false;
+            for (; k < k1; ++k) {
+                if (r[k].df == kSkippedDf)
+                    return fail(ctx, FPGM_EDEVICE, "internal: the device skipped a test of a live edge");
+                ++executed;
+                inb += r[k].in_band;
+                if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
+                    S.max_rel_err_g2_fp32 =
+                        std::max(S.max_rel_err_g2_fp32, std::fabs((double)r[k].g2_32 - r[k].g2_64) / r[k].g2_64);
+                if (done) continue;
+                w.tests += 1;
+                if (r[k].indep) {
+                    done = true;
+                    any_done = true;
+                    done_round[g] = R.id;
+                    w.removed = 1;
+                    w.zsep = (int64_t)I.zlevel.size();
+                    for (int i = 0; i < d; ++i) I.zlevel.push_back(R.zpool[R.tv[k].z_off + i]);
+                }
+            }
+        }
+        ctx->k_bytes -= skipped_bytes;  // batch_wait counted every submitted test
+        if (ctx->trace)
+            fprintf(stderr, "[fpgm]   round %d: %zu tests, %lld skipped (device skipped %lld)\n", R.id, R.tv.size(),
+                    (long long)host_skipped, (long long)dev_skipped);
+        if (any_done)
+            live.erase(std::remove_if(live.begin(), live.end(),
+                                      [&](int64_t g) { return done_round[g] != INT32_MAX; }),
+                       live.end());
+        R.pending = false;
+        return FPGM_OK;
+    };
+
+    int cur = 0;
+    if (!live.empty()) {
+        st = build(rd[0]);
+        if (st != FPGM_OK) return st;
+    }
+    while (rd[cur].pending) {
+        Round &A = rd[cur], &B = rd[cur ^ 1];
+        const bool ahead = !plain && ctx->sched.pipeline;
+        if (ahead && !live.empty()) {  // round r+1 goes to the device behind round r
+            st = build(B);
+            if (st != FPGM_OK) return st;
+        }
+        st = process(A);
+        if (st != FPGM_OK) return st;
+        if (!ahead && !live.empty()) {
+            st = build(B);
+            if (st != FPGM_OK) return st;
+        }
+        cur ^= 1;
+    }
+    if (!plain) S.kernel_seconds_per_level[d] += (ctx->k_ms - kms_level0) * 1e-3;
+    int64_t tests = 0, left = 0;
+    for (PcInst &I : P) {
+        if (!I.cont) {
+            left += (int64_t)I.ex.size();
+            continue;
+        }
+        int64_t wkeep = 0, lt = 0;
+        for (size_t e = 0; e < I.walks.size(); ++e) {
+            const EdgeWalk &w = I.walks[e];
+            lt += w.tests;
+            if (w.removed) {
+                I.seps.push_back(PcInst::Sep{w.x, w.y, d, d, (int64_t)I.sepz.size()});
+                for (int i = 0; i < d; ++i) I.sepz.push_back(I.zlevel[w.zsep + i]);
+            } else {
+                I.ex[wkeep] = w.x;
+                I.ey[wkeep] = w.y;
+                ++wkeep;
+            }
+        }
+        I.ex.resize(wkeep);
+        I.ey.resize(wkeep);
+        I.num_ci_test += lt;
+        tests += lt;
+        left += wkeep;
+        // FreeDegree (PCStable.cpp:620-626): continue while max degree - 1 > d
+        std::vector<int32_t> deg((size_t)n, 0);
+        for (size_t e = 0; e < I.ex.size(); ++e) {
+            ++deg[I.ex[e]];
+            ++deg[I.ey[e]];
+        }
+        const int maxdeg = n > 0 ? *std::max_element(deg.begin(), deg.end()) : 0;
+        I.cont = (maxdeg - 1 > d);
+    }
+    S.tests_per_level[d] = tests;
+    S.executed_per_level[d] = executed;
+    S.in_band_per_level[d] = inb;
+    S.rounds_per_level[d] = rounds;
+    S.edges_after_level[d] = left;
+    S.seconds_per_level[d] = now_s() - tl;
+    S.n_levels = d + 1;
+    log_level(o, d, tests, left, now_s() - t_start);
+    if (ctx->trace)
+        fprintf(stderr, "[fpgm]   level %d host: %.4f s making rounds (%.4f s descriptors), %.4f s taking "
+                "results (%.4f s waiting), %.4f s level total\n", d, t_build, ctx->t_prep - prep0, t_process,
+                ctx->t_wait - wait0, now_s() - tl);
+    return FPGM_OK;
+}
+
 // The CI stage of every instance, level-synchronously: level d's tests of all
 // instances that reached level d share the same rounds and launches.
 fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts &o, std::vector<PcInst> &P,
-                   fpgm_stats &S)
+                   fpgm_stats &S, std::vector<std::vector<PcInst>> *snaps = nullptr)
 {
     const float band = o.band > 0.f ? o.band : 0.02f;
     const bool plain = (o.mode == FPGM_MODE_PLAIN);
@@ -1305,7 +1662,11 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         bool gram = false;
         if (!plain && n_inst == 1 && !(o.flags & FPGM_F_NO_GRAM)) {
             const double ms0 = ctx->g_ms + ctx->g_pair_ms;
+            const int64_t gram_default = ctx->sched.gram_items_per_cu;
+            if (!ctx->tune.empty() && ctx->tune[0].level == 0 && ctx->tune[0].items_per_cu > 0)
+                ctx->sched.gram_items_per_cu = ctx->tune[0].items_per_cu;
             st = run_level0_gram(ctx, 0, alpha, o.flags, band, &gram);
+            ctx->sched.gram_items_per_cu = gram_default;
             if (st != FPGM_OK) return st;
             S.kernel_seconds_per_level[0] += (ctx->g_ms + ctx->g_pair_ms - ms0) * 1e-3;
         }
@@ -1328,6 +1689,7 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
             std::memcpy(&err, &ctx->h_l0_sum.p->max_rel_err_bits, sizeof(float));
             S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, (double)err);
             I.num_ci_test += np;
+            I.n_sep0 = I.seps.size();
             left = (int64_t)I.ex.size();
         } else {
             tv.clear();
@@ -1354,6 +1716,7 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
                         }
                     }
                 I.num_ci_test += np;
+                I.n_sep0 = I.seps.size();
                 left += (int64_t)I.ex.size();
             }
         }
@@ -1374,318 +1737,9 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         bool any = false;
         for (const PcInst &I : P) any = any || I.cont;
         if (!any) break;
-        if (d > kMaxD) return fail(ctx, FPGM_ETOOBIG, "conditioning depth exceeds the device limit");
-        const double tl = now_s();
-        const double prep0 = ctx->t_prep, wait0 = ctx->t_wait;
-        std::vector<std::pair<int32_t, int64_t>> active;  // (instance, edge)
-        for (int32_t i = 0; i < n_inst; ++i) {
-            PcInst &I = P[i];
-            if (!I.cont) continue;
-            // frozen copy of the adjacency (PCStable.cpp:272), per node ascending, from the
-            // surviving edges in lexicographic order: node u receives its neighbours v < u
-            // (edges (v, u), ascending v) before its neighbours v > u (edges (u, v))
-            const int64_t E = (int64_t)I.ex.size();
-            I.F.off.assign(n + 1, 0);
-            for (int64_t e = 0; e < E; ++e) {
-                ++I.F.off[I.ex[e] + 1];
-                ++I.F.off[I.ey[e] + 1];
-            }
-            for (int a = 0; a < n; ++a) I.F.off[a + 1] += I.F.off[a];
-            I.F.nb.assign((size_t)(2 * E), 0);
-            {
-                std::vector<int32_t> cur(I.F.off.begin(), I.F.off.end() - 1);
-                for (int64_t e = 0; e < E; ++e) {
-                    I.F.nb[cur[I.ex[e]]++] = I.ey[e];
-                    I.F.nb[cur[I.ey[e]]++] = I.ex[e];
-                }
-            }
-            I.walks.assign((size_t)E, EdgeWalk{});
-            I.choice.assign((size_t)E * d, 0);
-            I.zlevel.clear();
-            for (int64_t e = 0; e < E; ++e) {
-                EdgeWalk &w = I.walks[e];
-                w.x = I.ex[e];
-                w.y = I.ey[e];
-                w.tests = 0;
-                w.removed = 0;
-                w.zsep = -1;
-                if (start_side(w, I.F, d, 0, &I.choice[(size_t)e * d])) active.emplace_back(i, e);
-            }
-        }
-        int64_t executed = 0, inb = 0, rounds = 0;
-        const int64_t budget =
-            o.round_tests > 0
-                ? o.round_tests
-                : std::min<int64_t>((int64_t)1 << 20,
-                                    std::max<int64_t>(8192, (int64_t)(ctx->sched.round_bytes /
-                                                                      std::max(1.0, (d + 2) * avg_rows))));
-        const double kms_level0 = ctx->k_ms;
-        // Edge g of the level is active[g].  done_round[g]: the first round in which a test of
-        // edge g came out independent (INT32_MAX while none) -- the host's copy of the
-        // device's d_edge_done, filled in round order from the results.
-        const int64_t n_act = (int64_t)active.size();
-        std::vector<int32_t> done_round((size_t)n_act, INT32_MAX);
-        std::vector<int64_t> live((size_t)n_act);  // edges with tests left, not known to be done
-        for (int64_t g = 0; g < n_act; ++g) live[g] = g;
-        if (!plain && n_act > 0) {
-            HIP_TRY(ctx->d_edge_done.ensure(n_act, false, ctx->stream), "alloc edge flags");
-            // 0x7f7f7f7f: above every round number
-            HIP_TRY(hipMemsetAsync(ctx->d_edge_done.p, 0x7f, sizeof(int32_t) * n_act, ctx->stream), "clear edge flags");
-        }
-        struct Round {
-            int32_t id = 0;
-            int slot = 0;
-            std::vector<HostTest> tv;
-            std::vector<int32_t> zpool;
-            std::vector<DevResult> res;  // results of rounds run synchronously
-            const DevResult *r = nullptr;
-            bool pending = false, async = false;
-        };
-        Round rd[2];
-        rd[1].slot = 1;
-        int32_t next_id = 1;
-        struct Gen {  // one host thread's share of a round
-            std::vector<HostTest> tv;
-            std::vector<int32_t> z;
-            std::vector<int64_t> still;
-        };
-        std::vector<Gen> gen;
-
-        // Builds the next round from the live edges -- every live edge's next W tests in
-        // enumeration order; the walks advance past them -- and submits it.  Batched rounds
-        // run asynchronously: the host builds round r+1 while the device counts round r, and
-        // the device skips round r+1's tests of edges that round r (or earlier) found
-        // independent.  A test's outcome depends only on (x, y, Z) and the data, and tests of
-        // one round never skip one another, so each edge's first independent test in
-        // enumeration order -- its sepset -- is the sequential one.
-        double t_build = 0.0, t_process = 0.0;
-        auto build = [&](Round &R) -> fpgm_status {
-            const double tb0 = now_s();
-            struct BuildTimer {
-                double t0, &acc;
-                ~BuildTimer() { acc += now_s() - t0; }
-            } build_timer{tb0, t_build};
-            if (o.max_rounds > 0 && rounds >= o.max_rounds)
-                return fail(ctx, FPGM_ETIMEOUT, "level exceeded the round limit");
-            ++rounds;
-            R.id = next_id++;
-            const bool plain_round = plain && (o.plain_sample <= 0 || S.plain_tests_per_level[d] < o.plain_sample);
-            // tests per edge this round: the budget shared by the live edges, at least a
-            // geometric ramp (base^(round-1), capped) so long walks finish in few rounds
-            int64_t W = plain_round ? 1 : std::max<int64_t>(1, budget / (int64_t)live.size());
-            if (!plain_round) {
-                const double ramp = std::pow(ctx->sched.base, (double)(R.id - 1));
-                W = std::max<int64_t>(W, (int64_t)std::min<double>(ramp, (double)ctx->sched.cap));
-            }
-            // every live edge's tests, in live order; large rounds are generated in
-            // contiguous chunks of live edges on several host threads, then concatenated
-            const int64_t L = (int64_t)live.size();
-            const int nth = (L * W >= kParallelTests) ? ctx->host_threads : 1;
-            gen.resize((size_t)nth);
-#pragma omp parallel for num_threads(nth) schedule(static)
-            for (int ch = 0; ch < nth; ++ch) {
-                Gen &G = gen[ch];
-                G.tv.clear();
-                G.z.clear();
-                G.still.clear();
-                for (int64_t li = L * ch / nth; li < L * (ch + 1) / nth; ++li) {
-                    const int64_t g = live[li];
-                    PcInst &I = P[active[g].first];
-                    const int64_t e = active[g].second;
-                    EdgeWalk &w = I.walks[e];
-                    int32_t *c = &I.choice[(size_t)e * d];
-                    // an edge found dependent on its first `promote` sets rarely ends early:
-                    // give it the rest of its walk at once
-                    int64_t We = W;
-                    if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
-                    for (int64_t k = 0; k < We && w.side < 2; ++k) {
-                        G.tv.push_back(HostTest{w.x, w.y, d, active[g].first, (int64_t)G.z.size(), (int32_t)g});
-                        for (int i = 0; i < d; ++i) G.z.push_back(I.F.at(w.u, w.skip, c[i]));
-                        if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
-                    }
-                    if (w.side < 2) G.still.push_back(g);
-                }
-            }
-            R.tv.clear();
-            R.zpool.clear();
-            live.clear();
-            for (int ch = 0; ch < nth; ++ch) {
-                const Gen &G = gen[ch];
-                const int64_t zb = (int64_t)R.zpool.size();
-                R.zpool.insert(R.zpool.end(), G.z.begin(), G.z.end());
-                const size_t t0 = R.tv.size();
-                R.tv.insert(R.tv.end(), G.tv.begin(), G.tv.end());
-                for (size_t k = t0; k < R.tv.size(); ++k) R.tv[k].z_off += zb;
-                live.insert(live.end(), G.still.begin(), G.still.end());
-            }
-            R.pending = true;
-            R.async = false;
-            const int64_t nt = (int64_t)R.tv.size(), nz = (int64_t)R.zpool.size();
-            if (plain) {  // synchronous: every earlier round is processed, nothing to skip
-                fpgm_status st2 = run_tests(ctx, plain_round, o, d, R.tv, R.zpool.data(), nz, alpha, band, R.res, S);
-                R.r = R.res.data();
-                return st2;
-            }
-            struct SkipScope {  // the round number goes into the launch arguments
-                fpgm_ctx *c;
-                ~SkipScope() { c->skip_round = 0; }
-            } skip_scope{ctx};
-            ctx->skip_round = R.id;
-            Slot &sl = ctx->slot[R.slot];
-            int64_t i1 = 0;
-            fpgm_status st2 =
-                batch_submit(ctx, sl, R.tv.data(), nt, R.zpool.data(), nz, alpha, o.flags, band, 0, 0, 0, &i1, nullptr);
-            if (st2 != FPGM_OK) return st2;
-            if (i1 == nt) {
-                R.async = true;
-                return FPGM_OK;
-            }
-            // more than one launch sequence (workspace limit): finish the round here
-            R.res.resize((size_t)nt);
-            for (int64_t i0 = 0;;) {
-                st2 = batch_wait(ctx, sl);
-                if (st2 != FPGM_OK) return st2;
-                std::copy(sl.h_res.p, sl.h_res.p + (i1 - i0), R.res.begin() + i0);
-                i0 = i1;
-                if (i0 >= nt) break;
-                st2 = batch_submit(ctx, sl, R.tv.data(), nt, R.zpool.data(), nz, alpha, o.flags, band, 0, 0, i0, &i1,
-                                   nullptr);
-                if (st2 != FPGM_OK) return st2;
-            }
-            R.r = R.res.data();
-            return FPGM_OK;
-        };
-
-        // Takes a round's results in enumeration order: per edge, the tests up to its first
-        // independent one count (reference-equivalent); later ones are speculative; the tests
-        // of an edge done in an earlier round were skipped by the device and count nowhere.
-        auto process = [&](Round &R) -> fpgm_status {
-            struct ProcTimer {
-                double t0, &acc;
-                ~ProcTimer() { acc += now_s() - t0; }
-            } proc_timer{now_s(), t_process};
-            if (R.async) {
-                fpgm_status st2 = batch_wait(ctx, ctx->slot[R.slot]);
-                if (st2 != FPGM_OK) return st2;
-                R.r = ctx->slot[R.slot].h_res.p;
-            }
-            const DevResult *r = R.r;
-            double skipped_bytes = 0.0;
-            int64_t host_skipped = 0, dev_skipped = 0;
-            bool any_done = false;
-            for (size_t k = 0; k < R.tv.size();) {
-                const int32_t g = R.tv[k].edge;
-                size_t k1 = k;
-                while (k1 < R.tv.size() && R.tv[k1].edge == g) ++k1;
-                PcInst &I = P[active[g].first];
-                EdgeWalk &w = I.walks[active[g].second];
-                if (done_round[g] < R.id) {
-                    skipped_bytes += (double)(k1 - k) * (d + 2) * (double)ctx->inst[active[g].first].n_rows;
-                    if (ctx->trace)
-                        for (size_t j = k; j < k1; ++j) dev_skipped += (r[j].df == kSkippedDf);
-                    host_skipped += (int64_t)(k1 - k);
-                    k = k1;
-                    continue;
-                }
-                bool done = false;
-                for (; k < k1; ++k) {
-                    if (r[k].df == kSkippedDf)
-                        return fail(ctx, FPGM_EDEVICE, "internal: the device skipped a test of a live edge");
-                    ++executed;
-                    inb += r[k].in_band;
-                    if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
-                        S.max_rel_err_g2_fp32 =
-                            std::max(S.max_rel_err_g2_fp32, std::fabs((double)r[k].g2_32 - r[k].g2_64) / r[k].g2_64);
-                    if (done) continue;
-                    w.tests += 1;
-                    if (r[k].indep) {
-                        done = true;
-                        any_done = true;
-                        done_round[g] = R.id;
-                        w.removed = 1;
-                        w.zsep = (int64_t)I.zlevel.size();
-                        for (int i = 0; i < d; ++i) I.zlevel.push_back(R.zpool[R.tv[k].z_off + i]);
-                    }
-                }
-            }
-            ctx->k_bytes -= skipped_bytes;  // batch_wait counted every submitted test
-            if (ctx->trace)
-                fprintf(stderr, "[fpgm]   round %d: %zu tests, %lld skipped (device skipped %lld)\n", R.id, R.tv.size(),
-                        (long long)host_skipped, (long long)dev_skipped);
-            if (any_done)
-                live.erase(std::remove_if(live.begin(), live.end(),
-                                          [&](int64_t g) { return done_round[g] != INT32_MAX; }),
-                           live.end());
-            R.pending = false;
-            return FPGM_OK;
-        };
-
-        int cur = 0;
-        if (!live.empty()) {
-            st = build(rd[0]);
-            if (st != FPGM_OK) return st;
-        }
-        while (rd[cur].pending) {
-            Round &A = rd[cur], &B = rd[cur ^ 1];
-            const bool ahead = !plain && ctx->sched.pipeline;
-            if (ahead && !live.empty()) {  // round r+1 goes to the device behind round r
-                st = build(B);
-                if (st != FPGM_OK) return st;
-            }
-            st = process(A);
-            if (st != FPGM_OK) return st;
-            if (!ahead && !live.empty()) {
-                st = build(B);
-                if (st != FPGM_OK) return st;
-            }
-            cur ^= 1;
-        }
-        if (!plain) S.kernel_seconds_per_level[d] += (ctx->k_ms - kms_level0) * 1e-3;
-        int64_t tests = 0, left = 0;
-        for (PcInst &I : P) {
-            if (!I.cont) {
-                left += (int64_t)I.ex.size();
-                continue;
-            }
-            int64_t wkeep = 0, lt = 0;
-            for (size_t e = 0; e < I.walks.size(); ++e) {
-                const EdgeWalk &w = I.walks[e];
-                lt += w.tests;
-                if (w.removed) {
-                    I.seps.push_back(PcInst::Sep{w.x, w.y, d, d, (int64_t)I.sepz.size()});
-                    for (int i = 0; i < d; ++i) I.sepz.push_back(I.zlevel[w.zsep + i]);
-                } else {
-                    I.ex[wkeep] = w.x;
-                    I.ey[wkeep] = w.y;
-                    ++wkeep;
-                }
-            }
-            I.ex.resize(wkeep);
-            I.ey.resize(wkeep);
-            I.num_ci_test += lt;
-            tests += lt;
-            left += wkeep;
-            // FreeDegree (PCStable.cpp:620-626): continue while max degree - 1 > d
-            std::vector<int32_t> deg((size_t)n, 0);
-            for (size_t e = 0; e < I.ex.size(); ++e) {
-                ++deg[I.ex[e]];
-                ++deg[I.ey[e]];
-            }
-            const int maxdeg = n > 0 ? *std::max_element(deg.begin(), deg.end()) : 0;
-            I.cont = (maxdeg - 1 > d);
-        }
-        S.tests_per_level[d] = tests;
-        S.executed_per_level[d] = executed;
-        S.in_band_per_level[d] = inb;
-        S.rounds_per_level[d] = rounds;
-        S.edges_after_level[d] = left;
-        S.seconds_per_level[d] = now_s() - tl;
-        S.n_levels = d + 1;
-        log_level(o, d, tests, left, now_s() - t_start);
-        if (ctx->trace)
-            fprintf(stderr, "[fpgm]   level %d host: %.4f s making rounds (%.4f s descriptors), %.4f s taking "
-                    "results (%.4f s waiting), %.4f s level total\n", d, t_build, ctx->t_prep - prep0, t_process,
-                    ctx->t_wait - wait0, now_s() - tl);
+        if (snaps) snaps->push_back(P);  // the state level d starts from (autotuning replays it)
+        st = run_level(ctx, alpha, d, o, P, S, t_start);
+        if (st != FPGM_OK) return st;
     }
     S.ci_seconds = now_s() - t_start;
     for (int l = 0; l < S.n_levels; ++l) S.num_ci_test += S.tests_per_level[l];
@@ -1724,10 +1778,15 @@ fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const f
     st = run_pc(ctx, alpha, depth, o, P, S);
     if (st != FPGM_OK) return st;
     PcInst &I = P[0];
-    std::sort(I.seps.begin(), I.seps.end(),
-              [](const PcInst::Sep &a, const PcInst::Sep &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    // ascending (x, y): level 0's run is already in order, the later levels' removals (few)
+    // are sorted and merged into it
+    auto before = [](const PcInst::Sep &a, const PcInst::Sep &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; };
+    std::sort(I.seps.begin() + (std::ptrdiff_t)I.n_sep0, I.seps.end(), before);
+    std::inplace_merge(I.seps.begin(), I.seps.begin() + (std::ptrdiff_t)I.n_sep0, I.seps.end(), before);
     ctx->seps.clear();
     ctx->sep_z.clear();
+    ctx->seps.reserve(I.seps.size());
+    ctx->sep_z.reserve(I.sepz.size());
     for (const PcInst::Sep &s : I.seps) {
         fpgm_sepset r{};
         r.x = s.x;
@@ -1742,6 +1801,201 @@ fpgm_status fpgm_pc_skeleton(fpgm_ctx *ctx, double alpha, int32_t depth, const f
     S.sep_z_total = (int64_t)ctx->sep_z.size();
     std::memcpy(adj_out, I.adj.data(), I.adj.size());
     if (stats) *stats = S;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
+                          fpgm_level_tune *out, int32_t cap, int32_t *n_levels, double *seconds)
+{
+    if (!ctx || (cap > 0 && !out) || cap < 0) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->inst.size() != 1) return fail(ctx, FPGM_EINVAL, "fpgm_autotune needs a one-instance context");
+    fpgm_status st = check_stage_args(ctx, alpha, depth);
+    if (st != FPGM_OK) return st;
+    fpgm_pc_opts o{};
+    if (opts) o = *opts;
+    if (o.mode == FPGM_MODE_PLAIN) return fail(ctx, FPGM_EINVAL, "autotuning applies to the batched form");
+    const double t0 = now_s();
+    ctx->tune.clear();
+    ctx->tune_try = nullptr;
+
+    // one stage with the current settings, keeping the state each level starts from
+    std::vector<std::vector<PcInst>> snaps;
+    fpgm_stats S{};
+    std::vector<PcInst> P(1);
+    reset_kernel_stats(ctx);
+    const int64_t gram0 = ctx->g_launches;
+    st = run_pc(ctx, alpha, depth, o, P, S, &snaps);
+    if (st != FPGM_OK) return st;
+    const bool used_gram = ctx->g_launches > gram0;
+    const int L = S.n_levels;
+    auto current = [&](int lvl) {
+        fpgm_level_tune t{};
+        t.level = lvl;
+        t.pipeline = ctx->sched.pipeline;
+        t.items_per_cu = lvl == 0 ? ctx->sched.gram_items_per_cu : ctx->sched.items_per_cu;
+        for (int g = 0; g < kGroups; ++g) t.wg_per_cu[g] = ctx->sched.wpc[g];
+        t.max_slices = ctx->max_slices;
+        t.round_cap = ctx->sched.cap;
+        return t;
+    };
+    std::vector<fpgm_level_tune> res((size_t)L);
+    for (int l = 0; l < L; ++l) res[l] = current(l);
+
+    // level 0: the row split of the Gram tiles
+    if (used_gram) {
+        auto time_gram = [&](int64_t ipc) -> double {
+            double best = 1e30;
+            for (int r = 0; r < 3; ++r) {
+                const int64_t keep = ctx->sched.gram_items_per_cu;
+                ctx->sched.gram_items_per_cu = ipc;
+                const double ms0 = ctx->g_ms + ctx->g_pair_ms;
+                bool used = false;
+                const fpgm_status s2 = run_level0_gram(ctx, 0, alpha, o.flags, o.band > 0.f ? o.band : 0.02f, &used);
+                ctx->sched.gram_items_per_cu = keep;
+                if (s2 != FPGM_OK) return -1.0;
+                best = std::min(best, ctx->g_ms + ctx->g_pair_ms - ms0);
+            }
+            return best;
+        };
+        double best = time_gram(res[0].items_per_cu);
+        if (best < 0) return FPGM_EDEVICE;
+        res[0].default_ms = best;
+        for (int64_t ipc : {2, 4, 16, 32}) {
+            const double ms = time_gram(ipc);
+            if (ms < 0) return FPGM_EDEVICE;
+            if (ms < best * 0.98) {
+                best = ms;
+                res[0].items_per_cu = ipc;
+            }
+        }
+        res[0].tuned_ms = best;
+    } else {
+        res[0].level = -1;  // nothing to tune at level 0
+    }
+
+    // levels d >= 1: each replayed alone from its starting state under candidate settings;
+    // coordinate descent, a candidate kept when it is at least 2 % faster
+    fpgm_status fail_st = FPGM_OK;
+    auto time_level = [&](int d, const fpgm_level_tune &t, int reps) -> double {
+        double best = 1e30;
+        for (int r = 0; r < reps; ++r) {
+            std::vector<PcInst> Q = snaps[d - 1];
+            fpgm_stats S2{};
+            ctx->tune_try = &t;
+            const double t1 = now_s();
+            const fpgm_status s2 = run_level(ctx, alpha, d, o, Q, S2, t1);
+            const double dt = now_s() - t1;
+            ctx->tune_try = nullptr;
+            if (s2 != FPGM_OK) {
+                fail_st = s2;
+                return -1.0;
+            }
+            best = std::min(best, dt * 1e3);
+        }
+        return best;
+    };
+    for (int d = 1; d < L && d <= (int)snaps.size(); ++d) {
+        if (S.executed_per_level[d] == 0) {
+            res[d].level = -1;
+            continue;
+        }
+        const int reps = S.seconds_per_level[d] > 0.05 ? 1 : 3;
+        fpgm_level_tune cur = res[d];
+        double best = time_level(d, cur, reps);
+        if (best < 0) return fail_st;
+        cur.default_ms = best;
+        std::vector<fpgm_level_tune> cands;
+        auto try_all = [&]() -> fpgm_status {
+            for (const fpgm_level_tune &c : cands) {
+                const double ms = time_level(d, c, reps);
+                if (ms < 0) return fail_st;
+                if (ms < best * 0.98) {
+                    best = ms;
+                    cur = c;
+                    cur.default_ms = res[d].default_ms;
+                }
+            }
+            cands.clear();
+            return FPGM_OK;
+        };
+        res[d].default_ms = cur.default_ms;
+        {
+            fpgm_level_tune c = cur;
+            c.pipeline = 1 - cur.pipeline;
+            cands.push_back(c);
+        }
+        if ((st = try_all()) != FPGM_OK) return st;
+        for (int64_t ipc : {4, 8, 16, 32})
+            if (ipc != cur.items_per_cu) {
+                fpgm_level_tune c = cur;
+                c.items_per_cu = ipc;
+                cands.push_back(c);
+            }
+        if ((st = try_all()) != FPGM_OK) return st;
+        for (int g = 0; g < kGroups; ++g) {
+            for (int64_t f : {-1, 1}) {
+                fpgm_level_tune c = cur;
+                c.wg_per_cu[g] = f < 0 ? std::max<int64_t>(1, cur.wg_per_cu[g] / 2) : std::min<int64_t>(16, cur.wg_per_cu[g] * 2);
+                if (c.wg_per_cu[g] != cur.wg_per_cu[g]) cands.push_back(c);
+            }
+            if ((st = try_all()) != FPGM_OK) return st;
+        }
+        for (int64_t ms : {1, 2, 4, 16})
+            if (ms != cur.max_slices) {
+                fpgm_level_tune c = cur;
+                c.max_slices = ms;
+                cands.push_back(c);
+            }
+        if ((st = try_all()) != FPGM_OK) return st;
+        if (S.rounds_per_level[d] > 1)
+            for (int64_t rc : {4, 16})
+                if (rc != cur.round_cap) {
+                    fpgm_level_tune c = cur;
+                    c.round_cap = rc;
+                    cands.push_back(c);
+                }
+        if ((st = try_all()) != FPGM_OK) return st;
+        cur.level = d;
+        cur.tuned_ms = best;
+        res[d] = cur;
+    }
+    ctx->tune = res;
+    if (n_levels) *n_levels = L;
+    for (int l = 0; l < L && l < cap; ++l) out[l] = res[l];
+    if (seconds) *seconds = now_s() - t0;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_tune_set(fpgm_ctx *ctx, const fpgm_level_tune *levels, int32_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && !levels)) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::vector<fpgm_level_tune> t;
+    for (int32_t i = 0; i < n; ++i) {
+        const fpgm_level_tune &r = levels[i];
+        if (r.level < 0) continue;
+        if (r.level >= 64 || r.pipeline > 1 || r.items_per_cu < 0 || r.items_per_cu > 4096 || r.max_slices < 0 ||
+            r.round_cap < 0)
+            return fail(ctx, FPGM_EINVAL, "bad tuning record");
+        for (int g = 0; g < kGroups; ++g)
+            if (r.wg_per_cu[g] < 0 || r.wg_per_cu[g] > 64) return fail(ctx, FPGM_EINVAL, "bad tuning record");
+        if ((int)t.size() <= r.level) {
+            fpgm_level_tune none{};
+            none.level = -1;
+            t.resize((size_t)r.level + 1, none);
+        }
+        t[r.level] = r;
+    }
+    ctx->tune = t;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_tune_clear(fpgm_ctx *ctx)
+{
+    if (!ctx) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tune.clear();
     return FPGM_OK;
 }
 

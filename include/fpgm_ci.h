@@ -197,6 +197,31 @@ fpgm_status fpgm_sepset_get(const fpgm_ctx *ctx, fpgm_sepset *records, int64_t c
 fpgm_status fpgm_mle_counts(fpgm_ctx *ctx, int32_t n_nodes, const int32_t *par_off, const int32_t *parents,
                             int64_t *count_off, int32_t *counts, int64_t cap);
 
+/*
+ * Autotuning of the launch shapes and the round schedule (no counterpart in the reference,
+ * whose OpenMP schedule is fixed: PCStable.cpp:290-312).  Results never depend on these
+ * settings; only the time does.  fpgm_autotune runs the CI stage once with the current
+ * settings, keeping the state every level starts from, then replays each level alone under
+ * candidate settings (coordinate descent, each candidate timed on the device) and keeps the
+ * fastest per level in the context: later fpgm_pc_skeleton calls on the context use them.
+ * out (cap entries) receives one record per level; fpgm_tune_set applies saved records.
+ */
+typedef struct {
+    int32_t level;          /* 0: the MFMA Gram path; d >= 1: level d                       */
+    int32_t pipeline;       /* 1: round r+1 built while round r runs (-1: default)            */
+    int64_t items_per_cu;   /* work items a launch aims at per CU (level 0: Gram items)       */
+    int64_t wg_per_cu[5];   /* grid workgroups per CU: lane, lane-wide, shared, slice, device */
+    int64_t max_slices;     /* z slices of a table before the device-memory sink              */
+    int64_t round_cap;      /* cap of the per-edge round ramp                                  */
+    double default_ms;      /* the level's time with the settings before tuning                */
+    double tuned_ms;        /* with the chosen settings                                        */
+} fpgm_level_tune;
+
+fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opts *opts,
+                          fpgm_level_tune *out, int32_t cap, int32_t *n_levels, double *seconds);
+fpgm_status fpgm_tune_set(fpgm_ctx *ctx, const fpgm_level_tune *levels, int32_t n);
+fpgm_status fpgm_tune_clear(fpgm_ctx *ctx);
+
 /* chi-square helpers of the library's own FP64 CDF (the reference's stats::pchisq,
  * called at IndependenceTest.cpp:146,268,355), exposed for checks. */
 double fpgm_pchisq(double x, int32_t df);

@@ -344,3 +344,28 @@ def test_sliced_and_device_memory_sinks():
             assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
     assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
     np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+
+
+@pytest.mark.parametrize("key,alpha,depth,rows", [(2, 0.01, 5, 20_000), (3, 0.05, 4, 8_000)])
+def test_autotune_changes_time_only(key, alpha, depth, rows):
+    # the tuner replays every level under candidate launch shapes and round schedules;
+    # the tuned stage, a stage with the saved records and the cleared defaults all equal
+    # the oracle
+    w = configs.get(key)
+    s = w.sample(rows)
+    o = O.pc_skeleton(O.Data.of(s), alpha, depth, n_threads=8)
+    with _ctx(s) as ctx:
+        t = ctx.autotune(alpha, depth)
+        assert t["levels"] and t["seconds"] > 0
+        for r in t["levels"]:
+            assert r["tuned_ms"] <= r["default_ms"] + 1e-9
+            assert r["items_per_cu"] > 0 and min(r["wg_per_cu"]) >= 1 and r["pipeline"] in (0, 1)
+        a = ctx.pc_skeleton(alpha, depth)
+        ctx.tune_clear()
+        b = ctx.pc_skeleton(alpha, depth)
+        ctx.tune_set(t["levels"])
+        c = ctx.pc_skeleton(alpha, depth)
+        with pytest.raises(_lib.FpgmError):
+            ctx.tune_set([dict(t["levels"][0], level=99)])
+    for g in (a, b, c):
+        _assert_same(g, o)
