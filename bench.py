@@ -83,6 +83,80 @@ def cpu_baseline(sample, w, g, seconds, threads):
                       f"(est. {est:.1f} s for {g['num_ci_test']} tests)"}
 
 
+def bench_ensemble(args, w):
+    """The bootstrap ensemble (512 replicates of one base sample, 2 000 - 50 000 rows each):
+    every replicate's CI stage, level-synchronously in one context (fpgm_pc_skeleton_multi).
+    The depth is bounded (--depth, default 3 = levels 0..2): duplicated rows inflate G^2 in
+    bootstrap replicates, so deeper levels grow to billions of tests (2.15e9 at depth 4)."""
+    from oracle import oracle as O
+    depth = args.depth or 3
+    base = w.sample()
+    reps = list(w.replicates(base))
+    rows = sum(r.n_rows for r in reps)
+    log(f"workload {w.name}: {len(reps)} replicates, {rows} rows in all, depth {depth}")
+    ctx = _lib.Context.of_many(reps)
+    for i in range(args.warmup):
+        g = ctx.pc_skeleton_multi(w.alpha, depth, want_adj=False)
+        log(f"warmup {i}: {g['num_ci_test']} tests in {g['ci_seconds']:.3f} s")
+    total, kernel_ms, alg_bytes, launches = 0, 0.0, 0.0, 0
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        g = ctx.pc_skeleton_multi(w.alpha, depth, want_adj=False)
+        ks = ctx.kernel_stats()
+        total += g["num_ci_test"]
+        kernel_ms += ks["kernel_ms"]
+        alg_bytes += ks["alg_bytes"]
+        launches += ks["launches"]
+    elapsed = time.perf_counter() - t_start
+    value = total / elapsed
+    freq = g["edge_freq"]
+    log(f"timed {args.steps} steps: {elapsed:.3f} s, {value:.0f} tests/s; per level {g['tests_per_level']}")
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    out = {
+        "metric": "CI tests/sec (reference-equivalent, PC-stable CI stage, G2)",
+        "value": value, "unit": "CI tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (seeded 100-variable DAG, Dirichlet(1) CPTs, 50 000-row base sample, "
+                "512 bootstrap replicates)",
+        "config": {"workload": w.name, "n_vars": w.n_vars, "replicates": len(reps), "rows_total": rows,
+                   "rep_rows": list(w.rep_rows), "alpha": w.alpha, "depth": depth},
+        "num_ci_test": g["num_ci_test"], "tests_per_level": g["tests_per_level"],
+        "executed_per_level": g["executed_per_level"],
+        "edge_freq_sum": int(freq.sum()), "edge_freq_max": int(freq.max()),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_count_stat",
+                     "kernel_ms_per_step": kernel_ms / args.steps, "launches_per_step": launches / args.steps,
+                     "alg_bytes_per_step": alg_bytes / args.steps},
+    }
+    if not args.no_plain:
+        gp = ctx.pc_skeleton_multi(w.alpha, depth, mode=_lib.FPGM_MODE_PLAIN, plain_sample=args.plain_sample,
+                                   want_adj=False)
+        est = sum(t * ps / pt for t, pt, ps in zip(gp["tests_per_level"], gp["plain_tests_per_level"],
+                                                 gp["plain_seconds_per_level"]) if t and pt)
+        plain_rate = gp["num_ci_test"] / est if est > 0 else None
+        same = bool((gp["edge_freq"] == freq).all())
+        out["plain"] = {"value": plain_rate, "unit": "CI tests/s", "est_stage_seconds": est,
+                        "sample": f"first {args.plain_sample} tests of each level, one launch + host sync each",
+                        "edge_freq_identical": same}
+        out["batched_over_plain"] = value / plain_rate if plain_rate else None
+        log(f"plain form: {plain_rate:.0f} tests/s (sampled), identical edge frequencies: {same}")
+    if not args.no_cpu:
+        # the oracle on whole replicates, one after another, until about --cpu-seconds
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        k, tests, t0 = 0, 0, time.perf_counter()
+        while k < len(reps) and time.perf_counter() - t0 < args.cpu_seconds:
+            o = O.pc_skeleton(O.Data.of(reps[k]), w.alpha, depth, n_threads=threads)
+            tests += o["num_ci_test"]
+            k += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": tests / dt, "unit": "CI tests/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle PC-stable (depth {depth}) on the first {k} replicates: "
+                                         f"{tests} tests in {dt:.1f} s on {threads} threads"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def hbm_traffic_from_counters(fetch_csv, write_csv, stages, copy_bytes=1 << 30):
     """HBM bytes moved by the counting kernels per CI stage, from rocprofv3 --pmc
     FETCH_SIZE and WRITE_SIZE counter_collection.csv files (KiB per dispatch) of a
@@ -122,11 +196,125 @@ def summarize_pmc(pmc_dir):
               "hbm_gbs_measured": (t["fetch_bytes"] + t["write_bytes"]) / (kms * 1e-3) / 1e9})
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     rec = json.load(open(path)) if os.path.exists(path) else {}
-    rec[run["config"]["workload"]] = t
+    name = run["config"]["workload"]
+    rec[name] = t
     with open(path, "w") as fh:
         json.dump(rec, fh, indent=1)
         fh.write("\n")
-    print(json.dumps({run["config"]["workload"]: t}, indent=1))
+    lines = [
+        f"k_count_stat -- HBM traffic, workload {name}, per CI stage",
+        "Commands (GPU box, repository root, TMPDIR=/tmp), one counter per run, tuning records from a",
+        "bench run with --save-tune:",
+        "  rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/pmc_w3/fetch -o w3 -- python bench.py --steps 1 \\",
+        "      --warmup 0 --no-cpu --no-plain --copy-calibration --tune-file gpurun_out/tune_w3.json",
+        "  rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_w3/write -o w3 -- (the same bench command)",
+        "  (the FETCH run's JSON line saved as gpurun_out/pmc_w3/run.json), then, on any host:",
+        "  python bench.py --summarize-pmc gpurun_out/pmc_w3     -> profiles/hbm_traffic.json and this file",
+        f"Each profiled run executes {run['stages_in_run']} CI stages; totals are divided by that.",
+        "Calibration: --copy-calibration runs one 1 GiB device-to-device hipMemcpy (__amd_rocclr_copyBuffer);",
+        f"  each counter is scaled by 1 GiB over what it read for that copy: fetch scale "
+        f"{t['fetch_scale']:.4f}, write scale {t['write_scale']:.4f}.",
+        f"Counting kernels (k_count_stat<0..4>), {t['dispatches']:.0f} dispatches per stage:",
+        f"  fetch  {t['fetch_bytes']:.4g} bytes, write {t['write_bytes']:.4g} bytes (scaled)",
+        f"  total  {t['fetch_bytes'] + t['write_bytes']:.4g} bytes = {t['traffic_over_alg']:.3f} x the algorithmic "
+        f"bytes {alg:.4g}",
+        f"  per dispatch {t['bytes_per_launch']:.4g} bytes",
+        f"  kernel time {kms:.1f} ms per stage -> measured HBM rate {t['hbm_gbs_measured']:.0f} GB/s of 8000.",
+        "Reading: the traffic is below the algorithmic bytes because tests launched together share",
+        "columns (every level-1 test of an edge scans x and y; concurrent tests share z columns) and the",
+        "second read of a column comes from L2 / MALL.  HBM does not bound these kernels: the per-row",
+        "work (key arithmetic and one LDS read-modify-write per row and test) does -- the SQ counter",
+        "profile has the instruction mix.",
+    ]
+    with open(os.path.join(ROOT, "profiles", f"k_count_stat_{name}_hbm_traffic.txt"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(json.dumps({name: t}, indent=1))
+
+
+def _kernel_short(name):
+    return name.replace("void ", "").replace("fpgm::(anonymous namespace)::", "").split("(")[0]
+
+
+def summarize_trace(trace_dir, out_path):
+    """Kernel-trace profile text from DIR/*kernel_stats.csv (rocprofv3 --kernel-trace --stats) and
+    DIR/run.json (the profiled run's JSON line).  No GPU use."""
+    import csv
+    import glob
+    run = json.load(open(os.path.join(trace_dir, "run.json")))
+    stats = glob.glob(os.path.join(trace_dir, "*kernel_stats.csv"))[0]
+    rows = list(csv.DictReader(open(stats)))
+    stages = run["stages_in_run"]
+    tot = {_kernel_short(r["Name"]): (float(r["TotalDurationNs"]) / 1e6, int(r["Calls"])) for r in rows}
+    counting = [k for k in tot if k.startswith(("k_count_stat", "k_fin_wide", "k_compact"))]
+    cnt_ms = sum(tot[k][0] for k in counting) / stages
+    cnt_calls = sum(tot[k][1] for k in counting) / stages
+    rl = run["roofline"]
+    gram = next((v for k, v in tot.items() if k.startswith("k_gram_l0")), (0.0, 0))
+    lines = [
+        f"k_count_stat / k_gram_l0 -- kernel trace, workload {run['config']['workload']}",
+        "Command (GPU box, repository root, TMPDIR=/tmp; tuning records from a bench run with --save-tune):",
+        "  rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_kt -o w3 -- \\",
+        "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --copy-calibration \\",
+        "      --tune-file gpurun_out/tune_w3.json",
+        f"  then: python bench.py --summarize-trace gpurun_out/prof_kt   (run.json = that run's JSON line)",
+        f"The run executes the CI stage {stages} times (timed step + read-back run); per-stage figures are",
+        "the trace totals divided by that.",
+        "  k_count_stat<2>  per-lane 16-bit LDS sink, tables <= 64 cells",
+        "  k_count_stat<3>  per-lane 16-bit LDS sink, tables <= 128 cells (64 KiB LDS)",
+        "  k_count_stat<0>  shared LDS histogram, tables <= 8192 cells",
+        "  k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
+        "  k_count_stat<1>  device-memory sink (larger tables)",
+        "  k_fin_wide_*     grid-wide finalize of device-memory tables",
+        "  k_compact        per-round removal of tests whose edge an earlier round decided",
+        "  k_gram_l0        level 0 as the int8 one-hot Gram matrix (v_mfma_i32_32x32x32_i8)",
+        "  k_pairs_l0, k_pack_l0  level-0 G^2 / decision and result packing",
+        f"Counting launches (k_count_stat + k_fin_wide + k_compact): {cnt_ms:.2f} ms per stage, "
+        f"{cnt_calls:.0f} dispatches per stage;",
+        f"bench.py's HIP-event figure for the same run: {rl['kernel_ms_per_step']:.2f} ms per stage.",
+        f"Algorithmic bytes per stage (sum over executed tests of (|Z|+2) * N): {rl['alg_bytes_per_step']:.4g}",
+        f"  -> achieved {rl['alg_bytes_per_step'] / (cnt_ms * 1e-3) / 1e9:.0f} GB/s by the trace, "
+        f"{rl['achieved']:.0f} GB/s by the events (peak {rl['peak']:.0f}); the effective rate exceeds the",
+        "     HBM peak because tests launched together share columns through L2 / MALL (measured HBM",
+        "     traffic: the hbm_traffic profile).",
+    ]
+    if gram[1]:
+        g_ms = gram[0] / gram[1]
+        lines.append(f"Level 0 (k_gram_l0): {g_ms:.2f} ms per launch; 1.73e13 int8 ops per launch at "
+                     f"1000 vars (W = 4075 one-hot columns padded to 4096, 528 tiles of 128 x 128, 2 N each)")
+        lines.append(f"  -> {1.7301504e13 / (g_ms * 1e-3) / 1e15:.2f} POP/s of the 5.0 POP/s dense int8 peak.")
+    lines += ["", open(stats).read()]
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines))
+    print("\n".join(lines[:30]))
+
+
+def summarize_sq(sq_dir, out_path):
+    """SQ instruction-counter profile text from DIR/*counter_collection.csv.  No GPU use."""
+    import collections
+    import csv
+    import glob
+    path = glob.glob(os.path.join(sq_dir, "*counter_collection.csv"))[0]
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    for r in csv.DictReader(open(path)):
+        k = _kernel_short(r["Kernel_Name"])
+        agg[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp[k].add(r["Dispatch_Id"])
+    names = ["SQ_WAVES", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS",
+             "SQ_INSTS_VMEM_RD", "SQ_LDS_BANK_CONFLICT"]
+    lines = ["kernel_stats of SQ counters (totals over all dispatches of each kernel)",
+             "Command (GPU box, repository root, TMPDIR=/tmp):",
+             "  rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES \\",
+             "      SQ_INSTS_VMEM_RD SQ_LDS_BANK_CONFLICT SQ_INSTS_SALU -f csv -d gpurun_out/pmc_sq -o sq -- \\",
+             "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --tune-file gpurun_out/tune_w3.json",
+             "  then: python bench.py --summarize-sq gpurun_out/pmc_sq",
+             "kernel                 disp " + " ".join(f"{n[3:]:>17s}" for n in names)]
+    for k in sorted(agg, key=lambda k: -agg[k]["SQ_BUSY_CYCLES"]):
+        d = agg[k]
+        lines.append(f"{k:22s} {len(disp[k]):4d} " + " ".join(f"{d[n]:17.4g}" for n in names))
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def copy_calibration(n_bytes=1 << 30):
@@ -153,6 +341,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=3)
     ap.add_argument("--plain-sample", type=int, default=2000, help="plain-form tests per level")
+    ap.add_argument("--depth", type=int, default=0, help="depth limit (bootstrap ensemble only; default 3)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -160,6 +349,11 @@ def main():
     ap.add_argument("--summarize-pmc", default="", metavar="DIR",
                     help="only write profiles/hbm_traffic.json from DIR/{fetch,write}/ rocprofv3 "
                          "--pmc CSVs and DIR/run.json (the profiled run's output line); no GPU use")
+    ap.add_argument("--summarize-trace", default="", metavar="DIR",
+                    help="only write the kernel-trace profile from DIR (rocprofv3 --kernel-trace --stats output "
+                         "and run.json); no GPU use")
+    ap.add_argument("--summarize-sq", default="", metavar="DIR",
+                    help="only write the SQ counter profile from DIR (rocprofv3 --pmc SQ_* output); no GPU use")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -171,6 +365,13 @@ def main():
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc)
         return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace,
+                        os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_kernel_trace.txt"))
+        return
+    if args.summarize_sq:
+        summarize_sq(args.summarize_sq, os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_sq_counters.txt"))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the CI stage of one PC-stable run is not sharded across GPUs in "
@@ -180,6 +381,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
     w = configs.get(args.workload)
+    if w.n_replicates:
+        bench_ensemble(args, w)
+        return
     t0 = time.perf_counter()
     sample = w.sample()
     log(f"workload {w.name}: {sample.n_vars} vars x {sample.n_rows} rows generated in "
