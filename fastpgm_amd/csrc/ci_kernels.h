@@ -137,8 +137,10 @@ hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s);
 hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
                                 const WideTest *wt, int64_t n_wide, hipStream_t s);
 // p-values of the rechecked tests; with edge_done, an independent one marks its edge (round).
+// It also zeroes `clear` (n_clear counters: the launch sequence's work queues) for the next
+// sequence on the stream.
 hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests, int32_t *edge_done,
-                         int32_t round, hipStream_t s);
+                         int32_t round, uint32_t *clear, int n_clear, hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);
 

@@ -16,6 +16,8 @@
 // max(band * crit, error bound) of the chi-square critical value (or df exceeds the
 // tabulated range, or the caller asked for exact results) the workgroup recomputes
 // G^2 in FP64 with the reference's expression and evaluates the FP64 p-value.
+#include <algorithm>
+
 #include "ci_kernels.h"
 #include "chisq.h"
 #include "fpgm_ci.h"
@@ -994,9 +996,10 @@ __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64
 // FP64 p-value of the tests the statistic kernel rechecked (one thread per test):
 // p = 1 - pchisq(G^2, df), independent iff p > alpha (IndependenceTest.cpp:146-154).
 __global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests,
-                                                 int32_t *edge_done, int32_t round)
+                                                 int32_t *edge_done, int32_t round, uint32_t *clear, int n_clear)
 {
     const int64_t t = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (clear && blockIdx.x == 0 && threadIdx.x < n_clear) clear[threadIdx.x] = 0u;
     if (t >= n) return;
     DevResult &R = res[t];
     if (!R.need_p) return;
@@ -1045,11 +1048,12 @@ hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int6
 }
 
 hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests, int32_t *edge_done,
-                         int32_t round, hipStream_t s)
+                         int32_t round, uint32_t *clear, int n_clear, hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
-    const int64_t blocks = (n + kWG - 1) / kWG;
-    hipLaunchKernelGGL(k_pvalue, dim3((unsigned)blocks), dim3(kWG), 0, s, res, n, alpha, tests, edge_done, round);
+    if (n <= 0 && !clear) return hipSuccess;
+    const int64_t blocks = std::max<int64_t>(1, (n + kWG - 1) / kWG);
+    hipLaunchKernelGGL(k_pvalue, dim3((unsigned)blocks), dim3(kWG), 0, s, res, n, alpha, tests, edge_done, round,
+                       clear, n_clear);
     return hipGetLastError();
 }
 

@@ -348,7 +348,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     need.resize((size_t)m);
     int too_big = 0;
     const int nth = (m >= kParallelTests) ? ctx->host_threads : 1;
-#pragma omp parallel for num_threads(nth) schedule(static) reduction(| : too_big)
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static) reduction(| : too_big)
     for (int64_t k = 0; k < m; ++k) {
         const HostTest &h = tests[i0 + k];
         DevTest T{};
@@ -447,7 +447,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         acc += grp_len[gi];
     }
     const int nth2 = (nb >= kParallelTests) ? ctx->host_threads : 1;
-#pragma omp parallel for num_threads(nth2) schedule(static)
+#pragma omp parallel for if (nth2 > 1) num_threads(nth2) schedule(static)
     for (int64_t k = 0; k < nb; ++k) {
         dt[k].item0 += (int32_t)grp_begin[gidx[k]];
         const int64_t cnt = (int64_t)dt[k].n_chunks * dt[k].n_slices;
@@ -493,9 +493,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
     HIP_TRY(ctx->d_done.ensure(std::max<int64_t>(nb, 1024), true, ctx->stream), "alloc counters");
     HIP_TRY(S.d_res.ensure(nb, false, ctx->stream), "alloc results");
-    HIP_TRY(S.d_queue.ensure(kGroups * (kXcds + 1), false, ctx->stream), "alloc work queues");
-    HIP_TRY(hipMemsetAsync(S.d_queue.p, 0, sizeof(uint32_t) * kGroups * (kXcds + 1), ctx->stream),
-            "clear work queues");
+    if (!S.d_queue.p) {  // zero once here; afterwards every sequence's k_pvalue re-zeroes them
+        HIP_TRY(S.d_queue.ensure(kGroups * (kXcds + 1), false, ctx->stream), "alloc work queues");
+        HIP_TRY(hipMemsetAsync(S.d_queue.p, 0, sizeof(uint32_t) * S.d_queue.cap, ctx->stream), "clear work queues");
+    }
     HIP_TRY(S.h_res.ensure(nb, ctx->stream), "alloc pinned results");
     if (zlen > 0)
         HIP_TRY(hipMemcpyAsync(S.d_zpool.p, S.h_zpool.p, sizeof(int32_t) * zlen, hipMemcpyHostToDevice, ctx->stream),
@@ -565,7 +566,9 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     }
     if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[kGroups], ctx->stream), "event");
     HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
-    HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, S.d_tests.p, a.edge_done, a.round, ctx->stream), "launch pvalue");
+    HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, S.d_tests.p, a.edge_done, a.round, S.d_queue.p, kGroups * (kXcds + 1),
+                          ctx->stream),
+            "launch pvalue");
     HIP_TRY(hipMemcpyAsync(S.h_res.p, S.d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost, ctx->stream),
             "download results");
     HIP_TRY(hipEventRecord(S.ev_done, ctx->stream), "event");
@@ -752,7 +755,7 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
     HIP_TRY(hipEventRecord(e_mid, ctx->stream), "event");
     HIP_TRY(launch_pairs_l0(p, ctx->stream), "launch pairs");
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
-    HIP_TRY(launch_pvalue(S.d_res.p, np, alpha, nullptr, nullptr, 0, ctx->stream), "launch pvalue");
+    HIP_TRY(launch_pvalue(S.d_res.p, np, alpha, nullptr, nullptr, 0, nullptr, 0, ctx->stream), "launch pvalue");
     HIP_TRY(hipMemsetAsync(ctx->d_l0_sum.p, 0, sizeof(PairSummary), ctx->stream), "clear level-0 summary");
     HIP_TRY(launch_pack_l0(S.d_res.p, np, ctx->d_l0_flags.p, ctx->d_l0_sum.p, ctx->stream), "launch pack");
     HIP_TRY(hipMemcpyAsync(ctx->h_l0_flags.p, ctx->d_l0_flags.p, (size_t)np, hipMemcpyDeviceToHost, ctx->stream),
@@ -1415,7 +1418,7 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
         const int64_t L = (int64_t)live.size();
         const int nth = (L * W >= kParallelTests) ? ctx->host_threads : 1;
         gen.resize((size_t)nth);
-#pragma omp parallel for num_threads(nth) schedule(static)
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
         for (int ch = 0; ch < nth; ++ch) {
             Gen &G = gen[ch];
             G.tv.clear();
