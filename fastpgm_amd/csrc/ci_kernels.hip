@@ -285,15 +285,18 @@ __device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
 }
 
 // Counts the (up to) 4 rows whose keys are the bytes of K; rows >= nvalid are skipped.
-template <int kCells>
-__device__ __forceinline__ void lane_put4(char *lb, uint32_t base, uint32_t inc, uint32_t K, int nvalid)
+// A row's byte offset in its wave pair's block is key * 256 + lane * 4: one v_perm_b32
+// puts key byte j of K into byte 1 and lane * 4 (lane4, byte 0) into byte 0; the block of
+// the wave pair is the instruction's immediate offset (kPairOff).
+template <uint32_t kPairOff>
+__device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, int nvalid)
 {
-    constexpr uint32_t kM = (uint32_t)(kCells - 1) << 8;
-    const uint32_t off[4] = {((K << 8) & kM) | base, (K & kM) | base, ((K >> 8) & kM) | base,
-                             ((K >> 16) & kM) | base};
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        if (j < nvalid) atomicAdd(reinterpret_cast<uint32_t *>(lb + off[j]), inc);
+        if (j < nvalid) {
+            const uint32_t off = __builtin_amdgcn_perm(K, lane4, 0x0C0C0000u | ((4u + (uint32_t)j) << 8));
+            atomicAdd(reinterpret_cast<uint32_t *>(lb + kPairOff + off), inc);
+        }
 }
 
 // Packed keys of the 4 rows of word q, |Z| = D known at compile time.
@@ -309,9 +312,9 @@ __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *
     return pk_mad(K, cy2, word_of(c.v[X + 1], q));
 }
 
-template <int D, int kCells>
+template <int D, uint32_t kPairOff>
 __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTest &T, const Shared &sh,
-                                                  int64_t r_begin, int64_t r_end, char *lb, uint32_t base,
+                                                  int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
                                                   uint32_t inc)
 {
     const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
@@ -329,11 +332,11 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lane_put4<kCells>(lb, base, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), 4);
+            for (int q = 0; q < 4; ++q) lane_put4<kPairOff>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), 4);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                lane_put4<kCells>(lb, base, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), (int)nv - 4 * q);
+                lane_put4<kPairOff>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), (int)nv - 4 * q);
         }
     };
     Cols<D> b0, b1;
@@ -351,9 +354,9 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
 }
 
 // Runtime |Z| (d > 3).
-template <int kCells>
+template <uint32_t kPairOff>
 __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevTest &T, const Shared &sh,
-                                                    int64_t r_begin, int64_t r_end, char *lb, uint32_t base,
+                                                    int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
                                                     uint32_t inc)
 {
     const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
@@ -374,27 +377,36 @@ __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevT
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q));
-            lane_put4<kCells>(lb, base, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
+            lane_put4<kPairOff>(lb, lane4, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
         }
     }
 }
 
 // blk: the workgroup's counters, [pair][cell][lane] 32-bit words, kCells cells a pair.
+template <uint32_t kPairOff>
+__device__ __forceinline__ void count_rows_lane_pair(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                     int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
+                                                     uint32_t inc)
+{
+    switch (T.d) {
+    case 0: count_rows_lane_d<0, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
+    case 1: count_rows_lane_d<1, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
+    case 2: count_rows_lane_d<2, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
+    case 3: count_rows_lane_d<3, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
+    default: count_rows_lane_any<kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
+    }
+}
+
 template <int kCells>
 __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest &T, const Shared &sh,
                                                 int64_t r_begin, int64_t r_end, uint32_t *blk)
 {
     char *lb = reinterpret_cast<char *>(blk);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t base = ((wave >> 1) * kCells * 64 + lane) * 4;
     const uint32_t inc = (wave & 1) ? 0x10000u : 1u;
-    switch (T.d) {
-    case 0: count_rows_lane_d<0, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
-    case 1: count_rows_lane_d<1, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
-    case 2: count_rows_lane_d<2, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
-    case 3: count_rows_lane_d<3, kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
-    default: count_rows_lane_any<kCells>(I, T, sh, r_begin, r_end, lb, base, inc); break;
-    }
+    // the wave pair's block is a compile-time offset: one branch per wave, uniform
+    if (wave >> 1) count_rows_lane_pair<(uint32_t)kCells * 256u>(I, T, sh, r_begin, r_end, lb, lane * 4u, inc);
+    else count_rows_lane_pair<0u>(I, T, sh, r_begin, r_end, lb, lane * 4u, inc);
 }
 
 template <int kSink>
