@@ -68,12 +68,14 @@ class Stats(C.Structure):
 
 
 class LevelTune(C.Structure):
-    _fields_ = [("level", C.c_int32), ("pipeline", C.c_int32), ("items_per_cu", C.c_int64),
+    _fields_ = [("level", C.c_int32), ("pipeline", C.c_int32), ("concurrent", C.c_int32), ("pad", C.c_int32),
+                ("items_per_cu", C.c_int64),
                 ("wg_per_cu", C.c_int64 * 5), ("max_slices", C.c_int64), ("round_cap", C.c_int64),
                 ("default_ms", C.c_double), ("tuned_ms", C.c_double)]
 
     def as_dict(self):
-        return {"level": self.level, "pipeline": self.pipeline, "items_per_cu": self.items_per_cu,
+        return {"level": self.level, "pipeline": self.pipeline, "concurrent": self.concurrent,
+                "items_per_cu": self.items_per_cu,
                 "wg_per_cu": list(self.wg_per_cu), "max_slices": self.max_slices, "round_cap": self.round_cap,
                 "default_ms": self.default_ms, "tuned_ms": self.tuned_ms}
 
@@ -81,6 +83,7 @@ class LevelTune(C.Structure):
     def from_dict(cls, d):
         t = cls()
         t.level, t.pipeline, t.items_per_cu = int(d["level"]), int(d["pipeline"]), int(d["items_per_cu"])
+        t.concurrent = int(d.get("concurrent", -1))
         for g in range(5):
             t.wg_per_cu[g] = int(d["wg_per_cu"][g])
         t.max_slices, t.round_cap = int(d["max_slices"]), int(d["round_cap"])

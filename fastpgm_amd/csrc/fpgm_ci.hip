@@ -141,7 +141,9 @@ struct Slot {
     std::vector<int32_t> mode_of;
     std::vector<int64_t> need;  // workspace int32 entries of each test of the batch being built
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
-    hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing
+    hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing: start of each group
+    hipEvent_t ev_gend[kGroups] = {};     // and its end (groups may run concurrently)
+    hipEvent_t ev_fork = nullptr, ev_join[kGroups] = {};  // concurrent sink launches
     bool pending = false;
     int64_t n = 0;       // tests of the pending launch sequence
     double bytes = 0.0;  // their algorithmic bytes
@@ -157,6 +159,11 @@ struct Slot {
             if ((e = hipEventCreate(ev)) != hipSuccess) return e;
         for (hipEvent_t &ev : ev_grp)
             if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+        for (hipEvent_t &ev : ev_gend)
+            if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+        if ((e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming)) != hipSuccess) return e;
+        for (hipEvent_t &ev : ev_join)
+            if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
         return e;
     }
     void release()
@@ -180,6 +187,11 @@ struct Slot {
             if (ev) (void)hipEventDestroy(ev);
         for (hipEvent_t ev : ev_grp)
             if (ev) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : ev_gend)
+            if (ev) (void)hipEventDestroy(ev);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (hipEvent_t ev : ev_join)
+            if (ev) (void)hipEventDestroy(ev);
     }
 };
 
@@ -200,6 +212,7 @@ struct InstHost {
 struct fpgm_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {};  // streams of a round's slice and device-memory sink launches
     const uint8_t *d_cols = nullptr;  // all instances' columns (owned unless borrowed)
     bool owns_cols = false;
     std::vector<InstHost> inst;       // dataset instances (1, or the bootstrap ensemble)
@@ -220,6 +233,7 @@ struct fpgm_ctx {
         int64_t wpc[5] = {4, 2, 4, 2, 8};   // grid workgroups per CU: lane, lane-wide, shared, slice, device
         int pipeline = 1;                    // build round r+1 while the device counts round r
         int64_t gram_items_per_cu = 8;       // level-0 Gram work items (tile pairs x row splits) per CU
+        int concurrent = 1;                  // slice and device-memory sinks on streams of their own
     } sched;
     // per-level scheduling and launch shapes chosen by fpgm_autotune (index = level); a
     // candidate being timed by the tuner overrides them
@@ -544,9 +558,27 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         HIP_TRY(launch_compact(c, max_len, ctx->stream), "launch compact");
         a.items = S.d_items_live.p;
     }
+    // one launch per sink group.  The LDS sinks (lane, lane-wide, shared) run one after
+    // another on the context stream: run together they compete for the same CUs' LDS and
+    // measured slower.  The slice and device-memory sinks, bound by LDS capacity and device
+    // atomics rather than by the CUs they occupy, get a stream each, so that they overlap
+    // the rest; the context stream joins them before the p-value kernel.  Tests of
+    // different groups touch disjoint workspace, counters and results.
+    int n_grp = 0;
+    for (int gi = 0; gi < kGroups; ++gi) n_grp += (grp_len[gi] > 0);
+    const bool concurrent = ctx->sched.concurrent && n_grp > 1;
+    if (concurrent) HIP_TRY(hipEventRecord(S.ev_fork, ctx->stream), "event");
+    bool joined[kGroups] = {false, false, false, false, false};
     for (int gi = 0; gi < kGroups; ++gi) {
-        if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[gi], ctx->stream), "event");
         if (grp_len[gi] == 0) continue;
+        hipStream_t gs = ctx->stream;
+        const int side = kGroupSink[gi] == kSinkSlice ? 0 : kGroupSink[gi] == kSinkGlobal ? 1 : -1;
+        if (concurrent && side >= 0) {
+            gs = ctx->side[side];
+            HIP_TRY(hipStreamWaitEvent(gs, S.ev_fork, 0), "stream wait");
+            joined[gi] = true;
+        }
+        if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[gi], gs), "event");
         a.item_begin = grp_begin[gi];
         a.n_items = grp_len[gi];
         a.queue = S.d_queue.p + gi * kXcds;
@@ -555,16 +587,18 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         const int64_t wg_per_cu = ctx->sched.wpc[gi];
         int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wg_per_cu);
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
-        HIP_TRY(launch_count_stat(a, sink, (int)grid, ctx->stream), "launch count_stat");
+        HIP_TRY(launch_count_stat(a, sink, (int)grid, gs), "launch count_stat");
         ctx->k_launches += 1;
         if (sink == kSinkGlobal && n_wt > 0) {
-            HIP_TRY(launch_finalize_wide(a, S.d_wide_items.p, n_wi, S.d_wide_parts.p, S.d_wide_tests.p, n_wt,
-                                         ctx->stream),
+            HIP_TRY(launch_finalize_wide(a, S.d_wide_items.p, n_wi, S.d_wide_parts.p, S.d_wide_tests.p, n_wt, gs),
                     "launch finalize");
             ctx->k_launches += 2;
         }
+        if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_gend[gi], gs), "event");
+        if (joined[gi]) HIP_TRY(hipEventRecord(S.ev_join[gi], gs), "event");
     }
-    if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[kGroups], ctx->stream), "event");
+    for (int gi = 0; gi < kGroups; ++gi)
+        if (joined[gi]) HIP_TRY(hipStreamWaitEvent(ctx->stream, S.ev_join[gi], 0), "stream wait");
     HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
     HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, S.d_tests.p, a.edge_done, a.round, S.d_queue.p, kGroups * (kXcds + 1),
                           ctx->stream),
@@ -613,7 +647,7 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
             const Slot::GroupInfo &G = S.grp[gi];
             if (G.items == 0) continue;
             float gms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&gms, S.ev_grp[gi], S.ev_grp[gi + 1]), "event time");
+            HIP_TRY(hipEventElapsedTime(&gms, S.ev_grp[gi], S.ev_gend[gi]), "event time");
             fprintf(stderr,
                     "[fpgm]   launch %-9s tests %7lld items %8lld mean |Z| %.2f cells mean %.0f max %.0f: "
                     "%8.3f ms, %7.0f GB/s\n",
@@ -933,6 +967,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_ROUND_PROMOTE")) ctx->sched.promote = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_MAX_SLICES")) ctx->max_slices = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_PIPELINE")) ctx->sched.pipeline = std::atoi(e) != 0;
+    if (const char *e = std::getenv("FPGM_CONCURRENT")) ctx->sched.concurrent = std::atoi(e) != 0;
     if (const char *e = std::getenv("FPGM_ROUND_BYTES")) ctx->sched.round_bytes = std::max(1e6, std::atof(e));
     if (const char *e = std::getenv("FPGM_ITEMS_PER_CU"))
         ctx->sched.items_per_cu = std::max<int64_t>(1, std::atoll(e));
@@ -944,6 +979,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipStream_t &sd : ctx->side) HIP_TRY(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ctx->ev1), "hipEventCreate");
     for (Slot &sl : ctx->slot) HIP_TRY(sl.init(), "hipEventCreate");
@@ -1092,6 +1128,11 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     ctx->d_edge_done.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipStream_t sd : ctx->side)
+        if (sd) {
+            (void)hipStreamSynchronize(sd);
+            (void)hipStreamDestroy(sd);
+        }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1282,6 +1323,7 @@ void apply_tune(fpgm_ctx *ctx, const fpgm_level_tune &t)
     if (t.max_slices > 0) ctx->max_slices = t.max_slices;
     if (t.round_cap > 0) ctx->sched.cap = t.round_cap;
     if (t.pipeline >= 0) ctx->sched.pipeline = t.pipeline;
+    if (t.concurrent >= 0) ctx->sched.concurrent = t.concurrent;
 }
 
 // Level d >= 1 of every instance still running (SearchAtDepth, PCStable.cpp:265-388): the
@@ -1836,6 +1878,7 @@ fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm
         fpgm_level_tune t{};
         t.level = lvl;
         t.pipeline = ctx->sched.pipeline;
+        t.concurrent = ctx->sched.concurrent;
         t.items_per_cu = lvl == 0 ? ctx->sched.gram_items_per_cu : ctx->sched.items_per_cu;
         for (int g = 0; g < kGroups; ++g) t.wg_per_cu[g] = ctx->sched.wpc[g];
         t.max_slices = ctx->max_slices;
@@ -1929,6 +1972,12 @@ fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm
             cands.push_back(c);
         }
         if ((st = try_all()) != FPGM_OK) return st;
+        {
+            fpgm_level_tune c = cur;
+            c.concurrent = 1 - cur.concurrent;
+            cands.push_back(c);
+        }
+        if ((st = try_all()) != FPGM_OK) return st;
         for (int64_t ipc : {4, 8, 16, 32})
             if (ipc != cur.items_per_cu) {
                 fpgm_level_tune c = cur;
@@ -1978,7 +2027,7 @@ fpgm_status fpgm_tune_set(fpgm_ctx *ctx, const fpgm_level_tune *levels, int32_t 
     for (int32_t i = 0; i < n; ++i) {
         const fpgm_level_tune &r = levels[i];
         if (r.level < 0) continue;
-        if (r.level >= 64 || r.pipeline > 1 || r.items_per_cu < 0 || r.items_per_cu > 4096 || r.max_slices < 0 ||
+        if (r.level >= 64 || r.pipeline > 1 || r.concurrent > 1 || r.items_per_cu < 0 || r.items_per_cu > 4096 || r.max_slices < 0 ||
             r.round_cap < 0)
             return fail(ctx, FPGM_EINVAL, "bad tuning record");
         for (int g = 0; g < kGroups; ++g)

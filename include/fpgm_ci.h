@@ -209,6 +209,9 @@ fpgm_status fpgm_mle_counts(fpgm_ctx *ctx, int32_t n_nodes, const int32_t *par_o
 typedef struct {
     int32_t level;          /* 0: the MFMA Gram path; d >= 1: level d                       */
     int32_t pipeline;       /* 1: round r+1 built while round r runs (-1: default)            */
+    int32_t concurrent;     /* 1: slice and device-memory sinks overlap the others on streams
+                               of their own, 0: one stream (-1: default)                       */
+    int32_t pad;
     int64_t items_per_cu;   /* work items a launch aims at per CU (level 0: Gram items)       */
     int64_t wg_per_cu[5];   /* grid workgroups per CU: lane, lane-wide, shared, slice, device */
     int64_t max_slices;     /* z slices of a table before the device-memory sink              */

@@ -141,6 +141,26 @@ def test_skeleton_pipelined_rounds_same_results(monkeypatch, key, alpha, depth, 
     assert sum(c["rounds_per_level"]) > sum(b["rounds_per_level"])
 
 
+@pytest.mark.parametrize("key,alpha,depth,rows", [(2, 0.01, 5, 20_000), (3, 0.05, 4, 8_000)])
+def test_skeleton_concurrent_sink_launches_same_results(monkeypatch, key, alpha, depth, rows):
+    # FPGM_CONCURRENT=1 puts a round's slice and device-memory sink launches on streams of
+    # their own, joined before the p-value kernel, 0 keeps every launch on one stream; with
+    # small rounds too, both schedules must give the oracle's skeleton, sepsets and test
+    # counts (the depth-4/5 levels of these samples have slice and device-memory tests)
+    w = configs.get(key)
+    s = w.sample(rows)
+    o = O.pc_skeleton(O.Data.of(s), alpha, depth, n_threads=8)
+    out = []
+    for conc, slices in (("0", "8"), ("1", "8"), ("1", "1")):
+        monkeypatch.setenv("FPGM_CONCURRENT", conc)
+        monkeypatch.setenv("FPGM_MAX_SLICES", slices)  # 1: every sliced table to the device sink
+        with _ctx(s) as ctx:
+            out.append(ctx.pc_skeleton(alpha, depth))
+            out.append(ctx.pc_skeleton(alpha, depth, round_tests=256))
+    for g in out:
+        _assert_same(g, o)
+
+
 def test_skeleton_depth_and_alpha():
     w = configs.get(2)
     s = w.sample(20_000)
@@ -360,6 +380,7 @@ def test_autotune_changes_time_only(key, alpha, depth, rows):
         for r in t["levels"]:
             assert r["tuned_ms"] <= r["default_ms"] + 1e-9
             assert r["items_per_cu"] > 0 and min(r["wg_per_cu"]) >= 1 and r["pipeline"] in (0, 1)
+            assert r["concurrent"] in (0, 1)
         a = ctx.pc_skeleton(alpha, depth)
         ctx.tune_clear()
         b = ctx.pc_skeleton(alpha, depth)
