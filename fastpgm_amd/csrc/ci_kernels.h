@@ -19,10 +19,13 @@ constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
 constexpr int kCritTab = 4096;            // critical values tabulated for df = 1..kCritTab
 
 // One dataset instance: uint8 columns, column v at cols + v*stride, n_rows valid bytes.
+// The counting kernels read the packed copy instead: column v at pcols + v*stride, its
+// values at colw[v] bits each (8, 4 or 2; see k_pack_cols for the layout).
 struct DevInst {
     const uint8_t *cols;
     int64_t stride;
     int64_t n_rows;
+    const uint8_t *pcols;
 };
 
 // One test as the device sees it (48 bytes).
@@ -46,6 +49,7 @@ constexpr int32_t kTFExact = 4;      // FP64 statistic and p-value regardless of
 constexpr int32_t kTFKeep = 8;       // leave the counted table in the workspace (parity reads)
 constexpr int32_t kTFLane = 16;      // per-lane 16-bit LDS counters (cells <= kLaneCells)
 constexpr int32_t kTFWide = 32;      // statistic by the grid-wide finalize kernels, not the counting kernel
+constexpr int32_t kTFNarrow = 64;    // every column of the test packed at 4 or 2 bits per value
 
 // Grid-wide finalize of large device-memory tables: one workgroup per range of z slices
 // writes a partial, then one thread per test sums its partials in order.
@@ -81,6 +85,7 @@ struct LaunchArgs {
     const DevTest *tests;
     const int32_t *zpool;
     const int32_t *card;
+    const uint8_t *colw;     // bits per value of each packed column: 8, 4 or 2
     const uint64_t *items;   // work items: test << 32 | (slice * n_chunks + chunk)
     int64_t item_begin;      // this launch's items: [item_begin, item_begin + n_items)
     int64_t n_items;         // (an upper bound when n_items_dev is set)
@@ -141,6 +146,9 @@ hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int6
 // sequence on the stream.
 hipError_t launch_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests, int32_t *edge_done,
                          int32_t round, uint32_t *clear, int n_clear, hipStream_t s);
+// Packed copy of one instance's columns (16-row groups; colw[v] bits per value).
+hipError_t launch_pack_cols(const uint8_t *cols, uint8_t *pcols, int64_t stride, int n_vars, const uint8_t *colw,
+                            hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);
 

@@ -1,8 +1,9 @@
 // Fused count + G^2 kernels of the PC-stable CI stage, written for gfx950 (CDNA4).
 //
 // One work item = (test, row chunk).  A 256-thread workgroup streams the chunk's
-// uint8 columns with 16-byte loads (16 rows per thread per step, 4096 rows per
-// workgroup step), forms the contingency-table index of every row
+// packed columns (16 rows per thread per step, 4096 rows per workgroup step: one
+// 16-, 8- or 4-byte load per column as it holds 8, 4 or 2 bits per value), forms
+// the contingency-table index of every row
 //     key = ((z_1 * c(Z_2) + z_2) ... ) * c(X) + x) * c(Y) + y
 // (Counts3D / Counts2D layout, CellTable.cpp:254-291,430-441), and increments an
 // LDS histogram, one copy per wave when the table is small enough.  A test whose
@@ -35,6 +36,89 @@ __device__ __forceinline__ uint4 ld16(const uint8_t *p)
 {
     const u32x4 v = *(const __attribute__((address_space(1))) u32x4 *)(p);
     return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Packed columns (k_pack_cols): each group of 16 rows, rows 16g .. 16g+15, takes 16, 8
+// or 4 bytes at byte offset 16g * w / 8 of the column, w = bits per value:
+//   w = 8: byte b = row b;
+//   w = 4: 32-bit word h (h = 0, 1) holds rows 8h+j (j = 0..3) in the low nibble of
+//          byte j and rows 8h+4+j in its high nibble;
+//   w = 2: the one 32-bit word holds row 4q+j in bits 2q..2q+1 of byte j.
+// So that unpacking is one mask (and one shift) per 4 rows: word q of the unpacked
+// group holds rows 4q..4q+3, one per byte, as an 8-bit column would.
+// w is uniform over the wave (one test per workgroup), so the unpacking branches are
+// scalar.  The loads themselves never branch: a group is read with one load at byte
+// offset r >> sh (sh = 0, 1, 2 for w = 8, 4, 2) of the fixed size of the loop, 16
+// bytes (NW = 4, any width) or 8 bytes (NW = 2, widths 4 and 2), the bytes past the
+// group being the next group's (or the column's last bytes: never past the column).
+// A load under a branch would need a register copy at the join, and that copy would
+// wait for the load, which is still in flight to overlap the counting of the previous
+// step.
+template <int NW>
+struct RawOf {
+    typedef uint4 T;
+};
+template <>
+struct RawOf<2> {
+    typedef uint2 T;
+};
+
+typedef unsigned int u32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef unsigned int u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+template <int NW>
+__device__ __forceinline__ typename RawOf<NW>::T ldw(const uint8_t *col, uint32_t sh, int64_t r);
+
+template <>
+__device__ __forceinline__ uint4 ldw<4>(const uint8_t *col, uint32_t sh, int64_t r)
+{
+    const u32x4a4 v = *(const __attribute__((address_space(1))) u32x4a4 *)(col + (r >> sh));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+template <>
+__device__ __forceinline__ uint2 ldw<2>(const uint8_t *col, uint32_t sh, int64_t r)
+{
+    const u32x2a4 v = *(const __attribute__((address_space(1))) u32x2a4 *)(col + (r >> sh));
+    return make_uint2(v.x, v.y);
+}
+
+// load shift of a width
+__device__ __forceinline__ uint32_t wshift(uint32_t w)
+{
+    return w == 8 ? 0u : w == 4 ? 1u : 2u;
+}
+
+__device__ __forceinline__ uint4 unpack4(uint32_t x, uint32_t y)
+{
+    const uint32_t m = 0x0F0F0F0Fu;
+    return make_uint4(x & m, (x >> 4) & m, y & m, (y >> 4) & m);
+}
+
+__device__ __forceinline__ uint4 unpack2(uint32_t x)
+{
+    const uint32_t m = 0x03030303u;
+    return make_uint4(x & m, (x >> 2) & m, (x >> 4) & m, (x >> 6) & m);
+}
+
+__device__ __forceinline__ uint4 unpackw(const uint4 &p, uint32_t w)
+{
+    if (w == 8) return p;
+    if (w == 4) return unpack4(p.x, p.y);
+    return unpack2(p.x);
+}
+
+__device__ __forceinline__ uint4 unpackw(const uint2 &p, uint32_t w)
+{
+    if (w == 4) return unpack4(p.x, p.y);
+    return unpack2(p.x);
+}
+
+__device__ __forceinline__ uint32_t uniform(int32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ __forceinline__ uint32_t byte_of(const uint4 &v, int b)
@@ -90,6 +174,8 @@ __device__ __forceinline__ void wave_nsum(float &s, float &c)
 struct Shared {
     int32_t zcol[kMaxZLds];
     int32_t zcard[kMaxZLds];
+    int32_t zw[kMaxZLds];  // bits per value of the packed Z columns
+    int32_t wx, wy;        // and of X and Y
     int32_t red_i[kWG / 64];
     float red_f[3 * (kWG / 64)];
     double red_d[kWG / 64];
@@ -144,43 +230,68 @@ __device__ __forceinline__ void put(int32_t *h, uint32_t key, const Shared &sh)
     else sink<kSink>(h, key);
 }
 
-template <int D>
+template <int D, int NW = 4>
 struct Cols {
-    uint4 v[(D > 0 ? D : 1) + 2];
+    typename RawOf<NW>::T v[(D > 0 ? D : 1) + 2];
 };
 
-// Loads the 16-byte groups at row r of Z (compile-time count D), X and Y.
-template <int D>
-__device__ __forceinline__ void load_cols(const uint8_t *const *zp, const uint8_t *colx,
-                                          const uint8_t *coly, int64_t r, Cols<D> &c)
-{
+// The packed columns of a test: Z (compile-time count D) at 0..D-1, X at X, Y at X+1,
+// X = max(D, 1); slot 0 is unused when D = 0.
+template <int D, int NW>
+struct ColSet {
+    static constexpr int X = (D > 0 ? D : 1);
+    const uint8_t *p[X + 2];
+    uint32_t w[X + 2], s[X + 2];
+    __device__ __forceinline__ ColSet(const DevInst &I, const DevTest &T, const Shared &sh)
+    {
 #pragma unroll
-    for (int j = 0; j < D; ++j) c.v[j] = ld16(zp[j] + r);
-    c.v[(D > 0 ? D : 1)] = ld16(colx + r);
-    c.v[(D > 0 ? D : 1) + 1] = ld16(coly + r);
-}
+        for (int j = 0; j < D; ++j) {
+            p[j] = I.pcols + (int64_t)sh.zcol[j] * I.stride;
+            w[j] = uniform(sh.zw[j]);
+        }
+        p[X] = I.pcols + (int64_t)T.x * I.stride;
+        w[X] = uniform(sh.wx);
+        p[X + 1] = I.pcols + (int64_t)T.y * I.stride;
+        w[X + 1] = uniform(sh.wy);
+#pragma unroll
+        for (int j = 0; j < X + 2; ++j) s[j] = wshift(w[j]);
+    }
+    // the packed 16-row groups at row r
+    __device__ __forceinline__ void load(int64_t r, Cols<D, NW> &c) const
+    {
+#pragma unroll
+        for (int j = 0; j < D; ++j) c.v[j] = ldw<NW>(p[j], s[j], r);
+        c.v[X] = ldw<NW>(p[X], s[X], r);
+        c.v[X + 1] = ldw<NW>(p[X + 1], s[X + 1], r);
+    }
+    // ... as bytes, row b of the group in byte b
+    __device__ __forceinline__ void unpack(const Cols<D, NW> &c, Cols<D> &u) const
+    {
+#pragma unroll
+        for (int j = 0; j < D; ++j) u.v[j] = unpackw(c.v[j], w[j]);
+        u.v[X] = unpackw(c.v[X], w[X]);
+        u.v[X + 1] = unpackw(c.v[X + 1], w[X + 1]);
+    }
+};
 
 // Rows [r_begin, r_end) of test T into the sink, |Z| = D known at compile time.
 // Loads for the next step are issued before the current step is counted.
-template <int D, int kSink>
+template <int D, int kSink, int NW>
 __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T, const Shared &sh,
                                              int64_t r_begin, int64_t r_end, int32_t *h)
 {
-    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
-    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
-    const uint8_t *zp[D > 0 ? D : 1];
+    const ColSet<D, NW> cs(I, T, sh);
     uint32_t zc[D > 0 ? D : 1];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        zp[j] = I.cols + (int64_t)sh.zcol[j] * I.stride;
-        zc[j] = (uint32_t)sh.zcard[j];
-    }
+    for (int j = 0; j < D; ++j) zc[j] = (uint32_t)sh.zcard[j];
     const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
     constexpr int X = (D > 0 ? D : 1);
     int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
     // two register buffers used in turn: the loads of the next step are in flight
     // while the current one is counted, and no register copy waits for them
-    auto step = [&](const Cols<D> &cur, int64_t r) {
+    auto step = [&](const Cols<D, NW> &raw, int64_t r) {
+        Cols<D> cur;
+        cs.unpack(raw, cur);
         uint32_t key[kRowsPerThread];
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
@@ -202,16 +313,16 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
                 if (b < nv) put<kSink>(h, key[b], sh);
         }
     };
-    Cols<D> b0, b1;
-    if (r < r_end) load_cols<D>(zp, colx, coly, r, b0);
+    Cols<D, NW> b0, b1;
+    if (r < r_end) cs.load(r, b0);
     while (r < r_end) {
         // the loads are unconditional (clamped to a valid step): no branch around them
         const int64_t r1 = r + kRowsPerStep;
-        load_cols<D>(zp, colx, coly, (r1 < r_end) ? r1 : r, b1);
+        cs.load((r1 < r_end) ? r1 : r, b1);
         step(b0, r);
         if (r1 >= r_end) break;
         const int64_t r2 = r1 + kRowsPerStep;
-        load_cols<D>(zp, colx, coly, (r2 < r_end) ? r2 : r1, b0);
+        cs.load((r2 < r_end) ? r2 : r1, b0);
         step(b1, r1);
         r = r2;
     }
@@ -222,8 +333,9 @@ template <int kSink>
 __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &T, const Shared &sh,
                                                int64_t r_begin, int64_t r_end, int32_t *h)
 {
-    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
-    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
+    const uint8_t *colx = I.pcols + (int64_t)T.x * I.stride;
+    const uint8_t *coly = I.pcols + (int64_t)T.y * I.stride;
+    const uint32_t wx = uniform(sh.wx), wy = uniform(sh.wy);
     const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
     const int d = T.d;
     for (int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread; r < r_end; r += kRowsPerStep) {
@@ -231,14 +343,15 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) key[b] = 0;
         for (int j = 0; j < d; ++j) {
-            const uint4 v = ld16(I.cols + (int64_t)sh.zcol[j] * I.stride + r);
+            const uint32_t wz = uniform(sh.zw[j]);
+            const uint4 v = unpackw(ldw<4>(I.pcols + (int64_t)sh.zcol[j] * I.stride, wshift(wz), r), wz);
             const uint32_t c = (uint32_t)sh.zcard[j];
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
                 key[b] = (kSink == kSinkGlobal) ? key[b] * c + byte_of(v, b) : mad24(key[b], c, byte_of(v, b));
         }
-        const uint4 vx = ld16(colx + r);
-        const uint4 vy = ld16(coly + r);
+        const uint4 vx = unpackw(ldw<4>(colx, wshift(wx), r), wx);
+        const uint4 vy = unpackw(ldw<4>(coly, wshift(wy), r), wy);
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
             if (kSink == kSinkGlobal) key[b] = (key[b] * cx + byte_of(vx, b)) * cy + byte_of(vy, b);
@@ -265,12 +378,11 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
 //
 // Every key is below kCells <= 128, so the keys of 4 rows are formed at once in one
 // 32-bit word, one byte per row: the Horner steps run as packed 16-bit multiply-adds
-// (v_pk_mad_u16) on the raw column words, and no byte ever carries into the next
-// because every partial key stays below 256.  A row's LDS byte offset is then
-// (k << 8) | base, taken from the packed word with a shift and a mask (base holds
-// the pair and lane bits, which do not overlap).  Rows past the end of the data only
-// ever carry into later rows of the same word, which are past the end as well and
-// are not counted.
+// (v_pk_mad_u16) on the unpacked column words (one row per byte), and no byte ever
+// carries into the next because every partial key stays below 256.  A row's LDS byte
+// offset comes from that word with one byte permute (lane_put4).  Rows past the end
+// of the data only ever carry into later rows of the same word, which are past the
+// end as well and are not counted.
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pk_mad(uint32_t a, uint32_t b, uint32_t c)
@@ -285,17 +397,19 @@ __device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
 }
 
 // Counts the (up to) 4 rows whose keys are the bytes of K; rows >= nvalid are skipped.
-// A row's byte offset in its wave pair's block is key * 256 + lane * 4: one v_perm_b32
-// puts key byte j of K into byte 1 and lane * 4 (lane4, byte 0) into byte 0; the block of
-// the wave pair is the instruction's immediate offset (kPairOff).
-template <uint32_t kPairOff>
-__device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, int nvalid)
+// A row's byte offset in the counters is pair * kCells * 256 + key * 256 + lane * 4: the
+// wave pair's bit is ORed into every key byte (bias: 0, or kCells in each byte, a bit no
+// key uses), then one v_perm_b32 puts key byte j into byte 1 and lane * 4 (lane4, byte 0)
+// into byte 0.
+__device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, uint32_t bias,
+                                          int nvalid)
 {
+    K |= bias;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
         if (j < nvalid) {
             const uint32_t off = __builtin_amdgcn_perm(K, lane4, 0x0C0C0000u | ((4u + (uint32_t)j) << 8));
-            atomicAdd(reinterpret_cast<uint32_t *>(lb + kPairOff + off), inc);
+            atomicAdd(reinterpret_cast<uint32_t *>(lb + off), inc);
         }
 }
 
@@ -312,91 +426,77 @@ __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *
     return pk_mad(K, cy2, word_of(c.v[X + 1], q));
 }
 
-template <int D, uint32_t kPairOff>
+template <int D, int NW>
 __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTest &T, const Shared &sh,
                                                   int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
-                                                  uint32_t inc)
+                                                  uint32_t inc, uint32_t bias)
 {
-    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
-    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
-    const uint8_t *zp[D > 0 ? D : 1];
+    const ColSet<D, NW> cs(I, T, sh);
     uint32_t zc2[D > 0 ? D : 1];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        zp[j] = I.cols + (int64_t)sh.zcol[j] * I.stride;
-        zc2[j] = (uint32_t)sh.zcard[j] * 0x10001u;
-    }
+    for (int j = 0; j < D; ++j) zc2[j] = (uint32_t)sh.zcard[j] * 0x10001u;
     const uint32_t cx2 = (uint32_t)T.cx * 0x10001u, cy2 = (uint32_t)T.cy * 0x10001u;
     int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
-    auto step = [&](const Cols<D> &cur, int64_t r) {
+    auto step = [&](const Cols<D, NW> &raw, int64_t r) {
+        Cols<D> cur;
+        cs.unpack(raw, cur);
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lane_put4<kPairOff>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), 4);
+            for (int q = 0; q < 4; ++q) lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), bias, 4);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                lane_put4<kPairOff>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), (int)nv - 4 * q);
+                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), bias, (int)nv - 4 * q);
         }
     };
-    Cols<D> b0, b1;
-    if (r < r_end) load_cols<D>(zp, colx, coly, r, b0);
+    Cols<D, NW> b0, b1;
+    if (r < r_end) cs.load(r, b0);
     while (r < r_end) {
         const int64_t r1 = r + kRowsPerStep;
-        load_cols<D>(zp, colx, coly, (r1 < r_end) ? r1 : r, b1);
+        cs.load((r1 < r_end) ? r1 : r, b1);
         step(b0, r);
         if (r1 >= r_end) break;
         const int64_t r2 = r1 + kRowsPerStep;
-        load_cols<D>(zp, colx, coly, (r2 < r_end) ? r2 : r1, b0);
+        cs.load((r2 < r_end) ? r2 : r1, b0);
         step(b1, r1);
         r = r2;
     }
 }
 
-// Runtime |Z| (d > 3).
-template <uint32_t kPairOff>
+// Runtime |Z|: d > 3, or a column of the test kept at 8 bits.
 __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevTest &T, const Shared &sh,
                                                     int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
-                                                    uint32_t inc)
+                                                    uint32_t inc, uint32_t bias)
 {
-    const uint8_t *colx = I.cols + (int64_t)T.x * I.stride;
-    const uint8_t *coly = I.cols + (int64_t)T.y * I.stride;
+    const uint8_t *colx = I.pcols + (int64_t)T.x * I.stride;
+    const uint8_t *coly = I.pcols + (int64_t)T.y * I.stride;
+    const uint32_t wx = uniform(sh.wx), wy = uniform(sh.wy);
     const uint32_t cx2 = (uint32_t)T.cx * 0x10001u, cy2 = (uint32_t)T.cy * 0x10001u;
     const int d = T.d;
     for (int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread; r < r_end; r += kRowsPerStep) {
         uint32_t K[4] = {0u, 0u, 0u, 0u};
         for (int j = 0; j < d; ++j) {
-            const uint4 v = ld16(I.cols + (int64_t)sh.zcol[j] * I.stride + r);
+            const uint32_t wz = uniform(sh.zw[j]);
+            const uint4 v = unpackw(ldw<4>(I.pcols + (int64_t)sh.zcol[j] * I.stride, wshift(wz), r), wz);
             const uint32_t c2 = (uint32_t)sh.zcard[j] * 0x10001u;
 #pragma unroll
             for (int q = 0; q < 4; ++q) K[q] = pk_mad(K[q], c2, word_of(v, q));
         }
-        const uint4 vx = ld16(colx + r);
-        const uint4 vy = ld16(coly + r);
+        const uint4 vx = unpackw(ldw<4>(colx, wshift(wx), r), wx);
+        const uint4 vy = unpackw(ldw<4>(coly, wshift(wy), r), wy);
         const int64_t nv = r_end - r;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q));
-            lane_put4<kPairOff>(lb, lane4, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
+            lane_put4(lb, lane4, inc, k, bias, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
         }
     }
 }
 
 // blk: the workgroup's counters, [pair][cell][lane] 32-bit words, kCells cells a pair.
-template <uint32_t kPairOff>
-__device__ __forceinline__ void count_rows_lane_pair(const DevInst &I, const DevTest &T, const Shared &sh,
-                                                     int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
-                                                     uint32_t inc)
-{
-    switch (T.d) {
-    case 0: count_rows_lane_d<0, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
-    case 1: count_rows_lane_d<1, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
-    case 2: count_rows_lane_d<2, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
-    case 3: count_rows_lane_d<3, kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
-    default: count_rows_lane_any<kPairOff>(I, T, sh, r_begin, r_end, lb, lane4, inc); break;
-    }
-}
-
+// |Z| <= 3 takes loops with |Z| known at compile time: with 8-byte loads when every
+// column of the test is packed below 8 bits, 16-byte loads otherwise.
 template <int kCells>
 __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest &T, const Shared &sh,
                                                 int64_t r_begin, int64_t r_end, uint32_t *blk)
@@ -404,22 +504,48 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
     char *lb = reinterpret_cast<char *>(blk);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t inc = (wave & 1) ? 0x10000u : 1u;
-    // the wave pair's block is a compile-time offset: one branch per wave, uniform
-    if (wave >> 1) count_rows_lane_pair<(uint32_t)kCells * 256u>(I, T, sh, r_begin, r_end, lb, lane * 4u, inc);
-    else count_rows_lane_pair<0u>(I, T, sh, r_begin, r_end, lb, lane * 4u, inc);
+    const uint32_t bias = (wave >> 1) ? (uint32_t)kCells * 0x01010101u : 0u;
+    const uint32_t lane4 = lane * 4u;
+    if (uniform(T.flags) & kTFNarrow) {
+        switch (T.d) {
+        case 0: count_rows_lane_d<0, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 1: count_rows_lane_d<1, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 2: count_rows_lane_d<2, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 3: count_rows_lane_d<3, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        default: break;
+        }
+    }
+    switch (T.d) {
+    case 0: count_rows_lane_d<0, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+    case 1: count_rows_lane_d<1, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+    case 2: count_rows_lane_d<2, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+    case 3: count_rows_lane_d<3, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+    default: break;
+    }
+    count_rows_lane_any(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
 }
 
 template <int kSink>
 __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, const Shared &sh,
                                            int64_t r_begin, int64_t r_end, int32_t *h)
 {
-    switch (T.d) {
-    case 0: count_rows_d<0, kSink>(I, T, sh, r_begin, r_end, h); break;
-    case 1: count_rows_d<1, kSink>(I, T, sh, r_begin, r_end, h); break;
-    case 2: count_rows_d<2, kSink>(I, T, sh, r_begin, r_end, h); break;
-    case 3: count_rows_d<3, kSink>(I, T, sh, r_begin, r_end, h); break;
-    default: count_rows_any<kSink>(I, T, sh, r_begin, r_end, h); break;
+    if (uniform(T.flags) & kTFNarrow) {
+        switch (T.d) {
+        case 0: count_rows_d<0, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
+        case 1: count_rows_d<1, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
+        case 2: count_rows_d<2, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
+        case 3: count_rows_d<3, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
+        default: break;
+        }
     }
+    switch (T.d) {
+    case 0: count_rows_d<0, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+    case 1: count_rows_d<1, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+    case 2: count_rows_d<2, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+    case 3: count_rows_d<3, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+    default: break;
+    }
+    count_rows_any<kSink>(I, T, sh, r_begin, r_end, h);
 }
 
 // Critical value of the chi-square test at the call's alpha: tabulated (exact FP64
@@ -680,8 +806,12 @@ struct LdsWords {
         (kMode == kSinkGlobal) ? 1 : (kMode == kSinkLaneW || kMode == kSinkSlice) ? 2 * kLdsCells : kLdsCells;
 };
 
+// Waves per SIMD the register allocation must allow: the LDS of the per-lane and shared
+// sinks leaves room for 4 workgroups per CU (one wave each per SIMD), so those sinks are
+// held to 128 VGPRs; the 64 KiB sinks fit 2 workgroups per CU whatever their registers.
 template <int kMode>
-__global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
+    (kMode == kSinkLane || kMode == kSinkShared) ? 4 : 1))) void k_count_stat(LaunchArgs a)
 {
     __shared__ int32_t lds[LdsWords<kMode>::value];
     __shared__ int32_t small[kSmallCells];
@@ -737,6 +867,11 @@ __global__ __launch_bounds__(kWG) void k_count_stat(LaunchArgs a)
             const int32_t zc = a.zpool[T.z_off + j];
             sh.zcol[j] = zc;
             sh.zcard[j] = a.card[zc];
+            sh.zw[j] = a.colw[zc];
+        }
+        if (tid == 0) {
+            sh.wx = a.colw[T.x];
+            sh.wy = a.colw[T.y];
         }
         __syncthreads();
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
@@ -1005,6 +1140,34 @@ __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64
     if (flag) atomicOr(bad, 1u);
 }
 
+// Packed copy of the columns (layout at ldw): one thread per 16-row group, every group
+// of the stride (rows past n_rows pack whatever the padding holds; the counting kernels
+// never count them).  Each value is masked to w bits, so that no value can spill into
+// a neighbour's bits.
+__global__ __launch_bounds__(kWG) void k_pack_cols(const uint8_t *cols, uint8_t *pcols, int64_t stride, int n_vars,
+                                                   const uint8_t *colw)
+{
+    const int v = blockIdx.y;
+    if (v >= n_vars) return;
+    const uint32_t w = colw[v];
+    const uint8_t *col = cols + (int64_t)v * stride;
+    uint8_t *out = pcols + (int64_t)v * stride;
+    for (int64_t r = ((int64_t)blockIdx.x * kWG + threadIdx.x) * 16; r < stride; r += (int64_t)gridDim.x * kWG * 16) {
+        const uint4 q = ld16(col + r);
+        if (w == 8) {
+            *reinterpret_cast<uint4 *>(out + r) = q;
+        } else if (w == 4) {
+            const uint32_t m = 0x0F0F0F0Fu;
+            *reinterpret_cast<uint2 *>(out + (r >> 1)) =
+                make_uint2((q.x & m) | ((q.y & m) << 4), (q.z & m) | ((q.w & m) << 4));
+        } else {
+            const uint32_t m = 0x03030303u;
+            *reinterpret_cast<uint32_t *>(out + (r >> 2)) =
+                (q.x & m) | ((q.y & m) << 2) | ((q.z & m) << 4) | ((q.w & m) << 6);
+        }
+    }
+}
+
 // FP64 p-value of the tests the statistic kernel rechecked (one thread per test):
 // p = 1 - pchisq(G^2, df), independent iff p > alpha (IndependenceTest.cpp:146-154).
 __global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, double alpha, const DevTest *tests,
@@ -1077,6 +1240,17 @@ hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_ro
     const int gx = (int)(steps < 64 ? steps : 64);
     hipLaunchKernelGGL(k_check_values, dim3(gx, n_vars), dim3(kWG), 0, s, cols, stride, n_rows, n_vars, card,
                        bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_cols(const uint8_t *cols, uint8_t *pcols, int64_t stride, int n_vars, const uint8_t *colw,
+                            hipStream_t s)
+{
+    if (n_vars <= 0 || stride <= 0) return hipSuccess;
+    if (stride % 16 != 0) return hipErrorInvalidValue;
+    const int64_t steps = (stride + (int64_t)kWG * 16 - 1) / ((int64_t)kWG * 16);
+    const int gx = (int)(steps < 64 ? steps : 64);
+    hipLaunchKernelGGL(k_pack_cols, dim3(gx, n_vars), dim3(kWG), 0, s, cols, pcols, stride, n_vars, colw);
     return hipGetLastError();
 }
 

@@ -220,6 +220,12 @@ struct fpgm_ctx {
     int32_t n_vars = 0;
     std::vector<int32_t> card;
     int32_t *d_card = nullptr;
+    // packed columns read by the counting kernels: bits per value by cardinality (2 up to
+    // 4 states, 4 up to 16, else 8); FPGM_PACK=0 keeps 8 bits for every column
+    bool pack = true;
+    std::vector<uint8_t> colw;
+    uint8_t *d_colw = nullptr;
+    uint8_t *d_pcols = nullptr;  // owned; nullptr when every column has 8 bits (the uint8 columns are read)
     std::string err;
     std::mutex mu;
     int n_cu = 256;
@@ -376,8 +382,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.cx = ctx->card[h.x];
         T.cy = ctx->card[h.y];
         int64_t dimz = 1;
+        bool narrow = ctx->colw[h.x] != 8 && ctx->colw[h.y] != 8;
         for (int j = 0; j < h.d; ++j) {
             dimz *= ctx->card[zpool[h.z_off + j]];
+            narrow = narrow && ctx->colw[zpool[h.z_off + j]] != 8;
             if (dimz > kMaxCells) break;
         }
         const int64_t cells = dimz * T.cx * T.cy;
@@ -390,7 +398,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.n_slices = 1;
         T.slice_cells = (int32_t)cells;
         const int64_t marg = dimz * (T.cx + T.cy + 1);
-        T.flags = tflags;
+        T.flags = tflags | (narrow ? kTFNarrow : 0);
         bool fits;
         int gi;
         if (cells <= kLaneCells) {
@@ -524,6 +532,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     a.tests = S.d_tests.p;
     a.zpool = S.d_zpool.p;
     a.card = ctx->d_card;
+    a.colw = ctx->d_colw;
     a.items = S.d_items.p;
     a.rows_per_chunk = rpc;
     a.ws = ctx->d_ws.p;
@@ -976,6 +985,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         if (std::sscanf(e, "%ld,%ld,%ld,%ld,%ld", &v[0], &v[1], &v[2], &v[3], &v[4]) == kGroups)
             for (int g = 0; g < kGroups; ++g) ctx->sched.wpc[g] = std::min<int64_t>(64, std::max<int64_t>(1, v[g]));
     }
+    if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0;
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1004,11 +1014,43 @@ static bool rows_ok(int64_t n_rows, int64_t row_stride)
     return n_rows >= 0 && row_stride >= n_rows && n_rows <= ((int64_t)1 << 31) - 1;
 }
 
-// Uploads the instance table and scans every column for values >= card.
+// Packs the columns for the counting kernels, uploads the instance table and scans
+// every column for values >= card.
 static fpgm_status finish_ctx(fpgm_ctx *ctx)
 {
+    const int n = ctx->n_vars;
+    ctx->colw.assign((size_t)n, 8);
+    bool any_packed = false;
+    if (ctx->pack)
+        for (int v = 0; v < n; ++v) {
+            const int c = ctx->card[v];
+            ctx->colw[v] = (uint8_t)(c <= 4 ? 2 : c <= 16 ? 4 : 8);
+            any_packed |= ctx->colw[v] != 8;
+        }
+    HIP_TRY(hipMalloc((void **)&ctx->d_colw, (size_t)std::max(1, n)), "hipMalloc widths");
+    HIP_TRY(hipMemcpyAsync(ctx->d_colw, ctx->colw.data(), (size_t)n, hipMemcpyHostToDevice, ctx->stream),
+            "upload widths");
+    std::vector<const uint8_t *> pcols(ctx->inst.size());
+    for (size_t i = 0; i < pcols.size(); ++i) pcols[i] = ctx->inst[i].cols;
+    if (any_packed) {
+        // one allocation, each instance at the offset of its uint8 columns (same stride)
+        int64_t total = 0;
+        std::vector<int64_t> off(ctx->inst.size());
+        for (size_t i = 0; i < off.size(); ++i) {
+            off[i] = total;
+            total += ctx->inst[i].stride * n;
+        }
+        HIP_TRY(hipMalloc((void **)&ctx->d_pcols, (size_t)std::max<int64_t>(total, 16)), "hipMalloc packed columns");
+        for (size_t i = 0; i < off.size(); ++i) {
+            const InstHost &I = ctx->inst[i];
+            HIP_TRY(launch_pack_cols(I.cols, ctx->d_pcols + off[i], I.stride, n, ctx->d_colw, ctx->stream),
+                    "launch pack");
+            pcols[i] = ctx->d_pcols + off[i];
+        }
+    }
     std::vector<DevInst> di(ctx->inst.size());
-    for (size_t i = 0; i < di.size(); ++i) di[i] = DevInst{ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows};
+    for (size_t i = 0; i < di.size(); ++i)
+        di[i] = DevInst{ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows, pcols[i]};
     HIP_TRY(hipMalloc((void **)&ctx->d_inst, sizeof(DevInst) * di.size()), "hipMalloc instances");
     HIP_TRY(hipMemcpyAsync(ctx->d_inst, di.data(), sizeof(DevInst) * di.size(), hipMemcpyHostToDevice, ctx->stream),
             "upload instances");
@@ -1107,6 +1149,8 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     if (ctx->owns_cols && ctx->d_cols) (void)hipFree((void *)ctx->d_cols);
     if (ctx->d_inst) (void)hipFree(ctx->d_inst);
     if (ctx->d_card) (void)hipFree(ctx->d_card);
+    if (ctx->d_colw) (void)hipFree(ctx->d_colw);
+    if (ctx->d_pcols) (void)hipFree(ctx->d_pcols);
     if (ctx->d_crit) (void)hipFree(ctx->d_crit);
     for (Slot &sl : ctx->slot) sl.release();
     ctx->d_ws.release();

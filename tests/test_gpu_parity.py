@@ -34,6 +34,62 @@ def test_counts_bit_exact(n_rows):
             assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z)), (x, y, z)
 
 
+@pytest.mark.parametrize("n_rows", [15, 4099, 70001])
+def test_counts_bit_exact_packed_widths(monkeypatch, n_rows):
+    # cardinalities 1..40: columns packed at 2 bits (<= 4 states), 4 bits (<= 16) and kept
+    # at 8 bits, mixed within one test; counts equal the oracle's and the unpacked form's
+    s = datagen.random_sample(300 + n_rows, 14, n_rows, 1, 40)
+    s.card[:4] = [2, 4, 16, 17]  # the width boundaries
+    for v in range(4):
+        s.cols[v, :n_rows] = np.random.default_rng(v).integers(0, s.card[v], n_rows)
+    D = O.Data.of(s)
+    rng = np.random.default_rng(n_rows + 1)
+    tests = [t for t in _rand_tests(rng, s.n_vars, 60, 3)
+             if np.prod([s.card[v] for v in (t[0], t[1]) + t[2]]) <= 1 << 22]
+    with _ctx(s) as ctx:
+        got = [ctx.counts(x, y, z) for (x, y, z) in tests]
+    monkeypatch.setenv("FPGM_PACK", "0")
+    with _ctx(s) as ctx:
+        unpacked = [ctx.counts(x, y, z) for (x, y, z) in tests]
+    for (x, y, z), g, u in zip(tests, got, unpacked):
+        o = O.counts(D, x, y, z)
+        assert np.array_equal(g, o), (x, y, z)
+        assert np.array_equal(u, o), (x, y, z)
+
+
+def test_device_columns_with_dirty_padding():
+    # columns handed over in device memory: the padding bytes past n_rows hold 0xFF, which
+    # the packed copy must neither count nor let spill into valid rows
+    # (device memory from the HIP runtime the library uses; torch carries its own runtime)
+    import ctypes as C
+    _lib.load()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    s = datagen.random_sample(41, 9, 5003, 2, 12)
+    cols = np.ascontiguousarray(s.cols.copy())
+    cols[:, s.n_rows:] = 0xFF
+    D = O.Data.of(s)
+    rng = np.random.default_rng(3)
+    o = O.pc_skeleton(D, 0.05, 1000, n_threads=8)
+    dptr = C.c_void_p()
+    assert hip.hipMalloc(C.byref(dptr), cols.nbytes) == 0
+    try:
+        assert hip.hipMemcpy(dptr, cols.ctypes.data, cols.nbytes, 1) == 0  # host to device
+        ctx = _lib.Context(n_rows=s.n_rows, card=s.card, device_ptr=dptr.value, n_vars=cols.shape[0],
+                           stride=cols.shape[1])
+        try:
+            for (x, y, z) in _rand_tests(rng, s.n_vars, 30, 3):
+                assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z)), (x, y, z)
+            g = ctx.pc_skeleton(0.05, 1000)
+        finally:
+            ctx.close()
+    finally:
+        hip.hipFree(dptr)
+    _assert_same(g, o)
+
+
 def test_counts_large_tables_global_path():
     # card 8: d = 3 gives 8^3 * 64 = 32768 cells (beyond the LDS table)
     s = datagen.random_sample(5, 8, 200_003, 8, 8)
