@@ -140,6 +140,7 @@ struct Slot {
     HostBuf<WideTest> h_wide_tests;
     std::vector<int32_t> mode_of;
     std::vector<int64_t> need;  // workspace int32 entries of each test of the batch being built
+    std::vector<uint8_t> fit;   // table and marginals fit the workgroup's LDS scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
     hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing: start of each group
     hipEvent_t ev_gend[kGroups] = {};     // and its end (groups may run concurrently)
@@ -212,7 +213,7 @@ struct InstHost {
 struct fpgm_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side[2] = {};  // streams of a round's slice and device-memory sink launches
+    hipStream_t side[5] = {};  // streams of a round's slice, device-memory and small LDS-sink launches
     const uint8_t *d_cols = nullptr;  // all instances' columns (owned unless borrowed)
     bool owns_cols = false;
     std::vector<InstHost> inst;       // dataset instances (1, or the bootstrap ensemble)
@@ -343,17 +344,13 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     for (int64_t i = i0; i < n; ++i) N = std::max(N, ctx->inst[tests[i].inst].n_rows);
     const int64_t steps = std::max<int64_t>(1, (N + kRowsPerStep - 1) / kRowsPerStep);
 
-    // rows per chunk, common to the launch: enough work items to fill every CU
+    // rows per chunk, one value per sink group (each group is a launch of its own): the
+    // LDS sinks share one, enough work items over all the launch's tests to fill every CU
+    // (cutting a small group of them into more chunks measured slower: every chunk
+    // clears, reduces and merges its LDS table); the z-slice and device-memory sinks,
+    // which run beside the others on streams of their own, get enough items each
     const int64_t target_items = (int64_t)ctx->n_cu * ctx->sched.items_per_cu;
-    int64_t rpc;
-    if (rows_per_chunk_forced > 0) {
-        rpc = (rows_per_chunk_forced + kRowsPerStep - 1) / kRowsPerStep * kRowsPerStep;
-    } else {
-        int64_t cpt = std::max<int64_t>(1, (target_items + (n - i0) - 1) / (n - i0));
-        cpt = std::min(cpt, steps);
-        rpc = (steps + cpt - 1) / cpt * kRowsPerStep;
-    }
-    rpc = std::min<int64_t>(rpc, kLaneMaxRowsPerChunk);
+    int64_t rpc[kGroups];
 
     // descriptors go straight into the slot's pinned buffer.  Per test, in parallel: its
     // descriptor, sink group (gidx), work items and workspace need; then, in order, the
@@ -364,11 +361,15 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     DevTest *dt = S.h_tests.p;
     std::vector<int32_t> &gidx = S.mode_of;
     std::vector<int64_t> &need = S.need;
+    std::vector<uint8_t> &fit = S.fit;
     gidx.resize((size_t)m);
     need.resize((size_t)m);
+    fit.resize((size_t)m);
     int too_big = 0;
+    int64_t units[kGroups] = {0, 0, 0, 0, 0};  // per group: the tests' z slices (work items per chunk)
     const int nth = (m >= kParallelTests) ? ctx->host_threads : 1;
-#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static) reduction(| : too_big)
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static) reduction(| : too_big) \
+    reduction(+ : units[:kGroups])
     for (int64_t k = 0; k < m; ++k) {
         const HostTest &h = tests[i0 + k];
         DevTest T{};
@@ -377,7 +378,6 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.d = h.d;
         T.inst = h.inst;
         T.edge = h.edge;
-        const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[h.inst].n_rows + rpc - 1) / rpc);
         T.z_off = (int32_t)h.z_off;
         T.cx = ctx->card[h.x];
         T.cy = ctx->card[h.y];
@@ -394,7 +394,6 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             continue;
         }
         T.dimz = (int32_t)dimz;
-        T.n_chunks = (int32_t)n_chunks;
         T.n_slices = 1;
         T.slice_cells = (int32_t)cells;
         const int64_t marg = dimz * (T.cx + T.cy + 1);
@@ -430,14 +429,39 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                 }
             }
         }
-        const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fits;
-        if (!need_ws) T.flags |= kTFLdsMarg;
         T.ws_off = -1;
-        need[k] = need_ws ? (cells + marg + 63) / 64 * 64 : 0;
+        fit[k] = fits ? 1 : 0;
         gidx[k] = gi;
+        units[gi] += T.n_slices;
         dt[k] = T;
     }
     if (too_big) return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
+    for (int g = 0; g < kGroups; ++g) {
+        int64_t r;
+        if (rows_per_chunk_forced > 0) {
+            r = (rows_per_chunk_forced + kRowsPerStep - 1) / kRowsPerStep * kRowsPerStep;
+        } else {
+            const int64_t u = (kGroupSink[g] == kSinkSlice || kGroupSink[g] == kSinkGlobal)
+                                  ? std::max<int64_t>(units[g], 1)
+                                  : m;
+            const int64_t cpt = std::min(steps, std::max<int64_t>(1, (target_items + u - 1) / u));
+            r = (steps + cpt - 1) / cpt * kRowsPerStep;
+        }
+        rpc[g] = std::min<int64_t>(r, kLaneMaxRowsPerChunk);
+    }
+    // chunks and workspace of every test, by its group's chunk size
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
+    for (int64_t k = 0; k < m; ++k) {
+        DevTest &T = dt[k];
+        const int64_t r = rpc[gidx[k]];
+        const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[T.inst].n_rows + r - 1) / r);
+        T.n_chunks = (int32_t)n_chunks;
+        const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
+        const int64_t marg = (int64_t)T.dimz * (T.cx + T.cy + 1);
+        const bool need_ws = (tflags & kTFKeep) || n_chunks > 1 || !fit[k];
+        if (!need_ws) T.flags |= kTFLdsMarg;
+        need[k] = need_ws ? (cells + marg + 63) / 64 * 64 : 0;
+    }
     int64_t ws_used = 0, n_items_total = 0, nb = 0;
     int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
     S.bytes = 0.0;
@@ -534,7 +558,6 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     a.card = ctx->d_card;
     a.colw = ctx->d_colw;
     a.items = S.d_items.p;
-    a.rows_per_chunk = rpc;
     a.ws = ctx->d_ws.p;
     a.done = ctx->d_done.p;
     a.crit = ctx->d_crit;
@@ -571,8 +594,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     // another on the context stream: run together they compete for the same CUs' LDS and
     // measured slower.  The slice and device-memory sinks, bound by LDS capacity and device
     // atomics rather than by the CUs they occupy, get a stream each, so that they overlap
-    // the rest; the context stream joins them before the p-value kernel.  Tests of
-    // different groups touch disjoint workspace, counters and results.
+    // the rest, and so does an LDS group of fewer work items than CUs (it cannot fill the
+    // chip, and its few long items would hold the context stream); the context stream
+    // joins them before the p-value kernel.  Tests of different groups touch disjoint
+    // workspace, counters and results.
     int n_grp = 0;
     for (int gi = 0; gi < kGroups; ++gi) n_grp += (grp_len[gi] > 0);
     const bool concurrent = ctx->sched.concurrent && n_grp > 1;
@@ -581,7 +606,8 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     for (int gi = 0; gi < kGroups; ++gi) {
         if (grp_len[gi] == 0) continue;
         hipStream_t gs = ctx->stream;
-        const int side = kGroupSink[gi] == kSinkSlice ? 0 : kGroupSink[gi] == kSinkGlobal ? 1 : -1;
+        int side = kGroupSink[gi] == kSinkSlice ? 0 : kGroupSink[gi] == kSinkGlobal ? 1 : -1;
+        if (side < 0 && grp_len[gi] < ctx->n_cu) side = 2 + gi;  // lane, lane-wide, shared: 2, 3, 4
         if (concurrent && side >= 0) {
             gs = ctx->side[side];
             HIP_TRY(hipStreamWaitEvent(gs, S.ev_fork, 0), "stream wait");
@@ -590,6 +616,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[gi], gs), "event");
         a.item_begin = grp_begin[gi];
         a.n_items = grp_len[gi];
+        a.rows_per_chunk = rpc[gi];
         a.queue = S.d_queue.p + gi * kXcds;
         a.n_items_dev = a.edge_done ? live_count + gi : nullptr;
         const int sink = kGroupSink[gi];
