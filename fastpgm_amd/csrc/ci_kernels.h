@@ -120,6 +120,9 @@ constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
 constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 64 KiB (<= 128 cells)
 constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
+// Row loop of the shared sink's column layout (tables <= kColCells): not a launch of its own.
+constexpr int kSinkColumns = 5;
+constexpr int kColCells = kLdsCells / 32;  // [cell][32] int32 words in the shared sink's LDS (256)
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 
 // Compaction of a round's work items before its counting launches: items of tests whose

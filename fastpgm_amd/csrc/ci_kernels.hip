@@ -227,6 +227,7 @@ template <int kSink>
 __device__ __forceinline__ void put(int32_t *h, uint32_t key, const Shared &sh)
 {
     if (kSink == kSinkSlice) sink_slice(h, key, (uint32_t)sh.slice_lo, (uint32_t)sh.slice_len);
+    else if (kSink == kSinkColumns) atomicAdd(&h[key << 5], 1);  // h: the lane's column
     else sink<kSink>(h, key);
 }
 
@@ -525,6 +526,19 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
     count_rows_lane_any(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
 }
 
+// |Z| <= 3, every column packed below 8 bits (the caller checks)
+template <int kSink>
+__device__ __forceinline__ void count_rows_narrow(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                  int64_t r_begin, int64_t r_end, int32_t *h)
+{
+    switch (T.d) {
+    case 0: count_rows_d<0, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
+    case 1: count_rows_d<1, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
+    case 2: count_rows_d<2, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
+    default: count_rows_d<3, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
+    }
+}
+
 template <int kSink>
 __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, const Shared &sh,
                                            int64_t r_begin, int64_t r_end, int32_t *h)
@@ -538,12 +552,16 @@ __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, c
         default: break;
         }
     }
-    switch (T.d) {
-    case 0: count_rows_d<0, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-    case 1: count_rows_d<1, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-    case 2: count_rows_d<2, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-    case 3: count_rows_d<3, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-    default: break;
+    // the shared sink holds two row loops (copies and columns) and must stay within 128
+    // VGPRs: its tests with an 8-bit column take the runtime-|Z| loop
+    if (kSink != kSinkShared) {
+        switch (T.d) {
+        case 0: count_rows_d<0, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+        case 1: count_rows_d<1, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+        case 2: count_rows_d<2, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+        case 3: count_rows_d<3, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
+        default: break;
+        }
     }
     count_rows_any<kSink>(I, T, sh, r_begin, r_end, h);
 }
@@ -902,6 +920,24 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             }
             __syncthreads();
             tab = small;
+        } else if (kMode == kSinkShared && cells <= kColCells && (T.flags & kTFNarrow) && T.d <= 3) {
+            // Column layout: one table of [cell][32] words for the workgroup, lane l
+            // counting in column l & 31, so the 32 lanes of a half-wave always hit 32
+            // distinct banks, however skewed the data (waves meet only through the
+            // atomics, never within one instruction).  Then thread t sums cell t, reading
+            // its columns from (t + j) & 31 on: again one bank per lane.
+            const int c32 = (int)cells;
+            for (int i = tid; i < c32 * 32; i += kWG) lds[i] = 0;
+            __syncthreads();
+            count_rows_narrow<kSinkColumns>(I, T, sh, r_begin, r_end, lds + (lane & 31));
+            __syncthreads();
+            int32_t sum = 0;
+            if (tid < c32)
+                for (int j = 0; j < 32; ++j) sum += lds[tid * 32 + ((tid + j) & 31)];
+            __syncthreads();
+            if (tid < c32) lds[tid] = sum;
+            __syncthreads();
+            tab = lds;
         } else if (kMode == kSinkShared) {
             // As many histogram copies as fit: `sub` copies per wave, lane l counting into
             // copy l % sub of its wave, so skewed data collides on one address in at most
