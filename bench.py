@@ -220,11 +220,13 @@ def summarize_pmc(pmc_dir):
         f"bytes {alg:.4g}",
         f"  per dispatch {t['bytes_per_launch']:.4g} bytes",
         f"  kernel time {kms:.1f} ms per stage -> measured HBM rate {t['hbm_gbs_measured']:.0f} GB/s of 8000.",
-        "Reading: the traffic is below the algorithmic bytes because tests launched together share",
-        "columns (every level-1 test of an edge scans x and y; concurrent tests share z columns) and the",
-        "second read of a column comes from L2 / MALL.  HBM does not bound these kernels: the per-row",
-        "work (key arithmetic and one LDS read-modify-write per row and test) does -- the SQ counter",
-        "profile has the instruction mix.",
+        "Reading: the traffic is below the algorithmic bytes because the kernels read the packed",
+        f"columns ({run['roofline'].get('packed', {}).get('bytes_per_step', 0):.4g} bytes per stage at 2, 4 "
+        "or 8 bits per value by cardinality) and tests",
+        "launched together share columns (every level-1 test of an edge scans x and y; concurrent tests",
+        "share z columns): the second read of a column comes from L2 / MALL.  HBM does not bound these",
+        "kernels: the per-row work (unpacking, key arithmetic and one LDS read-modify-write per row and",
+        "test) does -- the SQ counter profile has the instruction mix.",
     ]
     with open(os.path.join(ROOT, "profiles", f"k_count_stat_{name}_hbm_traffic.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
@@ -259,19 +261,25 @@ def summarize_trace(trace_dir, out_path):
         f"  then: python bench.py --summarize-trace gpurun_out/prof_kt   (run.json = that run's JSON line)",
         f"The run executes the CI stage {stages} times (timed step + read-back run); per-stage figures are",
         "the trace totals divided by that.",
+        "  (every counting kernel reads the packed columns: 2, 4 or 8 bits per value by cardinality)",
         "  k_count_stat<2>  per-lane 16-bit LDS sink, tables <= 64 cells",
         "  k_count_stat<3>  per-lane 16-bit LDS sink, tables <= 128 cells (64 KiB LDS)",
-        "  k_count_stat<0>  shared LDS histogram, tables <= 8192 cells",
+        "  k_count_stat<0>  shared LDS histogram, tables <= 8192 cells ([cell][32] columns up to 256)",
         "  k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
         "  k_count_stat<1>  device-memory sink (larger tables)",
         "  k_fin_wide_*     grid-wide finalize of device-memory tables",
         "  k_compact        per-round removal of tests whose edge an earlier round decided",
         "  k_gram_l0        level 0 as the int8 one-hot Gram matrix (v_mfma_i32_32x32x32_i8)",
         "  k_pairs_l0, k_pack_l0  level-0 G^2 / decision and result packing",
-        f"Counting launches (k_count_stat + k_fin_wide + k_compact): {cnt_ms:.2f} ms per stage, "
+        f"Counting launches (k_count_stat + k_fin_wide + k_compact): {cnt_ms:.2f} ms per stage summed over "
         f"{cnt_calls:.0f} dispatches per stage;",
-        f"bench.py's HIP-event figure for the same run: {rl['kernel_ms_per_step']:.2f} ms per stage.",
+        f"bench.py's HIP-event figure for the same run: {rl['kernel_ms_per_step']:.2f} ms per stage (elapsed",
+        "  from each round's first to its last counting launch: the slice and device-memory launches,",
+        "  and LDS groups smaller than the CU count, run on streams of their own beside the others, so",
+        "  the trace's sum of durations exceeds the elapsed time).",
         f"Algorithmic bytes per stage (sum over executed tests of (|Z|+2) * N): {rl['alg_bytes_per_step']:.4g}",
+        f"  (in the packed layout the kernels read: {rl.get('packed', {}).get('bytes_per_step', 0):.4g} bytes, "
+        f"{rl.get('packed', {}).get('bytes_per_step', 0) / (cnt_ms * 1e-3) / 1e9:.0f} GB/s by the trace)",
         f"  -> achieved {rl['alg_bytes_per_step'] / (cnt_ms * 1e-3) / 1e9:.0f} GB/s by the trace, "
         f"{rl['achieved']:.0f} GB/s by the events (peak {rl['peak']:.0f}); the effective rate exceeds the",
         "     HBM peak because tests launched together share columns through L2 / MALL (measured HBM",
@@ -416,7 +424,7 @@ def main():
 
     # timed region: the CI stage only (fpgm_pc_skeleton), nothing read back but the
     # adjacency and the stats struct
-    total_tests, kernel_ms, alg_bytes, launches = 0, 0.0, 0.0, 0
+    total_tests, kernel_ms, alg_bytes, packed_bytes, launches = 0, 0.0, 0.0, 0.0, 0
     adjs = [np.zeros((sample.n_vars, sample.n_vars), dtype=np.uint8) for _ in range(args.steps)]
     n_sep = []
     t_start = time.perf_counter()
@@ -426,6 +434,7 @@ def main():
         total_tests += st.num_ci_test
         kernel_ms += ks["kernel_ms"]
         alg_bytes += ks["alg_bytes"]
+        packed_bytes += ks["packed_bytes"]
         launches += ks["launches"]
         n_sep.append(st.n_sepsets)
     elapsed = time.perf_counter() - t_start
@@ -469,7 +478,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_count_stat", "kernel_ms_per_step": kernel_ms / args.steps,
                      "launches_per_step": launches / args.steps,
-                     "alg_bytes_per_step": alg_bytes / args.steps},
+                     "alg_bytes_per_step": alg_bytes / args.steps,
+                     # the same tests in the layout the kernels read (columns packed at 2, 4 or 8
+                     # bits per value by cardinality): the bytes they cannot avoid reading
+                     "packed": {"bytes_per_step": packed_bytes / args.steps,
+                                "achieved": packed_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0,
+                                "frac": (packed_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                         if kernel_ms > 0 else 0.0)}},
     }
 
     traffic = None

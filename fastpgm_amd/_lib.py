@@ -29,7 +29,7 @@ FPGM_F_NO_GRAM = 4
 EXPORTS = [
     "fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ctx_destroy", "fpgm_last_error", "fpgm_status_str",
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
-    "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats",
+    "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats", "fpgm_last_packed_bytes",
     "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
     "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear",
 ]
@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
     L.fpgm_chisq_crit.argtypes = [i32, f64]
     L.fpgm_chisq_crit.restype = f64
     L.fpgm_last_kernel_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
+    L.fpgm_last_packed_bytes.argtypes = [vp, C.POINTER(f64)]
     L.fpgm_ctx_create_multi.argtypes = [vp, vp, vp, i32, i32, vp, C.POINTER(vp)]
     L.fpgm_last_gram_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), C.POINTER(i64)]
     L.fpgm_pc_skeleton_multi.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, vp, vp, C.POINTER(Stats)]
@@ -134,7 +135,7 @@ def load(path: str = LIB_PATH):
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
                  "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
-                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear"):
+                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -359,7 +360,9 @@ class Context:
     def kernel_stats(self):
         ms, b, n = C.c_double(), C.c_double(), C.c_int64()
         self._check(self._L.fpgm_last_kernel_stats(self.h, C.byref(ms), C.byref(b), C.byref(n)))
-        return {"kernel_ms": ms.value, "alg_bytes": b.value, "launches": n.value}
+        pb = C.c_double()
+        self._check(self._L.fpgm_last_packed_bytes(self.h, C.byref(pb)))
+        return {"kernel_ms": ms.value, "alg_bytes": b.value, "launches": n.value, "packed_bytes": pb.value}
 
 
 def pchisq(x, df):

@@ -148,6 +148,7 @@ struct Slot {
     bool pending = false;
     int64_t n = 0;       // tests of the pending launch sequence
     double bytes = 0.0;  // their algorithmic bytes
+    double pbytes = 0.0; // and the bytes of their packed columns
     struct GroupInfo {
         int64_t tests = 0, items = 0, dsum = 0;
         double csum = 0.0, cmax = 0.0, bytes = 0.0;
@@ -265,7 +266,7 @@ struct fpgm_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int trace = 0;               // > 0: one line per counting launch on stderr
     double t_prep = 0.0, t_wait = 0.0;  // host seconds building launches / waiting for them (trace)
-    double k_ms = 0.0, k_bytes = 0.0;
+    double k_ms = 0.0, k_bytes = 0.0, k_pbytes = 0.0;
     int64_t k_launches = 0;
 
     // level-0 Gram path
@@ -329,6 +330,14 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S);
 // Builds and enqueues one launch sequence for tests [i0, i1) of a call, as many as fit
 // the workspace limit (*i1_out receives i1); nothing waits here.  The slot's pinned
 // buffers stage the descriptors, and its results arrive in slot.h_res once
+// Bytes per row of a test's packed columns (X, Y and Z at their widths).
+double packed_row_bytes(const fpgm_ctx *ctx, const HostTest &h, const int32_t *zpool)
+{
+    int bits = ctx->colw[h.x] + ctx->colw[h.y];
+    for (int j = 0; j < h.d; ++j) bits += ctx->colw[zpool[h.z_off + j]];
+    return bits / 8.0;
+}
+
 // batch_wait returns.  flags: FPGM_F_*, tflags: extra kTF* bits for every test.
 // keep_ws receives the workspace offset of test i0 when kTFKeep is set.
 fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t n, const int32_t *zpool,
@@ -465,6 +474,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     int64_t ws_used = 0, n_items_total = 0, nb = 0;
     int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
     S.bytes = 0.0;
+    S.pbytes = 0.0;
     for (; nb < m; ++nb) {
         DevTest &T = dt[nb];
         if (need[nb] > 0) {
@@ -478,6 +488,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         grp_len[gidx[nb]] += cnt;
         n_items_total += cnt;
         S.bytes += (double)(T.d + 2) * (double)ctx->inst[T.inst].n_rows;
+        S.pbytes += packed_row_bytes(ctx, tests[i0 + nb], zpool) * (double)ctx->inst[T.inst].n_rows;
         if (n_items_total > ((int64_t)1 << 30)) {
             ++nb;
             break;
@@ -677,6 +688,7 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
     HIP_TRY(hipEventElapsedTime(&ms, S.ev0, S.ev1), "event time");
     ctx->k_ms += ms;
     ctx->k_bytes += S.bytes;
+    ctx->k_pbytes += S.pbytes;
     if (ctx->trace) {
         static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
         for (int gi = 0; gi < kGroups; ++gi) {
@@ -720,6 +732,7 @@ void reset_kernel_stats(fpgm_ctx *ctx)
 {
     ctx->k_ms = 0.0;
     ctx->k_bytes = 0.0;
+    ctx->k_pbytes = 0.0;
     ctx->k_launches = 0;
     ctx->g_ms = 0.0;
     ctx->g_ops = 0.0;
@@ -1619,7 +1632,7 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
             R.r = ctx->slot[R.slot].h_res.p;
         }
         const DevResult *r = R.r;
-        double skipped_bytes = 0.0;
+        double skipped_bytes = 0.0, skipped_pbytes = 0.0;
         int64_t host_skipped = 0, dev_skipped = 0;
         bool any_done = false;
         for (size_t k = 0; k < R.tv.size();) {
@@ -1630,6 +1643,9 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
             EdgeWalk &w = I.walks[active[g].second];
             if (done_round[g] < R.id) {
                 skipped_bytes += (double)(k1 - k) * (d + 2) * (double)ctx->inst[active[g].first].n_rows;
+                for (size_t j = k; j < k1; ++j)
+                    skipped_pbytes +=
+                        packed_row_bytes(ctx, R.tv[j], R.zpool.data()) * (double)ctx->inst[active[g].first].n_rows;
                 if (ctx->trace)
                     for (size_t j = k; j < k1; ++j) dev_skipped += (r[j].df == kSkippedDf);
                 host_skipped += (int64_t)(k1 - k);
@@ -1658,6 +1674,7 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
             }
         }
         ctx->k_bytes -= skipped_bytes;  // batch_wait counted every submitted test
+        ctx->k_pbytes -= skipped_pbytes;
         if (ctx->trace)
             fprintf(stderr, "[fpgm]   round %d: %zu tests, %lld skipped (device skipped %lld)\n", R.id, R.tv.size(),
                     (long long)host_skipped, (long long)dev_skipped);
@@ -2185,6 +2202,13 @@ fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *i
     if (int8_ops) *int8_ops = ctx->g_ops;
     if (pairs_ms) *pairs_ms = ctx->g_pair_ms;
     if (launches) *launches = ctx->g_launches;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_last_packed_bytes(const fpgm_ctx *ctx, double *packed_bytes)
+{
+    if (!ctx) return FPGM_EINVAL;
+    if (packed_bytes) *packed_bytes = ctx->k_pbytes;
     return FPGM_OK;
 }
 

@@ -236,6 +236,10 @@ double fpgm_chisq_crit(int32_t df, double alpha);
 fpgm_status fpgm_last_kernel_stats(const fpgm_ctx *ctx, double *kernel_ms, double *alg_bytes,
                                    int64_t *launches);
 
+/* The same launches' bytes in the layout the counting kernels read: per test, N rows of
+ * its packed columns (2, 4 or 8 bits per value by cardinality) instead of (|Z|+2) * N. */
+fpgm_status fpgm_last_packed_bytes(const fpgm_ctx *ctx, double *packed_bytes);
+
 /* Level-0 Gram path of the last fpgm_pc_skeleton call: MFMA kernel time, int8 operations
  * (2 * rows * 128 * 128 per upper-triangle tile), per-pair statistic kernel time. */
 fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *int8_ops, double *pairs_ms,

@@ -57,6 +57,22 @@ def test_counts_bit_exact_packed_widths(monkeypatch, n_rows):
         assert np.array_equal(u, o), (x, y, z)
 
 
+def test_packed_bytes_accounting(monkeypatch):
+    # bytes of the packed columns the counting launches read: below the uint8 figure when
+    # columns are packed, equal to it when every column keeps 8 bits
+    w = configs.get(2)
+    s = w.sample(8_000)
+    with _ctx(s) as ctx:
+        ctx.pc_skeleton(w.alpha, 3)
+        a = ctx.kernel_stats()
+    monkeypatch.setenv("FPGM_PACK", "0")
+    with _ctx(s) as ctx:
+        ctx.pc_skeleton(w.alpha, 3)
+        b = ctx.kernel_stats()
+    assert 0 < a["packed_bytes"] < a["alg_bytes"]
+    assert a["alg_bytes"] == b["alg_bytes"] and b["packed_bytes"] == b["alg_bytes"]
+
+
 def test_device_columns_with_dirty_padding():
     # columns handed over in device memory: the padding bytes past n_rows hold 0xFF, which
     # the packed copy must neither count nor let spill into valid rows
