@@ -91,29 +91,31 @@ __device__ __forceinline__ uint32_t wshift(uint32_t w)
     return w == 8 ? 0u : w == 4 ? 1u : 2u;
 }
 
-__device__ __forceinline__ uint4 unpack4(uint32_t x, uint32_t y)
+// The unpacked bytes ORed with `orv` (0, or the per-lane sinks' bias in every byte: one
+// v_and_or_b32 with the mask, inside the branch of the width).
+__device__ __forceinline__ uint4 unpack4(uint32_t x, uint32_t y, uint32_t orv)
 {
     const uint32_t m = 0x0F0F0F0Fu;
-    return make_uint4(x & m, (x >> 4) & m, y & m, (y >> 4) & m);
+    return make_uint4((x & m) | orv, ((x >> 4) & m) | orv, (y & m) | orv, ((y >> 4) & m) | orv);
 }
 
-__device__ __forceinline__ uint4 unpack2(uint32_t x)
+__device__ __forceinline__ uint4 unpack2(uint32_t x, uint32_t orv)
 {
     const uint32_t m = 0x03030303u;
-    return make_uint4(x & m, (x >> 2) & m, (x >> 4) & m, (x >> 6) & m);
+    return make_uint4((x & m) | orv, ((x >> 2) & m) | orv, ((x >> 4) & m) | orv, ((x >> 6) & m) | orv);
 }
 
-__device__ __forceinline__ uint4 unpackw(const uint4 &p, uint32_t w)
+__device__ __forceinline__ uint4 unpackw(const uint4 &p, uint32_t w, uint32_t orv = 0u)
 {
-    if (w == 8) return p;
-    if (w == 4) return unpack4(p.x, p.y);
-    return unpack2(p.x);
+    if (w == 8) return make_uint4(p.x | orv, p.y | orv, p.z | orv, p.w | orv);
+    if (w == 4) return unpack4(p.x, p.y, orv);
+    return unpack2(p.x, orv);
 }
 
-__device__ __forceinline__ uint4 unpackw(const uint2 &p, uint32_t w)
+__device__ __forceinline__ uint4 unpackw(const uint2 &p, uint32_t w, uint32_t orv = 0u)
 {
-    if (w == 4) return unpack4(p.x, p.y);
-    return unpack2(p.x);
+    if (w == 4) return unpack4(p.x, p.y, orv);
+    return unpack2(p.x, orv);
 }
 
 __device__ __forceinline__ uint32_t uniform(int32_t v)
@@ -265,13 +267,13 @@ struct ColSet {
         c.v[X] = ldw<NW>(p[X], s[X], r);
         c.v[X + 1] = ldw<NW>(p[X + 1], s[X + 1], r);
     }
-    // ... as bytes, row b of the group in byte b
-    __device__ __forceinline__ void unpack(const Cols<D, NW> &c, Cols<D> &u) const
+    // ... as bytes, row b of the group in byte b (Y's bytes ORed with ybias)
+    __device__ __forceinline__ void unpack(const Cols<D, NW> &c, Cols<D> &u, uint32_t ybias = 0u) const
     {
 #pragma unroll
         for (int j = 0; j < D; ++j) u.v[j] = unpackw(c.v[j], w[j]);
         u.v[X] = unpackw(c.v[X], w[X]);
-        u.v[X + 1] = unpackw(c.v[X + 1], w[X + 1]);
+        u.v[X + 1] = unpackw(c.v[X + 1], w[X + 1], ybias);
     }
 };
 
@@ -398,14 +400,12 @@ __device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
 }
 
 // Counts the (up to) 4 rows whose keys are the bytes of K; rows >= nvalid are skipped.
-// A row's byte offset in the counters is pair * kCells * 256 + key * 256 + lane * 4: the
-// wave pair's bit is ORed into every key byte (bias: 0, or kCells in each byte, a bit no
-// key uses), then one v_perm_b32 puts key byte j into byte 1 and lane * 4 (lane4, byte 0)
+// A row's byte offset in the counters is pair * kCells * 256 + key * 256 + lane * 4: every
+// key byte already carries the wave pair's bias (0, or kCells: a bit no key uses; see
+// lane_key4), and one v_perm_b32 puts key byte j into byte 1 and lane * 4 (lane4, byte 0)
 // into byte 0.
-__device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, uint32_t bias,
-                                          int nvalid)
+__device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, int nvalid)
 {
-    K |= bias;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
         if (j < nvalid) {
@@ -415,6 +415,8 @@ __device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc
 }
 
 // Packed keys of the 4 rows of word q, |Z| = D known at compile time.
+// c's Y bytes carry the wave pair's bias (ORed in at unpacking): adding it to every y adds
+// it to every key, which stays below 256.
 template <int D>
 __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *zc2, uint32_t cx2, uint32_t cy2,
                                               int q)
@@ -440,15 +442,15 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
     int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
     auto step = [&](const Cols<D, NW> &raw, int64_t r) {
         Cols<D> cur;
-        cs.unpack(raw, cur);
+        cs.unpack(raw, cur, bias);
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), bias, 4);
+            for (int q = 0; q < 4; ++q) lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), 4);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), bias, (int)nv - 4 * q);
+                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), (int)nv - 4 * q);
         }
     };
     Cols<D, NW> b0, b1;
@@ -485,12 +487,12 @@ __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevT
             for (int q = 0; q < 4; ++q) K[q] = pk_mad(K[q], c2, word_of(v, q));
         }
         const uint4 vx = unpackw(ldw<4>(colx, wshift(wx), r), wx);
-        const uint4 vy = unpackw(ldw<4>(coly, wshift(wy), r), wy);
+        const uint4 vy = unpackw(ldw<4>(coly, wshift(wy), r), wy, bias);
         const int64_t nv = r_end - r;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q));
-            lane_put4(lb, lane4, inc, k, bias, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
+            lane_put4(lb, lane4, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
         }
     }
 }
