@@ -120,9 +120,8 @@ constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
 constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 64 KiB (<= 128 cells)
 constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
-// Row loop of the shared sink's column layout (tables <= kColCells): not a launch of its own.
-constexpr int kSinkColumns = 5;
-constexpr int kColCells = kLdsCells / 32;  // [cell][32] int32 words in the shared sink's LDS (256)
+constexpr int kColCells = kLdsCells / 32;  // shared sink's [cell][32] column layout up to 256 cells
+constexpr int64_t kColMinRows = 65536;     // ... for work items of at least this many rows
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 
 // Compaction of a round's work items before its counting launches: items of tests whose
@@ -140,7 +139,8 @@ struct CompactArgs {
     uint32_t *count;   // [5], zero before the launch
 };
 
-hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s);
+// packed: some column of the context is packed (the kernels' packed-column form)
+hipError_t launch_count_stat(const LaunchArgs &a, int mode, bool packed, int grid, hipStream_t s);
 hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s);
 hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
                                 const WideTest *wt, int64_t n_wide, hipStream_t s);

@@ -64,6 +64,10 @@ template <>
 struct RawOf<2> {
     typedef uint2 T;
 };
+template <>
+struct RawOf<8> {  // uint8 columns (no column packed): 16 rows in 16 bytes, nothing to unpack
+    typedef uint4 T;
+};
 
 typedef unsigned int u32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
 typedef unsigned int u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
@@ -76,6 +80,12 @@ __device__ __forceinline__ uint4 ldw<4>(const uint8_t *col, uint32_t sh, int64_t
 {
     const u32x4a4 v = *(const __attribute__((address_space(1))) u32x4a4 *)(col + (r >> sh));
     return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+template <>
+__device__ __forceinline__ uint4 ldw<8>(const uint8_t *col, uint32_t, int64_t r)
+{
+    return ld16(col + r);
 }
 
 template <>
@@ -225,11 +235,12 @@ __device__ __forceinline__ void sink_slice(int32_t *h, uint32_t key, uint32_t lo
     if (k < len) atomicAdd(&h[k], 1);
 }
 
+// ksh: the shared sink's key shift (5 in its column layout, h the lane's column; else 0)
 template <int kSink>
-__device__ __forceinline__ void put(int32_t *h, uint32_t key, const Shared &sh)
+__device__ __forceinline__ void put(int32_t *h, uint32_t key, const Shared &sh, uint32_t ksh)
 {
     if (kSink == kSinkSlice) sink_slice(h, key, (uint32_t)sh.slice_lo, (uint32_t)sh.slice_len);
-    else if (kSink == kSinkColumns) atomicAdd(&h[key << 5], 1);  // h: the lane's column
+    else if (kSink == kSinkShared) atomicAdd(&h[key << ksh], 1);
     else sink<kSink>(h, key);
 }
 
@@ -270,10 +281,17 @@ struct ColSet {
     // ... as bytes, row b of the group in byte b (Y's bytes ORed with ybias)
     __device__ __forceinline__ void unpack(const Cols<D, NW> &c, Cols<D> &u, uint32_t ybias = 0u) const
     {
+        if constexpr (NW == 8) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) u.v[j] = unpackw(c.v[j], w[j]);
-        u.v[X] = unpackw(c.v[X], w[X]);
-        u.v[X + 1] = unpackw(c.v[X + 1], w[X + 1], ybias);
+            for (int j = 0; j < X + 2; ++j) u.v[j] = c.v[j];
+            u.v[X + 1] = make_uint4(u.v[X + 1].x | ybias, u.v[X + 1].y | ybias, u.v[X + 1].z | ybias,
+                                    u.v[X + 1].w | ybias);
+        } else {
+#pragma unroll
+            for (int j = 0; j < D; ++j) u.v[j] = unpackw(c.v[j], w[j]);
+            u.v[X] = unpackw(c.v[X], w[X]);
+            u.v[X + 1] = unpackw(c.v[X + 1], w[X + 1], ybias);
+        }
     }
 };
 
@@ -281,7 +299,7 @@ struct ColSet {
 // Loads for the next step are issued before the current step is counted.
 template <int D, int kSink, int NW>
 __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T, const Shared &sh,
-                                             int64_t r_begin, int64_t r_end, int32_t *h)
+                                             int64_t r_begin, int64_t r_end, int32_t *h, uint32_t ksh)
 {
     const ColSet<D, NW> cs(I, T, sh);
     uint32_t zc[D > 0 ? D : 1];
@@ -309,11 +327,11 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int b = 0; b < kRowsPerThread; ++b) put<kSink>(h, key[b], sh);
+            for (int b = 0; b < kRowsPerThread; ++b) put<kSink>(h, key[b], sh, ksh);
         } else {
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                if (b < nv) put<kSink>(h, key[b], sh);
+                if (b < nv) put<kSink>(h, key[b], sh, ksh);
         }
     };
     Cols<D, NW> b0, b1;
@@ -334,7 +352,7 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
 // Runtime |Z| (d > 3).
 template <int kSink>
 __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &T, const Shared &sh,
-                                               int64_t r_begin, int64_t r_end, int32_t *h)
+                                               int64_t r_begin, int64_t r_end, int32_t *h, uint32_t ksh)
 {
     const uint8_t *colx = I.pcols + (int64_t)T.x * I.stride;
     const uint8_t *coly = I.pcols + (int64_t)T.y * I.stride;
@@ -363,11 +381,11 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int b = 0; b < kRowsPerThread; ++b) put<kSink>(h, key[b], sh);
+            for (int b = 0; b < kRowsPerThread; ++b) put<kSink>(h, key[b], sh, ksh);
         } else {
 #pragma unroll
             for (int b = 0; b < kRowsPerThread; ++b)
-                if (b < nv) put<kSink>(h, key[b], sh);
+                if (b < nv) put<kSink>(h, key[b], sh, ksh);
         }
     }
 }
@@ -500,7 +518,12 @@ __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevT
 // blk: the workgroup's counters, [pair][cell][lane] 32-bit words, kCells cells a pair.
 // |Z| <= 3 takes loops with |Z| known at compile time: with 8-byte loads when every
 // column of the test is packed below 8 bits, 16-byte loads otherwise.
-template <int kCells>
+// Row loops by column form.  kPacked (some column of the context is packed): tests whose
+// columns are all packed (kTFNarrow) with |Z| <= 3 take loops with |Z| known at compile
+// time and 8-byte loads; the others (d > 3, or a column of more than 16 states) the
+// runtime-|Z| loop.  Otherwise (uint8 columns) |Z| <= 3 takes compile-time loops with
+// 16-byte loads and nothing to unpack.  One form per kernel keeps its registers low.
+template <int kCells, bool kPacked>
 __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest &T, const Shared &sh,
                                                 int64_t r_begin, int64_t r_end, uint32_t *blk)
 {
@@ -509,63 +532,46 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
     const uint32_t inc = (wave & 1) ? 0x10000u : 1u;
     const uint32_t bias = (wave >> 1) ? (uint32_t)kCells * 0x01010101u : 0u;
     const uint32_t lane4 = lane * 4u;
-    if (uniform(T.flags) & kTFNarrow) {
+    constexpr int NW = kPacked ? 2 : 8;
+    if (T.d <= 3 && (!kPacked || (uniform(T.flags) & kTFNarrow))) {
         switch (T.d) {
-        case 0: count_rows_lane_d<0, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        case 1: count_rows_lane_d<1, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        case 2: count_rows_lane_d<2, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        case 3: count_rows_lane_d<3, 2>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        default: break;
+        case 0: count_rows_lane_d<0, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 1: count_rows_lane_d<1, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 2: count_rows_lane_d<2, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        default: count_rows_lane_d<3, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
         }
-    }
-    switch (T.d) {
-    case 0: count_rows_lane_d<0, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-    case 1: count_rows_lane_d<1, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-    case 2: count_rows_lane_d<2, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-    case 3: count_rows_lane_d<3, 4>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-    default: break;
     }
     count_rows_lane_any(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
 }
 
-// |Z| <= 3, every column packed below 8 bits (the caller checks)
-template <int kSink>
-__device__ __forceinline__ void count_rows_narrow(const DevInst &I, const DevTest &T, const Shared &sh,
-                                                  int64_t r_begin, int64_t r_end, int32_t *h)
+// |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
+// a packed context's test has all its columns packed)
+template <int kSink, bool kPacked>
+__device__ __forceinline__ void count_rows_fixed(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                 int64_t r_begin, int64_t r_end, int32_t *h, uint32_t ksh)
 {
+    constexpr int NW = kPacked ? 2 : 8;
     switch (T.d) {
-    case 0: count_rows_d<0, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
-    case 1: count_rows_d<1, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
-    case 2: count_rows_d<2, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
-    default: count_rows_d<3, kSink, 2>(I, T, sh, r_begin, r_end, h); break;
+    case 0: count_rows_d<0, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+    case 1: count_rows_d<1, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+    case 2: count_rows_d<2, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+    default: count_rows_d<3, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
     }
 }
 
-template <int kSink>
-__device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, const Shared &sh,
-                                           int64_t r_begin, int64_t r_end, int32_t *h)
+template <bool kPacked>
+__device__ __forceinline__ bool fixed_loop(const DevTest &T)
 {
-    if (uniform(T.flags) & kTFNarrow) {
-        switch (T.d) {
-        case 0: count_rows_d<0, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
-        case 1: count_rows_d<1, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
-        case 2: count_rows_d<2, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
-        case 3: count_rows_d<3, kSink, 2>(I, T, sh, r_begin, r_end, h); return;
-        default: break;
-        }
-    }
-    // the shared sink holds two row loops (copies and columns) and must stay within 128
-    // VGPRs: its tests with an 8-bit column take the runtime-|Z| loop
-    if (kSink != kSinkShared) {
-        switch (T.d) {
-        case 0: count_rows_d<0, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-        case 1: count_rows_d<1, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-        case 2: count_rows_d<2, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-        case 3: count_rows_d<3, kSink, 4>(I, T, sh, r_begin, r_end, h); return;
-        default: break;
-        }
-    }
-    count_rows_any<kSink>(I, T, sh, r_begin, r_end, h);
+    return T.d <= 3 && (!kPacked || (uniform(T.flags) & kTFNarrow));
+}
+
+template <int kSink, bool kPacked>
+__device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, const Shared &sh,
+                                           int64_t r_begin, int64_t r_end, int32_t *h, uint32_t ksh = 0u)
+{
+    const uint32_t k = uniform((int32_t)ksh);
+    if (fixed_loop<kPacked>(T)) count_rows_fixed<kSink, kPacked>(I, T, sh, r_begin, r_end, h, k);
+    else count_rows_any<kSink>(I, T, sh, r_begin, r_end, h, k);
 }
 
 // Critical value of the chi-square test at the call's alpha: tabulated (exact FP64
@@ -829,7 +835,7 @@ struct LdsWords {
 // Waves per SIMD the register allocation must allow: the LDS of the per-lane and shared
 // sinks leaves room for 4 workgroups per CU (one wave each per SIMD), so those sinks are
 // held to 128 VGPRs; the 64 KiB sinks fit 2 workgroups per CU whatever their registers.
-template <int kMode>
+template <int kMode, bool kPacked>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
     (kMode == kSinkLane || kMode == kSinkShared) ? 4 : 1))) void k_count_stat(LaunchArgs a)
 {
@@ -903,7 +909,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
             for (int i = tid; i < 2 * kCells * 64; i += kWG) pl[i] = 0;
             __syncthreads();
-            count_rows_lane<kCells>(I, T, sh, r_begin, r_end, pl);
+            count_rows_lane<kCells, kPacked>(I, T, sh, r_begin, r_end, pl);
             __syncthreads();
             // sum the 256 private counters of every cell: 8 threads per cell, each over
             // 8 lanes of both pairs (both halves), then a 3-step shuffle reduction
@@ -922,48 +928,51 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             }
             __syncthreads();
             tab = small;
-        } else if (kMode == kSinkShared && cells <= kColCells && (T.flags & kTFNarrow) && T.d <= 3) {
-            // Column layout: one table of [cell][32] words for the workgroup, lane l
-            // counting in column l & 31, so the 32 lanes of a half-wave always hit 32
-            // distinct banks, however skewed the data (waves meet only through the
-            // atomics, never within one instruction).  Then thread t sums cell t, reading
-            // its columns from (t + j) & 31 on: again one bank per lane.
-            const int c32 = (int)cells;
-            for (int i = tid; i < c32 * 32; i += kWG) lds[i] = 0;
-            __syncthreads();
-            count_rows_narrow<kSinkColumns>(I, T, sh, r_begin, r_end, lds + (lane & 31));
-            __syncthreads();
-            int32_t sum = 0;
-            if (tid < c32)
-                for (int j = 0; j < 32; ++j) sum += lds[tid * 32 + ((tid + j) & 31)];
-            __syncthreads();
-            if (tid < c32) lds[tid] = sum;
-            __syncthreads();
-            tab = lds;
         } else if (kMode == kSinkShared) {
-            // As many histogram copies as fit: `sub` copies per wave, lane l counting into
-            // copy l % sub of its wave, so skewed data collides on one address in at most
-            // 64 / sub lanes of a wave; copies are offset by 4 banks from one another, so
-            // lanes of different copies hitting the same cell use different banks.  Tables
-            // too large for one copy per wave share copies between waves.
+            // Tables of at most kColCells cells: column layout, one table of [cell][32]
+            // words for the workgroup, lane l counting in column l & 31 (key << 5 from the
+            // lane's column), so the 32 lanes of a half-wave always hit 32 distinct banks,
+            // however skewed the data (waves meet only through the atomics, never within
+            // one instruction); then thread t sums cell t, reading its columns from
+            // (t + j) & 31 on: again one bank per lane.  Larger tables: as many histogram
+            // copies as fit, `sub` copies per wave, lane l counting into copy l % sub of
+            // its wave, so skewed data collides on one address in at most 64 / sub lanes;
+            // copies are offset by 4 banks from one another, so lanes of different copies
+            // hitting the same cell use different banks; tables too large for one copy per
+            // wave share copies between waves.  One row loop serves both (the key shift).
             const int c32 = (int)cells;
-            const int stride = ((c32 + 63) & ~63) + 4;
-            int sub = 1;
-            while (sub < 16 && 4 * (2 * sub) * stride <= kLdsCells) sub *= 2;
-            int copies, my;
-            if (4 * stride <= kLdsCells) {
-                copies = 4 * sub;
-                my = wave * sub + (lane & (sub - 1));
-            } else {
-                copies = (2 * stride <= kLdsCells) ? 2 : 1;
-                my = wave % copies;
+            // (the column layout clears and reduces twice the words of the copies: it is
+            // used for items of enough rows to repay that)
+            const bool col = c32 <= kColCells && r_end - r_begin >= kColMinRows;
+            int copies = 1, my = 0, stride = c32, span = c32 * 32;
+            if (!col) {
+                stride = ((c32 + 63) & ~63) + 4;
+                int sub = 1;
+                while (sub < 16 && 4 * (2 * sub) * stride <= kLdsCells) sub *= 2;
+                if (4 * stride <= kLdsCells) {
+                    copies = 4 * sub;
+                    my = wave * sub + (lane & (sub - 1));
+                } else {
+                    copies = (2 * stride <= kLdsCells) ? 2 : 1;
+                    my = wave % copies;
+                }
+                span = (copies > 1) ? copies * stride : c32;
             }
-            const int span = (copies > 1) ? copies * stride : c32;
             for (int i = tid; i < span; i += kWG) lds[i] = 0;
             __syncthreads();
-            count_rows<kSinkShared>(I, T, sh, r_begin, r_end, lds + (copies > 1 ? my * stride : 0));
+            count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end,
+                                             col ? lds + (lane & 31) : lds + (copies > 1 ? my * stride : 0),
+                                             col ? 5u : 0u);
             __syncthreads();
-            if (copies > 1) {
+            if (col) {
+                int32_t sum = 0;
+                if (tid < c32)
+#pragma unroll 4
+                    for (int j = 0; j < 32; ++j) sum += lds[tid * 32 + ((tid + j) & 31)];
+                __syncthreads();
+                if (tid < c32) lds[tid] = sum;
+                __syncthreads();
+            } else if (copies > 1) {
                 for (int i = tid; i < c32; i += kWG) {
                     int32_t s = lds[i];
                     for (int k = 1; k < copies; ++k) s += lds[k * stride + i];
@@ -981,14 +990,14 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             }
             for (int i = tid; i < len; i += kWG) lds[i] = 0;
             __syncthreads();
-            count_rows<kSinkSlice>(I, T, sh, r_begin, r_end, lds);
+            count_rows<kSinkSlice, kPacked>(I, T, sh, r_begin, r_end, lds);
             __syncthreads();
             for (int i = tid; i < len; i += kWG) {
                 const int32_t v = lds[i];
                 if (v) atomicAdd(&g[lo + i], v);
             }
         } else {
-            count_rows<kSinkGlobal>(I, T, sh, r_begin, r_end, g);
+            count_rows<kSinkGlobal, kPacked>(I, T, sh, r_begin, r_end, g);
         }
         if (tab) {
             if (T.flags & kTFLdsMarg) {
@@ -1228,16 +1237,23 @@ __global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, doubl
 
 }  // namespace
 
-hipError_t launch_count_stat(const LaunchArgs &a, int mode, int grid, hipStream_t s)
+template <bool kPacked>
+void launch_count_stat_form(const LaunchArgs &a, int mode, int grid, hipStream_t s)
+{
+    switch (mode) {
+    case kSinkLane: hipLaunchKernelGGL((k_count_stat<kSinkLane, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkLaneW: hipLaunchKernelGGL((k_count_stat<kSinkLaneW, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkSlice: hipLaunchKernelGGL((k_count_stat<kSinkSlice, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkShared: hipLaunchKernelGGL((k_count_stat<kSinkShared, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_count_stat<kSinkGlobal, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    }
+}
+
+hipError_t launch_count_stat(const LaunchArgs &a, int mode, bool packed, int grid, hipStream_t s)
 {
     if (a.n_items <= 0) return hipSuccess;
-    switch (mode) {
-    case kSinkLane: hipLaunchKernelGGL(k_count_stat<kSinkLane>, dim3(grid), dim3(kWG), 0, s, a); break;
-    case kSinkLaneW: hipLaunchKernelGGL(k_count_stat<kSinkLaneW>, dim3(grid), dim3(kWG), 0, s, a); break;
-    case kSinkSlice: hipLaunchKernelGGL(k_count_stat<kSinkSlice>, dim3(grid), dim3(kWG), 0, s, a); break;
-    case kSinkShared: hipLaunchKernelGGL(k_count_stat<kSinkShared>, dim3(grid), dim3(kWG), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_count_stat<kSinkGlobal>, dim3(grid), dim3(kWG), 0, s, a); break;
-    }
+    if (packed) launch_count_stat_form<true>(a, mode, grid, s);
+    else launch_count_stat_form<false>(a, mode, grid, s);
     return hipGetLastError();
 }
 

@@ -119,6 +119,7 @@ struct HostBuf {
 // Counting launches are grouped by sink kind, in this order.
 constexpr int kGroups = 5;
 constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
+constexpr int64_t kPackMinRows = 262144;  // mean rows per instance from which columns are packed
 
 // Buffers of one in-flight batch.  Two slots let the host build one batch while the
 // device runs the other; the workspace and completion counters are shared, since both
@@ -223,8 +224,9 @@ struct fpgm_ctx {
     std::vector<int32_t> card;
     int32_t *d_card = nullptr;
     // packed columns read by the counting kernels: bits per value by cardinality (2 up to
-    // 4 states, 4 up to 16, else 8); FPGM_PACK=0 keeps 8 bits for every column
-    bool pack = true;
+    // 4 states, 4 up to 16, else 8), when the instances are large (kPackMinRows rows on
+    // average); FPGM_PACK=1 packs always, FPGM_PACK=0 never
+    int pack = -1;
     std::vector<uint8_t> colw;
     uint8_t *d_colw = nullptr;
     uint8_t *d_pcols = nullptr;  // owned; nullptr when every column has 8 bits (the uint8 columns are read)
@@ -634,7 +636,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         const int64_t wg_per_cu = ctx->sched.wpc[gi];
         int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wg_per_cu);
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
-        HIP_TRY(launch_count_stat(a, sink, (int)grid, gs), "launch count_stat");
+        HIP_TRY(launch_count_stat(a, sink, ctx->d_pcols != nullptr, (int)grid, gs), "launch count_stat");
         ctx->k_launches += 1;
         if (sink == kSinkGlobal && n_wt > 0) {
             HIP_TRY(launch_finalize_wide(a, S.d_wide_items.p, n_wi, S.d_wide_parts.p, S.d_wide_tests.p, n_wt, gs),
@@ -1025,7 +1027,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         if (std::sscanf(e, "%ld,%ld,%ld,%ld,%ld", &v[0], &v[1], &v[2], &v[3], &v[4]) == kGroups)
             for (int g = 0; g < kGroups; ++g) ctx->sched.wpc[g] = std::min<int64_t>(64, std::max<int64_t>(1, v[g]));
     }
-    if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0;
+    if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1061,7 +1063,13 @@ static fpgm_status finish_ctx(fpgm_ctx *ctx)
     const int n = ctx->n_vars;
     ctx->colw.assign((size_t)n, 8);
     bool any_packed = false;
-    if (ctx->pack)
+    // measured: at 10^6 rows (1000 variables) packing takes the stage from 250 to 208 ms;
+    // with 2 000 - 50 000 rows per instance (bootstrap replicates, whose columns stay in
+    // L2) the unpacking costs more than the bytes it saves (+5 %)
+    double mean_rows = 0.0;
+    for (const InstHost &I : ctx->inst) mean_rows += (double)I.n_rows / (double)ctx->inst.size();
+    const bool pack = ctx->pack > 0 || (ctx->pack < 0 && mean_rows >= (double)kPackMinRows);
+    if (pack)
         for (int v = 0; v < n; ++v) {
             const int c = ctx->card[v];
             ctx->colw[v] = (uint8_t)(c <= 4 ? 2 : c <= 16 ? 4 : 8);

@@ -10,6 +10,14 @@ from helpers import alarm_sample, alarm_struct
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["0", "1"], ids=["uint8", "packed"])
+def column_form(request, monkeypatch):
+    # every test under both forms of the columns the counting kernels read: uint8, and
+    # packed at 2, 4 or 8 bits per value (by default packing follows the instance size)
+    monkeypatch.setenv("FPGM_PACK", request.param)
+    return request.param
+
+
 def _ctx(sample):
     return _lib.Context.of(sample)
 
@@ -46,6 +54,7 @@ def test_counts_bit_exact_packed_widths(monkeypatch, n_rows):
     rng = np.random.default_rng(n_rows + 1)
     tests = [t for t in _rand_tests(rng, s.n_vars, 60, 3)
              if np.prod([s.card[v] for v in (t[0], t[1]) + t[2]]) <= 1 << 22]
+    monkeypatch.setenv("FPGM_PACK", "1")
     with _ctx(s) as ctx:
         got = [ctx.counts(x, y, z) for (x, y, z) in tests]
     monkeypatch.setenv("FPGM_PACK", "0")
@@ -62,6 +71,7 @@ def test_packed_bytes_accounting(monkeypatch):
     # columns are packed, equal to it when every column keeps 8 bits
     w = configs.get(2)
     s = w.sample(8_000)
+    monkeypatch.setenv("FPGM_PACK", "1")
     with _ctx(s) as ctx:
         ctx.pc_skeleton(w.alpha, 3)
         a = ctx.kernel_stats()
