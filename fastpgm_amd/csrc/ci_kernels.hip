@@ -546,6 +546,12 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
 
 // |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
 // a packed context's test has all its columns packed)
+// Largest |Z| with a compile-time loop: 3 for the LDS sinks held to 128 VGPRs; 8 for the
+// z-slice and device-memory sinks, whose LDS (or none) leaves them twice the registers and
+// whose deep-level tests would otherwise wait on every step's loads.
+template <int kSink>
+constexpr int kMaxFixedD = (kSink == kSinkSlice || kSink == kSinkGlobal) ? 8 : 3;
+
 template <int kSink, bool kPacked>
 __device__ __forceinline__ void count_rows_fixed(const DevInst &I, const DevTest &T, const Shared &sh,
                                                  int64_t r_begin, int64_t r_end, int32_t *h, uint32_t ksh)
@@ -555,14 +561,25 @@ __device__ __forceinline__ void count_rows_fixed(const DevInst &I, const DevTest
     case 0: count_rows_d<0, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
     case 1: count_rows_d<1, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
     case 2: count_rows_d<2, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
-    default: count_rows_d<3, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+    case 3: count_rows_d<3, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+    default:
+        if constexpr (kMaxFixedD<kSink> > 3) {
+            switch (T.d) {
+            case 4: count_rows_d<4, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+            case 5: count_rows_d<5, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+            case 6: count_rows_d<6, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+            case 7: count_rows_d<7, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+            default: count_rows_d<8, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
+            }
+        }
+        break;
     }
 }
 
-template <bool kPacked>
+template <bool kPacked, int kSink = kSinkShared>
 __device__ __forceinline__ bool fixed_loop(const DevTest &T)
 {
-    return T.d <= 3 && (!kPacked || (uniform(T.flags) & kTFNarrow));
+    return T.d <= kMaxFixedD<kSink> && (!kPacked || (uniform(T.flags) & kTFNarrow));
 }
 
 template <int kSink, bool kPacked>
@@ -570,7 +587,7 @@ __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, c
                                            int64_t r_begin, int64_t r_end, int32_t *h, uint32_t ksh = 0u)
 {
     const uint32_t k = uniform((int32_t)ksh);
-    if (fixed_loop<kPacked>(T)) count_rows_fixed<kSink, kPacked>(I, T, sh, r_begin, r_end, h, k);
+    if (fixed_loop<kPacked, kSink>(T)) count_rows_fixed<kSink, kPacked>(I, T, sh, r_begin, r_end, h, k);
     else count_rows_any<kSink>(I, T, sh, r_begin, r_end, h, k);
 }
 
