@@ -546,11 +546,13 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
 
 // |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
 // a packed context's test has all its columns packed)
-// Largest |Z| with a compile-time loop: 3 for the LDS sinks held to 128 VGPRs; 8 for the
-// z-slice and device-memory sinks, whose LDS (or none) leaves them twice the registers and
-// whose deep-level tests would otherwise wait on every step's loads.
-template <int kSink>
-constexpr int kMaxFixedD = (kSink == kSinkSlice || kSink == kSinkGlobal) ? 8 : 3;
+// Largest |Z| with a compile-time loop: 3 for the LDS sinks held to 128 VGPRs (4 for the
+// shared sink's packed form, whose narrow loads leave room); 8 for the z-slice and
+// device-memory sinks, whose LDS (or none) leaves them twice the registers and whose
+// deep-level tests would otherwise wait on every step's loads.
+template <int kSink, bool kPacked>
+constexpr int kMaxFixedD =
+    (kSink == kSinkSlice || kSink == kSinkGlobal) ? 8 : (kSink == kSinkShared && kPacked) ? 4 : 3;
 
 template <int kSink, bool kPacked>
 __device__ __forceinline__ void count_rows_fixed(const DevInst &I, const DevTest &T, const Shared &sh,
@@ -563,7 +565,9 @@ __device__ __forceinline__ void count_rows_fixed(const DevInst &I, const DevTest
     case 2: count_rows_d<2, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
     case 3: count_rows_d<3, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
     default:
-        if constexpr (kMaxFixedD<kSink> > 3) {
+        if constexpr (kMaxFixedD<kSink, kPacked> == 4) {
+            count_rows_d<4, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh);
+        } else if constexpr (kMaxFixedD<kSink, kPacked> > 4) {
             switch (T.d) {
             case 4: count_rows_d<4, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
             case 5: count_rows_d<5, kSink, NW>(I, T, sh, r_begin, r_end, h, ksh); break;
@@ -579,7 +583,7 @@ __device__ __forceinline__ void count_rows_fixed(const DevInst &I, const DevTest
 template <bool kPacked, int kSink = kSinkShared>
 __device__ __forceinline__ bool fixed_loop(const DevTest &T)
 {
-    return T.d <= kMaxFixedD<kSink> && (!kPacked || (uniform(T.flags) & kTFNarrow));
+    return T.d <= kMaxFixedD<kSink, kPacked> && (!kPacked || (uniform(T.flags) & kTFNarrow));
 }
 
 template <int kSink, bool kPacked>
