@@ -8,7 +8,7 @@
 
 namespace fpgm {
 
-constexpr int kGramTile = 128;     // output tile edge (4 waves x 64 x 64)
+constexpr int kGramTile = 128;     // output tile edge (2 waves x 128 x 64)
 constexpr int kPairMaxCard = 256;  // per-thread marginal array of the pair kernel
 
 struct GramArgs {

@@ -85,67 +85,86 @@ __device__ __forceinline__ void onehot_lut(uint8_t state, uint32_t &lo, uint32_t
     }
 }
 
-struct Ops4 {
-    uint4 a0, a1, b0, b1;
+// One workgroup of 2 waves per (tile pair, row split): wave w computes the 128 x 64 block
+// of rows ti*128 .. +128 and columns tj*128 + 64w .. +64 as 4 x 2 fragments of 32 x 32 --
+// 6 operand loads for 8 MFMAs per 32-row step (a 64 x 64 block per wave needs 4 for 4).
+constexpr int kGramWaves = 2;
+constexpr int kGramA = 4, kGramB = 2;  // fragments per wave
+
+struct Ops6 {
+    uint4 a[kGramA], b[kGramB];
 };
 
 template <bool kPerm>
-__global__ __launch_bounds__(256) void k_gram_l0(GramArgs g)
+__global__ __launch_bounds__(kGramWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_gram_l0(GramArgs g)
 {
     const int item = blockIdx.x;
     const int tp = item / g.k_splits, ks = item - tp * g.k_splits;
     const int ti = g.tiles[2 * tp], tj = g.tiles[2 * tp + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int h = lane >> 5, l32 = lane & 31;
-    const int ib = ti * kGramTile + (wave >> 1) * 64 + l32;
-    const int jb = tj * kGramTile + (wave & 1) * 64 + l32;
-    const uint8_t *pa0 = g.cols + (int64_t)g.row_var[ib] * g.stride + h * 16;
-    const uint8_t *pa1 = g.cols + (int64_t)g.row_var[ib + 32] * g.stride + h * 16;
-    const uint8_t *pb0 = g.cols + (int64_t)g.row_var[jb] * g.stride + h * 16;
-    const uint8_t *pb1 = g.cols + (int64_t)g.row_var[jb + 32] * g.stride + h * 16;
-    uint32_t la0, ha0, la1, ha1, lb0, hb0, lb1, hb1;
-    onehot_lut<kPerm>(g.row_state[ib], la0, ha0);
-    onehot_lut<kPerm>(g.row_state[ib + 32], la1, ha1);
-    onehot_lut<kPerm>(g.row_state[jb], lb0, hb0);
-    onehot_lut<kPerm>(g.row_state[jb + 32], lb1, hb1);
+    const int i_base = ti * kGramTile;
+    const int j_base = tj * kGramTile + wave * 64;
+    const uint8_t *pa[kGramA], *pb[kGramB];
+    uint32_t la[kGramA], ha[kGramA], lb[kGramB], hb[kGramB];
+#pragma unroll
+    for (int f = 0; f < kGramA; ++f) {
+        const int r = i_base + 32 * f + l32;
+        pa[f] = g.cols + (int64_t)g.row_var[r] * g.stride + h * 16;
+        onehot_lut<kPerm>(g.row_state[r], la[f], ha[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < kGramB; ++f) {
+        const int r = j_base + 32 * f + l32;
+        pb[f] = g.cols + (int64_t)g.row_var[r] * g.stride + h * 16;
+        onehot_lut<kPerm>(g.row_state[r], lb[f], hb[f]);
+    }
 
-    v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
+    v16i c[kGramA][kGramB];
+#pragma unroll
+    for (int f = 0; f < kGramA; ++f)
+#pragma unroll
+        for (int e = 0; e < kGramB; ++e) c[f][e] = v16i{0};
     const int64_t r_begin = (int64_t)ks * g.k_per_split;
     const int64_t r_end = (r_begin + g.k_per_split < g.n_rows) ? r_begin + g.k_per_split : g.n_rows;
     // full 32-row steps: no masks; the loads of the next step are issued before the
     // current step's one-hot formation and MFMAs (two register buffers in turn)
     const int64_t full_end = r_begin + (r_end - r_begin) / 32 * 32;
-    auto load = [&](int64_t r0, Ops4 &o) {
-        o.a0 = *reinterpret_cast<const uint4 *>(pa0 + r0);
-        o.a1 = *reinterpret_cast<const uint4 *>(pa1 + r0);
-        o.b0 = *reinterpret_cast<const uint4 *>(pb0 + r0);
-        o.b1 = *reinterpret_cast<const uint4 *>(pb1 + r0);
+    auto load = [&](int64_t r0, Ops6 &o) {
+#pragma unroll
+        for (int f = 0; f < kGramA; ++f) o.a[f] = *reinterpret_cast<const uint4 *>(pa[f] + r0);
+#pragma unroll
+        for (int f = 0; f < kGramB; ++f) o.b[f] = *reinterpret_cast<const uint4 *>(pb[f] + r0);
     };
-    auto step = [&](const Ops4 &o) {
-        const v4i a0 = onehot<kPerm>(o.a0, la0, ha0), a1 = onehot<kPerm>(o.a1, la1, ha1);
-        const v4i b0 = onehot<kPerm>(o.b0, lb0, hb0), b1 = onehot<kPerm>(o.b1, lb1, hb1);
-        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-        c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
-        c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
-        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+    auto mma = [&](const Ops6 &o, const uint4 *mask) {
+        v4i b[kGramB];
+#pragma unroll
+        for (int e = 0; e < kGramB; ++e) b[e] = onehot<kPerm>(o.b[e], lb[e], hb[e]);
+#pragma unroll
+        for (int f = 0; f < kGramA; ++f) {
+            v4i a = onehot<kPerm>(o.a[f], la[f], ha[f]);
+            if (mask) a = and4(a, *mask);
+#pragma unroll
+            for (int e = 0; e < kGramB; ++e) c[f][e] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[e], c[f][e], 0, 0, 0);
+        }
     };
     if (r_begin < full_end) {
         // the scheduling barriers keep the next step's loads ahead of this step's
         // arithmetic (left alone, the compiler sinks them to their first use)
-        Ops4 o0, o1;
+        Ops6 o0, o1;
         load(r_begin, o0);
         int64_t r0 = r_begin;
         for (; r0 + 32 < full_end; r0 += 64) {
             load(r0 + 32, o1);
             __builtin_amdgcn_sched_barrier(0);
-            step(o0);
+            mma(o0, nullptr);
             __builtin_amdgcn_sched_barrier(0);
             if (r0 + 64 < full_end) load(r0 + 64, o0);
             __builtin_amdgcn_sched_barrier(0);
-            step(o1);
+            mma(o1, nullptr);
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (r0 < full_end) step(o0);
+        if (r0 < full_end) mma(o0, nullptr);
     }
     if (full_end < r_end) {
         // last partial step: bytes past r_end are masked out (their values are arbitrary)
@@ -155,27 +174,21 @@ __global__ __launch_bounds__(256) void k_gram_l0(GramArgs g)
         m.y = byte_mask_word(valid, 1);
         m.z = byte_mask_word(valid, 2);
         m.w = byte_mask_word(valid, 3);
-        Ops4 o;
+        Ops6 o;
         load(full_end, o);
-        const v4i a0 = and4(onehot<kPerm>(o.a0, la0, ha0), m), a1 = and4(onehot<kPerm>(o.a1, la1, ha1), m);
-        const v4i b0 = onehot<kPerm>(o.b0, lb0, hb0), b1 = onehot<kPerm>(o.b1, lb1, hb1);
-        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-        c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
-        c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
-        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+        mma(o, &m);
     }
-    // accumulate the split's partial tile into C
-    const int i_base = ti * kGramTile + (wave >> 1) * 64;
-    const int j_base = tj * kGramTile + (wave & 1) * 64;
+    // accumulate the split's partial block into C
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int row = (q >> 2) * 8 + h * 4 + (q & 3);
-        int32_t *c_r0 = g.C + (int64_t)(i_base + row) * g.Wp + j_base + l32;
-        int32_t *c_r1 = g.C + (int64_t)(i_base + 32 + row) * g.Wp + j_base + l32;
-        if (c00[q]) atomicAdd(c_r0, c00[q]);
-        if (c01[q]) atomicAdd(c_r0 + 32, c01[q]);
-        if (c10[q]) atomicAdd(c_r1, c10[q]);
-        if (c11[q]) atomicAdd(c_r1 + 32, c11[q]);
+#pragma unroll
+        for (int f = 0; f < kGramA; ++f) {
+            int32_t *cr = g.C + (int64_t)(i_base + 32 * f + row) * g.Wp + j_base + l32;
+#pragma unroll
+            for (int e = 0; e < kGramB; ++e)
+                if (c[f][e][q]) atomicAdd(cr + 32 * e, c[f][e][q]);
+        }
     }
 }
 
@@ -351,8 +364,8 @@ hipError_t launch_pack_l0(const DevResult *res, int64_t n, uint8_t *flags, PairS
 hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
-    if (g.max_card <= 8) hipLaunchKernelGGL(k_gram_l0<true>, dim3(n_items), dim3(256), 0, s, g);
-    else hipLaunchKernelGGL(k_gram_l0<false>, dim3(n_items), dim3(256), 0, s, g);
+    if (g.max_card <= 8) hipLaunchKernelGGL(k_gram_l0<true>, dim3(n_items), dim3(kGramWaves * 64), 0, s, g);
+    else hipLaunchKernelGGL(k_gram_l0<false>, dim3(n_items), dim3(kGramWaves * 64), 0, s, g);
     return hipGetLastError();
 }
 
