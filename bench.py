@@ -263,7 +263,7 @@ def summarize_trace(trace_dir, out_path):
         "the trace totals divided by that.",
         "  (every counting kernel reads the packed columns: 2, 4 or 8 bits per value by cardinality)",
         "  k_count_stat<2>  per-lane 16-bit LDS sink, tables <= 64 cells",
-        "  k_count_stat<3>  per-lane 16-bit LDS sink, tables <= 128 cells (64 KiB LDS)",
+        "  k_count_stat<3>  per-lane 16-bit LDS sink, tables <= 96 cells (48 KiB LDS)",
         "  k_count_stat<0>  shared LDS histogram, tables <= 8192 cells ([cell][32] columns up to 256)",
         "  k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
         "  k_count_stat<1>  device-memory sink (larger tables)",

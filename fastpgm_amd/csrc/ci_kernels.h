@@ -12,7 +12,7 @@ constexpr int kRowsPerStep = kWG * kRowsPerThread;  // 4096 rows per workgroup s
 constexpr int kLdsCells = 8192;           // LDS table capacity in int32 cells (32 KiB)
 constexpr int kLaneWords = kLdsCells / (4 * 64);    // 16-bit counter pairs per lane (4 waves)
 constexpr int kLaneCells = 2 * kLaneWords;          // largest table counted per lane (64)
-constexpr int kLaneWideCells = 2 * kLaneCells;      // per-lane sink with 64 KiB of counters (128)
+constexpr int kLaneWideCells = 96;  // per-lane sink with 48 KiB of counters: 3 workgroups per CU
 constexpr int kSmallCells = 512;          // LDS table + marginals of per-lane tests
 constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bit counters never wrap
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
@@ -117,10 +117,10 @@ constexpr int32_t kSkippedDf = -1;
 constexpr int kSinkShared = 0;  // LDS histogram shared by waves (copies per wave when small)
 constexpr int kSinkGlobal = 1;  // device-memory table, device atomics (beyond LDS)
 constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
-constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 64 KiB (<= 128 cells)
+constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 48 KiB (<= 96 cells)
 constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
-constexpr int kColCells = kLdsCells / 32;  // shared sink's [cell][32] column layout up to 256 cells
+constexpr int kColCells = kLdsCells / 32;  // shared sink's column layout up to 256 cells ([cell][32]; [cell][64] up to 128)
 constexpr int64_t kColMinRows = 65536;     // ... for work items of at least this many rows
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 

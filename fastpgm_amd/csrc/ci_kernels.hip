@@ -390,14 +390,14 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
     }
 }
 
-// Per-lane sinks (kSinkLane: tables <= 64 cells, kSinkLaneW: <= 128): a private
+// Per-lane sinks (kSinkLane: tables <= 64 cells, kSinkLaneW: <= 96): a private
 // histogram per lane in 16-bit LDS counters.  The two waves of a wave pair share each
 // 32-bit word, the even wave counting in the low half and the odd wave in the high half,
 // so a wave's increment is a constant (1 or 1 << 16).  Layout [pair][cell][lane]: the
 // 64 lanes of a wave always hit 64 distinct words in distinct banks, so neither bank
 // conflicts nor atomic collisions occur however skewed the data.
 //
-// Every key is below kCells <= 128, so the keys of 4 rows are formed at once in one
+// Every key is below kCells <= 96, so the keys of 4 rows are formed at once in one
 // 32-bit word, one byte per row: the Horner steps run as packed 16-bit multiply-adds
 // (v_pk_mad_u16) on the unpacked column words (one row per byte), and no byte ever
 // carries into the next because every partial key stays below 256.  A row's LDS byte
@@ -850,7 +850,10 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
 template <int kMode>
 struct LdsWords {
     static constexpr int value =
-        (kMode == kSinkGlobal) ? 1 : (kMode == kSinkLaneW || kMode == kSinkSlice) ? 2 * kLdsCells : kLdsCells;
+        (kMode == kSinkGlobal) ? 1
+        : (kMode == kSinkLaneW) ? 2 * kLaneWideCells * 64
+        : (kMode == kSinkSlice) ? 2 * kLdsCells
+                                : kLdsCells;
 };
 
 // Waves per SIMD the register allocation must allow: the LDS of the per-lane and shared
@@ -950,12 +953,12 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             __syncthreads();
             tab = small;
         } else if (kMode == kSinkShared) {
-            // Tables of at most kColCells cells: column layout, one table of [cell][32]
-            // words for the workgroup, lane l counting in column l & 31 (key << 5 from the
-            // lane's column), so the 32 lanes of a half-wave always hit 32 distinct banks,
-            // however skewed the data (waves meet only through the atomics, never within
-            // one instruction); then thread t sums cell t, reading its columns from
-            // (t + j) & 31 on: again one bank per lane.  Larger tables: as many histogram
+            // Tables of at most kColCells cells: column layout, one table of [cell][64]
+            // words (up to 128 cells) or [cell][32] for the workgroup, lane l counting in
+            // column l (l & 31), key << 6 (5) from the lane's column, so the lanes of a wave
+            // (of a half-wave) always hit distinct banks, however skewed the data (waves meet
+            // only through the atomics, never within one instruction); then thread t sums
+            // cell t, reading its columns from (t + j) on: again one bank per lane.  Larger tables: as many histogram
             // copies as fit, `sub` copies per wave, lane l counting into copy l % sub of
             // its wave, so skewed data collides on one address in at most 64 / sub lanes;
             // copies are offset by 4 banks from one another, so lanes of different copies
@@ -965,7 +968,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             // (the column layout clears and reduces twice the words of the copies: it is
             // used for items of enough rows to repay that)
             const bool col = c32 <= kColCells && r_end - r_begin >= kColMinRows;
-            int copies = 1, my = 0, stride = c32, span = c32 * 32;
+            const int ncol = (c32 <= kColCells / 2) ? 64 : 32;  // columns: every lane its own when they fit
+            int copies = 1, my = 0, stride = c32, span = c32 * ncol;
             if (!col) {
                 stride = ((c32 + 63) & ~63) + 4;
                 int sub = 1;
@@ -982,14 +986,14 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             for (int i = tid; i < span; i += kWG) lds[i] = 0;
             __syncthreads();
             count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end,
-                                             col ? lds + (lane & 31) : lds + (copies > 1 ? my * stride : 0),
-                                             col ? 5u : 0u);
+                                             col ? lds + (lane & (ncol - 1)) : lds + (copies > 1 ? my * stride : 0),
+                                             col ? (ncol == 64 ? 6u : 5u) : 0u);
             __syncthreads();
             if (col) {
                 int32_t sum = 0;
                 if (tid < c32)
 #pragma unroll 4
-                    for (int j = 0; j < 32; ++j) sum += lds[tid * 32 + ((tid + j) & 31)];
+                    for (int j = 0; j < ncol; ++j) sum += lds[tid * ncol + ((tid + j) & (ncol - 1))];
                 __syncthreads();
                 if (tid < c32) lds[tid] = sum;
                 __syncthreads();

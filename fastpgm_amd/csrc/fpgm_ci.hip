@@ -240,7 +240,7 @@ struct fpgm_ctx {
         int64_t promote = 0;  // dependent tests after which an edge runs the rest of its walk
         double round_bytes = 2147483648.0;  // algorithmic bytes a round aims at (tests = bytes / ((d+2) N))
         int64_t items_per_cu = 16;          // work items a launch aims at per CU (row chunks per test)
-        int64_t wpc[5] = {4, 2, 4, 2, 8};   // grid workgroups per CU: lane, lane-wide, shared, slice, device
+        int64_t wpc[5] = {4, 3, 4, 2, 8};   // grid workgroups per CU: lane, lane-wide, shared, slice, device
         int pipeline = 1;                    // build round r+1 while the device counts round r
         int64_t gram_items_per_cu = 8;       // level-0 Gram work items (tile pairs x row splits) per CU
         int concurrent = 1;                  // slice and device-memory sinks on streams of their own
