@@ -13,6 +13,7 @@ constexpr int kLdsCells = 8192;           // LDS table capacity in int32 cells (
 constexpr int kLaneWords = kLdsCells / (4 * 64);    // 16-bit counter pairs per lane (4 waves)
 constexpr int kLaneCells = 2 * kLaneWords;          // largest table counted per lane (64)
 constexpr int kLaneWideCells = 96;  // per-lane sink with 48 KiB of counters: 3 workgroups per CU
+constexpr int kLaneWide128Cells = 128;  // ... its 64 KiB instance (2 workgroups per CU)
 constexpr int kSmallCells = 512;          // LDS table + marginals of per-lane tests
 constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bit counters never wrap
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
@@ -119,6 +120,7 @@ constexpr int kSinkGlobal = 1;  // device-memory table, device atomics (beyond L
 constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
 constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 48 KiB (<= 96 cells)
 constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
+constexpr int kSinkLaneW128 = 5;  // kSinkLaneW's instance for tables <= 128 cells (a launch mode, not a group)
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
 constexpr int kColCells = kLdsCells / 32;  // shared sink's column layout up to 256 cells ([cell][32]; [cell][64] up to 128)
 constexpr int64_t kColMinRows = 65536;     // ... for work items of at least this many rows

@@ -390,14 +390,15 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
     }
 }
 
-// Per-lane sinks (kSinkLane: tables <= 64 cells, kSinkLaneW: <= 96): a private
+// Per-lane sinks (kSinkLane: tables <= 64 cells, kSinkLaneW: <= 96, or <= 128 in its
+// 64 KiB instance kSinkLaneW128): a private
 // histogram per lane in 16-bit LDS counters.  The two waves of a wave pair share each
 // 32-bit word, the even wave counting in the low half and the odd wave in the high half,
 // so a wave's increment is a constant (1 or 1 << 16).  Layout [pair][cell][lane]: the
 // 64 lanes of a wave always hit 64 distinct words in distinct banks, so neither bank
 // conflicts nor atomic collisions occur however skewed the data.
 //
-// Every key is below kCells <= 96, so the keys of 4 rows are formed at once in one
+// Every key is below kCells <= 128, so the keys of 4 rows are formed at once in one
 // 32-bit word, one byte per row: the Horner steps run as packed 16-bit multiply-adds
 // (v_pk_mad_u16) on the unpacked column words (one row per byte), and no byte ever
 // carries into the next because every partial key stays below 256.  A row's LDS byte
@@ -852,6 +853,7 @@ struct LdsWords {
     static constexpr int value =
         (kMode == kSinkGlobal) ? 1
         : (kMode == kSinkLaneW) ? 2 * kLaneWideCells * 64
+        : (kMode == kSinkLaneW128) ? 2 * kLaneWide128Cells * 64
         : (kMode == kSinkSlice) ? 2 * kLdsCells
                                 : kLdsCells;
 };
@@ -926,9 +928,11 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
         const int32_t *tab = nullptr;  // the chunk's table in LDS, when there is one
-        if (kMode == kSinkLane || kMode == kSinkLaneW) {
+        if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128) {
             // per-lane 16-bit counters: [pair][cell][lane], even wave low half, odd wave high
-            constexpr int kCells = (kMode == kSinkLaneW) ? kLaneWideCells : kLaneCells;
+            constexpr int kCells = (kMode == kSinkLaneW)      ? kLaneWideCells
+                                   : (kMode == kSinkLaneW128) ? kLaneWide128Cells
+                                                              : kLaneCells;
             const int c32 = (int)cells;
             uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
             for (int i = tid; i < 2 * kCells * 64; i += kWG) pl[i] = 0;
@@ -1268,6 +1272,9 @@ void launch_count_stat_form(const LaunchArgs &a, int mode, int grid, hipStream_t
     switch (mode) {
     case kSinkLane: hipLaunchKernelGGL((k_count_stat<kSinkLane, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
     case kSinkLaneW: hipLaunchKernelGGL((k_count_stat<kSinkLaneW, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkLaneW128:
+        hipLaunchKernelGGL((k_count_stat<kSinkLaneW128, kPacked>), dim3(grid), dim3(kWG), 0, s, a);
+        break;
     case kSinkSlice: hipLaunchKernelGGL((k_count_stat<kSinkSlice, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
     case kSinkShared: hipLaunchKernelGGL((k_count_stat<kSinkShared, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
     default: hipLaunchKernelGGL((k_count_stat<kSinkGlobal, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;

@@ -120,6 +120,7 @@ struct HostBuf {
 constexpr int kGroups = 5;
 constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
 constexpr int64_t kPackMinRows = 262144;  // mean rows per instance from which columns are packed
+constexpr int kGroupPending = kGroups;     // a 97..128-cell table whose group is decided per launch
 
 // Buffers of one in-flight batch.  Two slots let the host build one batch while the
 // device runs the other; the workspace and completion counters are shared, since both
@@ -378,9 +379,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     fit.resize((size_t)m);
     int too_big = 0;
     int64_t units[kGroups] = {0, 0, 0, 0, 0};  // per group: the tests' z slices (work items per chunk)
+    int64_t units_pending = 0;
     const int nth = (m >= kParallelTests) ? ctx->host_threads : 1;
 #pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static) reduction(| : too_big) \
-    reduction(+ : units[:kGroups])
+    reduction(+ : units[:kGroups]) reduction(+ : units_pending)
     for (int64_t k = 0; k < m; ++k) {
         const HostTest &h = tests[i0 + k];
         DevTest T{};
@@ -418,6 +420,9 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         } else if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
             gi = 1;  // kSinkLaneW
             fits = cells + marg <= kSmallCells;
+        } else if (cells <= kLaneWide128Cells && !(ctx->sink_mask & 1)) {
+            gi = kGroupPending;  // lane-wide (128-cell instance) or shared, decided below
+            fits = cells + marg <= kSmallCells;
         } else if (cells <= kLdsCells) {
             gi = 2;  // kSinkShared
             T.flags |= kTFLdsTable;
@@ -443,10 +448,36 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.ws_off = -1;
         fit[k] = fits ? 1 : 0;
         gidx[k] = gi;
-        units[gi] += T.n_slices;
+        if (gi != kGroupPending) units[gi] += T.n_slices;
+        else ++units_pending;
         dt[k] = T;
     }
     if (too_big) return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
+    // Tables of 97..128 cells: with shared-sink tests in the launch anyway they join that
+    // launch, and the lane-wide launch keeps its 48 KiB instance (3 workgroups per CU);
+    // otherwise they stay per lane in the 64 KiB instance rather than add a launch.
+    // Measured: 1000 variables, level 1 127 -> 123 ms with the first; 300 variables (tables
+    // of at most 125 cells), level 1 6.1 ms with the second against 7.2 ms with a launch more.
+    bool lane_w128 = false;
+    if (units_pending > 0) {
+        const bool to_shared = units[2] > 0;
+        lane_w128 = !to_shared;
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
+        for (int64_t k = 0; k < m; ++k) {
+            if (gidx[k] != kGroupPending) continue;
+            DevTest &T = dt[k];
+            if (to_shared) {
+                const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
+                const int64_t marg = (int64_t)T.dimz * (T.cx + T.cy + 1);
+                T.flags |= kTFLdsTable;
+                fit[k] = (cells + marg <= kLdsCells) ? 1 : 0;
+                gidx[k] = 2;
+            } else {
+                gidx[k] = 1;
+            }
+        }
+        units[lane_w128 ? 1 : 2] += units_pending;
+    }
     for (int g = 0; g < kGroups; ++g) {
         int64_t r;
         if (rows_per_chunk_forced > 0) {
@@ -632,7 +663,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         a.rows_per_chunk = rpc[gi];
         a.queue = S.d_queue.p + gi * kXcds;
         a.n_items_dev = a.edge_done ? live_count + gi : nullptr;
-        const int sink = kGroupSink[gi];
+        const int sink = (gi == 1 && lane_w128) ? kSinkLaneW128 : kGroupSink[gi];
         const int64_t wg_per_cu = ctx->sched.wpc[gi];
         int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wg_per_cu);
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);

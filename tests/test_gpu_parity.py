@@ -116,6 +116,27 @@ def test_device_columns_with_dirty_padding():
     _assert_same(g, o)
 
 
+def test_lane_wide_instances_and_routing():
+    # tables of 97..128 cells run per lane (64 KiB instance) when a launch has no shared-
+    # sink table, and join the shared sink otherwise; 65..96 cells always per lane (48 KiB)
+    s = datagen.random_sample(61, 12, 30_011, 4, 8)
+    s.card[:6] = [8, 4, 4, 2, 6, 8]
+    for v in range(6):
+        s.cols[v, :s.n_rows] = np.random.default_rng(100 + v).integers(0, s.card[v], s.n_rows)
+    D = O.Data.of(s)
+    tests_lw = [t for t in [(0, 1, (2,)), (1, 2, (0,)), (0, 2, (3, 1)), (1, 4, (2,)), (0, 3, (4,))]
+                if 64 < np.prod([s.card[v] for v in (t[0], t[1]) + t[2]]) <= 128]
+    tests_mixed = tests_lw + [(0, 5, (4,))]  # 384 cells: a shared-sink table in the launch
+    assert len(tests_lw) >= 3
+    with _ctx(s) as ctx:
+        for tests in (tests_lw, tests_mixed):
+            r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+            arr = np.array([[x, y, len(z)] + list(z) + [0] * (2 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+            o = O.ci_batch(D, arr, 2, 0.05, n_threads=8)
+            assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+            np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+
+
 def test_counts_large_tables_global_path():
     # card 8: d = 3 gives 8^3 * 64 = 32768 cells (beyond the LDS table)
     s = datagen.random_sample(5, 8, 200_003, 8, 8)
