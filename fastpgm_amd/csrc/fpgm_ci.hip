@@ -520,14 +520,24 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.item0 = (int32_t)grp_len[gidx[nb]];  // within the group; shifted by the group start below
         grp_len[gidx[nb]] += cnt;
         n_items_total += cnt;
-        S.bytes += (double)(T.d + 2) * (double)ctx->inst[T.inst].n_rows;
-        S.pbytes += packed_row_bytes(ctx, tests[i0 + nb], zpool) * (double)ctx->inst[T.inst].n_rows;
         if (n_items_total > ((int64_t)1 << 30)) {
             ++nb;
             break;
         }
     }
     const int64_t i1 = i0 + nb;
+    {  // algorithmic and packed bytes of the launch sequence's tests
+        double b = 0.0, pb = 0.0;
+        const int nth3 = (nb >= kParallelTests) ? ctx->host_threads : 1;
+#pragma omp parallel for if (nth3 > 1) num_threads(nth3) schedule(static) reduction(+ : b, pb)
+        for (int64_t k = 0; k < nb; ++k) {
+            const double rows = (double)ctx->inst[dt[k].inst].n_rows;
+            b += (double)(dt[k].d + 2) * rows;
+            pb += packed_row_bytes(ctx, tests[i0 + k], zpool) * rows;
+        }
+        S.bytes = b;
+        S.pbytes = pb;
+    }
     // work items grouped by sink, one launch per group; within a group, tests keep the
     // caller's order (neighbouring tests share columns)
     HIP_TRY(S.h_items.ensure(std::max<int64_t>(n_items_total, 1), ctx->stream), "alloc pinned items");
@@ -575,7 +585,14 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                 "upload finalize tests");
     }
     HIP_TRY(S.h_zpool.ensure(std::max<int64_t>(zlen, 1), ctx->stream), "alloc pinned z pool");
-    if (zlen > 0) std::copy(zpool, zpool + zlen, S.h_zpool.p);
+    if (zlen > 0) {
+        const int nthz = (zlen >= 4 * kParallelTests) ? ctx->host_threads : 1;
+#pragma omp parallel for if (nthz > 1) num_threads(nthz) schedule(static)
+        for (int ch = 0; ch < nthz; ++ch) {
+            const int64_t a = zlen * ch / nthz, b = zlen * (ch + 1) / nthz;
+            std::copy(zpool + a, zpool + b, S.h_zpool.p + a);
+        }
+    }
     HIP_TRY(S.d_zpool.ensure(std::max<int64_t>(zlen, 1), false, ctx->stream), "alloc z pool");
     HIP_TRY(S.d_tests.ensure(nb, false, ctx->stream), "alloc tests");
     HIP_TRY(S.d_items.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
@@ -1043,7 +1060,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         return fail(ctx, FPGM_EDEVICE, std::string("device is not gfx950: ") + prop.gcnArchName);
     ctx->dev = dev;
     ctx->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    ctx->host_threads = std::max(1, std::min(8, omp_get_max_threads()));
+    ctx->host_threads = std::max(1, std::min(16, omp_get_max_threads()));
     if (const char *e = std::getenv("FPGM_ROUND_BASE")) ctx->sched.base = std::max(1.0, std::atof(e));
     if (const char *e = std::getenv("FPGM_ROUND_CAP")) ctx->sched.cap = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_ROUND_PROMOTE")) ctx->sched.promote = std::max<int64_t>(0, std::atoll(e));
@@ -1607,17 +1624,25 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
                 if (w.side < 2) G.still.push_back(g);
             }
         }
-        R.tv.clear();
-        R.zpool.clear();
+        // concatenated in chunk order (each chunk copied by its own thread)
+        std::vector<int64_t> t_off((size_t)nth + 1, 0), z_off((size_t)nth + 1, 0);
         live.clear();
         for (int ch = 0; ch < nth; ++ch) {
+            t_off[ch + 1] = t_off[ch] + (int64_t)gen[ch].tv.size();
+            z_off[ch + 1] = z_off[ch] + (int64_t)gen[ch].z.size();
+            live.insert(live.end(), gen[ch].still.begin(), gen[ch].still.end());
+        }
+        R.tv.resize((size_t)t_off[nth]);
+        R.zpool.resize((size_t)z_off[nth]);
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
+        for (int ch = 0; ch < nth; ++ch) {
             const Gen &G = gen[ch];
-            const int64_t zb = (int64_t)R.zpool.size();
-            R.zpool.insert(R.zpool.end(), G.z.begin(), G.z.end());
-            const size_t t0 = R.tv.size();
-            R.tv.insert(R.tv.end(), G.tv.begin(), G.tv.end());
-            for (size_t k = t0; k < R.tv.size(); ++k) R.tv[k].z_off += zb;
-            live.insert(live.end(), G.still.begin(), G.still.end());
+            std::copy(G.z.begin(), G.z.end(), R.zpool.begin() + z_off[ch]);
+            HostTest *dst = R.tv.data() + t_off[ch];
+            for (size_t k = 0; k < G.tv.size(); ++k) {
+                dst[k] = G.tv[k];
+                dst[k].z_off += z_off[ch];
+            }
         }
         R.pending = true;
         R.async = false;
