@@ -57,17 +57,23 @@ def cpu_baseline(sample, w, g, seconds, threads):
     adjs = adjacency_before_levels(sample.n_vars, g["sepsets"], L)
     per = max(0.5, seconds / max(1, sum(1 for t in g["tests_per_level"] if t > 0)))
     est, sampled, sampled_s = 0.0, 0, 0.0
-    for lvl in range(L):
+    # the level with the most tests gets what the others leave of the budget
+    main = int(np.argmax(g["tests_per_level"]))
+    order = [l for l in range(L) if l != main] + [main]
+    for lvl in order:
         tests_l = g["tests_per_level"][lvl]
         if tests_l == 0:
             continue
         n_edges = int(adjs[lvl][np.triu_indices(sample.n_vars, 1)].sum())
         m = min(n_edges, 16)
+        budget = per if lvl != main else max(per, seconds - sampled_s)
         t0 = time.perf_counter()
         k, _ = O.level_sample(D, w.alpha, lvl, adjs[lvl], m, threads)
         dt = time.perf_counter() - t0
-        if dt < per and m < n_edges:
-            m = min(n_edges, max(m + 1, int(m * per / max(dt, 1e-4))))
+        for _ in range(3):  # the first edges need not be typical: rescale until the budget is used
+            if dt >= 0.5 * budget or m >= n_edges:
+                break
+            m = min(n_edges, max(m + 1, int(m * budget / max(dt, 1e-4))))
             t0 = time.perf_counter()
             k, _ = O.level_sample(D, w.alpha, lvl, adjs[lvl], m, threads)
             dt = time.perf_counter() - t0

@@ -1696,45 +1696,81 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
             R.r = ctx->slot[R.slot].h_res.p;
         }
         const DevResult *r = R.r;
+        // edges are independent: large rounds are taken in chunks of whole edges on several
+        // host threads; the sepset appends (per-instance vectors) follow in test order
+        const size_t ntv = R.tv.size();
+        const int nthp = (ntv >= (size_t)kParallelTests) ? ctx->host_threads : 1;
+        std::vector<size_t> cb((size_t)nthp + 1, ntv);
+        cb[0] = 0;
+        for (int c = 1; c < nthp; ++c) {
+            size_t b = std::max(cb[c - 1], ntv * (size_t)c / (size_t)nthp);
+            while (b > 0 && b < ntv && R.tv[b].edge == R.tv[b - 1].edge) ++b;
+            cb[c] = b;
+        }
+        struct Part {
+            double skipped_bytes = 0.0, skipped_pbytes = 0.0, max_err = 0.0;
+            int64_t host_skipped = 0, dev_skipped = 0, executed = 0, in_band = 0;
+            bool any_done = false, bad = false;
+            std::vector<std::pair<int32_t, size_t>> sep;  // (edge, test) of each edge's first independent test
+        };
+        std::vector<Part> parts((size_t)nthp);
+#pragma omp parallel for if (nthp > 1) num_threads(nthp) schedule(static)
+        for (int c = 0; c < nthp; ++c) {
+            Part &Q = parts[c];
+            for (size_t k = cb[c]; k < cb[c + 1];) {
+                const int32_t g = R.tv[k].edge;
+                size_t k1 = k;
+                while (k1 < cb[c + 1] && R.tv[k1].edge == g) ++k1;
+                PcInst &I = P[active[g].first];
+                EdgeWalk &w = I.walks[active[g].second];
+                if (done_round[g] < R.id) {
+                    const double rows = (double)ctx->inst[active[g].first].n_rows;
+                    Q.skipped_bytes += (double)(k1 - k) * (d + 2) * rows;
+                    for (size_t j = k; j < k1; ++j) {
+                        Q.skipped_pbytes += packed_row_bytes(ctx, R.tv[j], R.zpool.data()) * rows;
+                        Q.dev_skipped += (r[j].df == kSkippedDf);
+                    }
+                    Q.host_skipped += (int64_t)(k1 - k);
+                    k = k1;
+                    continue;
+                }
+                bool done = false;
+                for (; k < k1; ++k) {
+                    if (r[k].df == kSkippedDf) Q.bad = true;
+                    ++Q.executed;
+                    Q.in_band += r[k].in_band;
+                    if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
+                        Q.max_err = std::max(Q.max_err, std::fabs((double)r[k].g2_32 - r[k].g2_64) / r[k].g2_64);
+                    if (done) continue;
+                    w.tests += 1;
+                    if (r[k].indep) {
+                        done = true;
+                        Q.any_done = true;
+                        done_round[g] = R.id;
+                        w.removed = 1;
+                        Q.sep.emplace_back(g, k);
+                    }
+                }
+            }
+        }
         double skipped_bytes = 0.0, skipped_pbytes = 0.0;
         int64_t host_skipped = 0, dev_skipped = 0;
         bool any_done = false;
-        for (size_t k = 0; k < R.tv.size();) {
-            const int32_t g = R.tv[k].edge;
-            size_t k1 = k;
-            while (k1 < R.tv.size() && R.tv[k1].edge == g) ++k1;
-            PcInst &I = P[active[g].first];
-            EdgeWalk &w = I.walks[active[g].second];
-            if (done_round[g] < R.id) {
-                skipped_bytes += (double)(k1 - k) * (d + 2) * (double)ctx->inst[active[g].first].n_rows;
-                for (size_t j = k; j < k1; ++j)
-                    skipped_pbytes +=
-                        packed_row_bytes(ctx, R.tv[j], R.zpool.data()) * (double)ctx->inst[active[g].first].n_rows;
-                if (ctx->trace)
-                    for (size_t j = k; j < k1; ++j) dev_skipped += (r[j].df == kSkippedDf);
-                host_skipped += (int64_t)(k1 - k);
-                k = k1;
-                continue;
-            }
-            bool done = false;
-            for (; k < k1; ++k) {
-                if (r[k].df == kSkippedDf)
-                    return fail(ctx, FPGM_EDEVICE, "internal: the device skipped a test of a live edge");
-                ++executed;
-                inb += r[k].in_band;
-                if (r[k].exact && r[k].df > 0 && r[k].g2_64 > 0.0)
-                    S.max_rel_err_g2_fp32 =
-                        std::max(S.max_rel_err_g2_fp32, std::fabs((double)r[k].g2_32 - r[k].g2_64) / r[k].g2_64);
-                if (done) continue;
-                w.tests += 1;
-                if (r[k].indep) {
-                    done = true;
-                    any_done = true;
-                    done_round[g] = R.id;
-                    w.removed = 1;
-                    w.zsep = (int64_t)I.zlevel.size();
-                    for (int i = 0; i < d; ++i) I.zlevel.push_back(R.zpool[R.tv[k].z_off + i]);
-                }
+        for (const Part &Q : parts) {
+            if (Q.bad) return fail(ctx, FPGM_EDEVICE, "internal: the device skipped a test of a live edge");
+            skipped_bytes += Q.skipped_bytes;
+            skipped_pbytes += Q.skipped_pbytes;
+            host_skipped += Q.host_skipped;
+            dev_skipped += Q.dev_skipped;
+            executed += Q.executed;
+            inb += Q.in_band;
+            S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, Q.max_err);
+            any_done = any_done || Q.any_done;
+            for (const auto &gk : Q.sep) {
+                PcInst &I = P[active[gk.first].first];
+                EdgeWalk &w = I.walks[active[gk.first].second];
+                w.zsep = (int64_t)I.zlevel.size();
+                for (int i = 0; i < d; ++i) I.zlevel.push_back(R.zpool[R.tv[gk.second].z_off + i]);
             }
         }
         ctx->k_bytes -= skipped_bytes;  // batch_wait counted every submitted test
