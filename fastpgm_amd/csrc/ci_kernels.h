@@ -18,6 +18,7 @@ constexpr int kSmallCells = 512;          // LDS table + marginals of per-lane t
 constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bit counters never wrap
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
 constexpr int kCritTab = 4096;            // critical values tabulated for df = 1..kCritTab
+constexpr int kCritLds = 128;             // ... of which the counting kernel keeps df <= kCritLds in LDS
 
 // One dataset instance: uint8 columns, column v at cols + v*stride, n_rows valid bytes.
 // The counting kernels read the packed copy instead: column v at pcols + v*stride, its

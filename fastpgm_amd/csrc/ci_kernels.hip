@@ -183,6 +183,8 @@ __device__ __forceinline__ void wave_nsum(float &s, float &c)
     }
 }
 
+static_assert(kMaxZLds <= kWG, "one staging thread per Z column");
+
 struct Shared {
     int32_t zcol[kMaxZLds];
     int32_t zcard[kMaxZLds];
@@ -191,10 +193,12 @@ struct Shared {
     int32_t red_i[kWG / 64];
     float red_f[3 * (kWG / 64)];
     double red_d[kWG / 64];
+    double crit[kCritLds];  // critical values for df = 1..kCritLds (filled by the counting kernel)
     int32_t flag;
     int32_t slice_lo, slice_len;  // key range of this work item's slice (kSinkSlice)
     int32_t df;
     int32_t need64;
+    int32_t indep;  // the FP32 decision (finalize)
     float g2_32;
     float abs_sum;
 };
@@ -599,8 +603,10 @@ __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, c
 // Critical value of the chi-square test at the call's alpha: tabulated (exact FP64
 // inversion on the host) for df <= kCritTab, Wilson-Hilferty beyond, whose relative
 // error there is below 1e-6 (far inside the recheck band).
-__device__ __forceinline__ double crit_of(const LaunchArgs &a, int df)
+// crit_lds: the first kCritLds entries of the table in LDS (the counting kernel's copy), or nullptr.
+__device__ __forceinline__ double crit_of(const LaunchArgs &a, int df, const double *crit_lds)
 {
+    if (crit_lds && df <= kCritLds) return crit_lds[df - 1];
     if (df <= kCritTab) return a.crit[df - 1];
     const double k = (double)df;
     const double q = 2.0 / (9.0 * k);
@@ -656,7 +662,8 @@ __device__ __forceinline__ void zslice32(const int32_t *tz, int cx, int cy, int3
 // independent; otherwise independent iff G^2 < chi-square critical value, unless G^2 lies
 // within the recheck band of it (or the caller asked for exact results), in which case
 // the FP64 statistic decides.  Writes the result record; returns 1 when FP64 is needed.
-__device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, float AS, DevResult &R)
+__device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, float AS, DevResult &R,
+                        const double *crit_lds = nullptr)
 {
     int need64 = 0;
     uint8_t indep = 0, in_band = 0;
@@ -665,7 +672,7 @@ __device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, f
         indep = 1;  // IndependenceTest.cpp:140-142
         need64 = exact;
     } else {
-        const double crit = crit_of(a, df);
+        const double crit = crit_of(a, df, crit_lds);
         // FP32 error bound: a few ulp of every term, summed, plus the critical-value
         // approximation beyond the table
         const double bound = 4e-6 * (double)AS + (df > kCritTab ? 1e-5 * crit : 0.0) + 1e-30;
@@ -695,9 +702,9 @@ __device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, f
 
 // A test decided independent marks its edge with the launch's round: later rounds skip
 // the edge's tests, tests of the same round never skip one another.
-__device__ __forceinline__ void mark_edge(const LaunchArgs &a, const DevTest &T, const DevResult &R)
+__device__ __forceinline__ void mark_edge(const LaunchArgs &a, const DevTest &T, bool indep, bool need_p)
 {
-    if (a.edge_done && T.edge >= 0 && R.indep && !R.need_p) atomicMin(&a.edge_done[T.edge], a.round);
+    if (a.edge_done && T.edge >= 0 && indep && !need_p) atomicMin(&a.edge_done[T.edge], a.round);
 }
 
 __device__ __forceinline__ bool skipped(const LaunchArgs &a, const DevTest &T)
@@ -802,7 +809,10 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
             AS += sh.red_f[3 * w + 2];
         }
         sh.df = df;
-        sh.need64 = decide32(a, T.flags, df, S + Cc, AS, a.res[t]);
+        DevResult R;
+        sh.need64 = decide32(a, T.flags, df, S + Cc, AS, R, sh.crit);
+        a.res[t] = R;
+        sh.indep = R.indep;
     }
     __syncthreads();
     if (sh.need64) {
@@ -844,7 +854,7 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
             R.need_p = (sh.df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
         }
     }
-    if (tid == 0) mark_edge(a, T, a.res[t]);
+    if (tid == 0) mark_edge(a, T, sh.indep != 0, sh.need64 && sh.df != 0);
     __syncthreads();
 }
 
@@ -877,55 +887,79 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
     // atomic counter, then help the other slabs.  Item costs differ (table sinks, |Z|,
     // tests skipped because their edge is done), so a static split leaves the launch
     // waiting for its most loaded workgroup.
+    //
+    // Thread kQ (last wave; the Z staging below uses the first) keeps the queue one item
+    // ahead of the workgroup, as a single reservation: at the top of each item it
+    // publishes the work-item word it loaded during the previous item and draws the ticket
+    // of the next item; it resolves that ticket once the item's rows are counted (the
+    // atomic's latency hidden behind the counting) and starts loading its word, which the
+    // statistic's time then hides.  Neither the queue atomic nor the work-item load sits
+    // between one item and the next.
+    constexpr int kQ = kWG - 1;
+    constexpr uint64_t kNoItem = ~0ull;
     const int home = blockIdx.x % kXcds;
     const int64_t n_items = a.n_items_dev ? (int64_t)*a.n_items_dev : a.n_items;
     const int64_t slab = (n_items + kXcds - 1) / kXcds;
-    __shared__ int64_t s_item[2];
-    int tried = 0;  // slabs exhausted so far (thread 0's state)
-    auto fetch = [&]() -> int64_t {
-        while (tried < kXcds) {
-            const int x = (home + tried) % kXcds;
-            const int64_t lo = (int64_t)x * slab;
-            const int64_t hi = (lo + slab < n_items) ? lo + slab : n_items;
-            if (lo < hi) {
-                const int64_t i = lo + (int64_t)atomicAdd(&a.queue[x], 1u);
-                if (i < hi) return a.item_begin + i;
-            }
-            ++tried;
-        }
-        return -1;
+    __shared__ uint64_t s_pk[2];
+    int tried = 0;         // slabs exhausted so far (kQ's state)
+    uint32_t ticket = 0;   // kQ's ticket on slab (home + tried) % kXcds
+    uint64_t pk_next = kNoItem;
+    auto draw = [&]() {
+        if (tried < kXcds) ticket = atomicAdd(&a.queue[(home + tried) % kXcds], 1u);
     };
-    if (tid == 0) s_item[0] = fetch();
+    // the work-item word of `ticket`, drawing on the next slabs while the slab is exhausted
+    // (every draw consumes a ticket or exhausts a slab, so the queue ends for every workgroup)
+    auto resolve = [&]() -> uint64_t {
+        while (tried < kXcds) {
+            const int64_t lo = (int64_t)((home + tried) % kXcds) * slab;
+            const int64_t hi = (lo + slab < n_items) ? lo + slab : n_items;
+            if (lo + (int64_t)ticket < hi) return a.items[a.item_begin + lo + ticket];
+            ++tried;
+            draw();
+        }
+        return kNoItem;
+    };
+    if (tid == kQ) {
+        draw();
+        pk_next = resolve();
+    }
+    for (int i = tid; i < kCritLds; i += kWG) sh.crit[i] = a.crit[i];
     for (int it = 0;; ++it) {
-        __syncthreads();  // s_item[it & 1] written, s_item[(it + 1) & 1] read by everyone
-        const int64_t item = s_item[it & 1];
-        if (item < 0) break;
-        if (tid == 0) s_item[(it + 1) & 1] = fetch();
-        const uint64_t pk = a.items[item];
+        if (tid == kQ) {
+            s_pk[it & 1] = pk_next;
+            if (pk_next != kNoItem) draw();  // the next item's ticket
+        }
+        __syncthreads();  // s_pk[it & 1] written; s_pk[(it + 1) & 1] read by everyone
+        const uint64_t pk = s_pk[it & 1];
+        if (pk == kNoItem) break;
         const int t = (int)(pk >> 32);
         const int64_t local = (int64_t)(uint32_t)pk;  // slice * n_chunks + chunk
         const DevTest T = a.tests[t];
-        if (skipped(a, T)) {  // uniform over the workgroup: the flag cannot drop below round meanwhile
-            if (tid == 0 && local == 0) a.res[t] = skipped_result();
-            continue;
-        }
         const DevInst I = a.inst[T.inst];
-        const int64_t slice = (kMode == kSinkSlice) ? local / T.n_chunks : 0;
-        const int64_t chunk = local - slice * T.n_chunks;
-        const int64_t r_begin = chunk * a.rows_per_chunk;
-        const int64_t r_end = (r_begin + a.rows_per_chunk < I.n_rows) ? r_begin + a.rows_per_chunk : I.n_rows;
-        const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
-        for (int j = tid; j < T.d; j += kWG) {
-            const int32_t zc = a.zpool[T.z_off + j];
-            sh.zcol[j] = zc;
-            sh.zcard[j] = a.card[zc];
-            sh.zw[j] = a.colw[zc];
+        // the Z columns are staged beside the edge flag's load, not after it (a skipped
+        // test leaves them unused); T.d <= kMaxZLds <= kWG
+        if (tid < T.d) {
+            const int32_t zc = a.zpool[T.z_off + tid];
+            sh.zcol[tid] = zc;
+            sh.zcard[tid] = a.card[zc];
+            sh.zw[tid] = a.colw[zc];
         }
         if (tid == 0) {
             sh.wx = a.colw[T.x];
             sh.wy = a.colw[T.y];
         }
+        const bool skip = skipped(a, T);  // uniform: the flag cannot drop below round meanwhile
         __syncthreads();
+        if (skip) {
+            if (tid == 0 && local == 0) a.res[t] = skipped_result();
+            if (tid == kQ) pk_next = resolve();  // published at the top of the next item
+            continue;
+        }
+        const int64_t slice = (kMode == kSinkSlice) ? local / T.n_chunks : 0;
+        const int64_t chunk = local - slice * T.n_chunks;
+        const int64_t r_begin = chunk * a.rows_per_chunk;
+        const int64_t r_end = (r_begin + a.rows_per_chunk < I.n_rows) ? r_begin + a.rows_per_chunk : I.n_rows;
+        const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
         const int32_t *tab = nullptr;  // the chunk's table in LDS, when there is one
         if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128) {
@@ -1028,6 +1062,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         } else {
             count_rows<kSinkGlobal, kPacked>(I, T, sh, r_begin, r_end, g);
         }
+        if (tid == kQ) pk_next = resolve();  // published at the top of the next item
         if (tab) {
             if (T.flags & kTFLdsMarg) {
                 finalize(a, T, t, tab, const_cast<int32_t *>(tab) + cells, sh);
@@ -1194,7 +1229,7 @@ __global__ __launch_bounds__(kWG) void k_fin_wide_reduce(LaunchArgs a, const Wid
         R.need_p = (df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
     }
     a.res[W.t] = R;
-    mark_edge(a, T, R);
+    mark_edge(a, T, R.indep, R.need_p);
 }
 
 __global__ __launch_bounds__(kWG) void k_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows,
