@@ -19,6 +19,7 @@ constexpr int64_t kLaneMaxRowsPerChunk = (int64_t)kRowsPerStep * 4000;  // 16-bi
 constexpr int kMaxZLds = 64;              // Z columns staged in LDS per test
 constexpr int kCritTab = 4096;            // critical values tabulated for df = 1..kCritTab
 constexpr int kCritLds = 128;             // ... of which the counting kernel keeps df <= kCritLds in LDS
+constexpr int kCritLdsShared = 640;       // ... (the shared sink's tables, up to kLdsCells cells, have larger df)
 
 // One dataset instance: uint8 columns, column v at cols + v*stride, n_rows valid bytes.
 // The counting kernels read the packed copy instead: column v at pcols + v*stride, its

@@ -193,7 +193,6 @@ struct Shared {
     int32_t red_i[kWG / 64];
     float red_f[3 * (kWG / 64)];
     double red_d[kWG / 64];
-    double crit[kCritLds];  // critical values for df = 1..kCritLds (filled by the counting kernel)
     int32_t flag;
     int32_t slice_lo, slice_len;  // key range of this work item's slice (kSinkSlice)
     int32_t df;
@@ -603,10 +602,10 @@ __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, c
 // Critical value of the chi-square test at the call's alpha: tabulated (exact FP64
 // inversion on the host) for df <= kCritTab, Wilson-Hilferty beyond, whose relative
 // error there is below 1e-6 (far inside the recheck band).
-// crit_lds: the first kCritLds entries of the table in LDS (the counting kernel's copy), or nullptr.
-__device__ __forceinline__ double crit_of(const LaunchArgs &a, int df, const double *crit_lds)
+// crit_lds: the first n_lds entries of the table in LDS (the counting kernel's copy).
+__device__ __forceinline__ double crit_of(const LaunchArgs &a, int df, const double *crit_lds, int n_lds)
 {
-    if (crit_lds && df <= kCritLds) return crit_lds[df - 1];
+    if (df <= n_lds) return crit_lds[df - 1];
     if (df <= kCritTab) return a.crit[df - 1];
     const double k = (double)df;
     const double q = 2.0 / (9.0 * k);
@@ -663,7 +662,7 @@ __device__ __forceinline__ void zslice32(const int32_t *tz, int cx, int cy, int3
 // within the recheck band of it (or the caller asked for exact results), in which case
 // the FP64 statistic decides.  Writes the result record; returns 1 when FP64 is needed.
 __device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, float AS, DevResult &R,
-                        const double *crit_lds = nullptr)
+                        const double *crit_lds = nullptr, int n_lds = 0)
 {
     int need64 = 0;
     uint8_t indep = 0, in_band = 0;
@@ -672,7 +671,7 @@ __device__ int decide32(const LaunchArgs &a, int32_t tflags, int df, float g2, f
         indep = 1;  // IndependenceTest.cpp:140-142
         need64 = exact;
     } else {
-        const double crit = crit_of(a, df, crit_lds);
+        const double crit = crit_of(a, df, crit_lds, n_lds);
         // FP32 error bound: a few ulp of every term, summed, plus the critical-value
         // approximation beyond the table
         const double bound = 4e-6 * (double)AS + (df > kCritTab ? 1e-5 * crit : 0.0) + 1e-30;
@@ -727,11 +726,24 @@ __device__ __forceinline__ DevResult skipped_result()
     return R;
 }
 
+// n / d for 0 <= n < 2^22 and 1 <= d < 2^22, rd = 1/d in FP32: the FP32 product is within
+// n * 2^-23 < 0.5 of n / d, so its truncation is off by at most one, which the
+// remainder's sign corrects.  (The small-table statistic divides cell and row indices
+// below 63 * 65536 by cardinalities.)
+__device__ __forceinline__ int32_t small_div(int32_t n, float rd, int32_t d)
+{
+    int32_t q = (int32_t)((float)n * rd);
+    const int32_t r = n - (int32_t)__umul24((uint32_t)q, (uint32_t)d);
+    q += (r >= d) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return q;
+}
+
 // Statistic of test t from its full table (LDS or global) with marginal scratch
 // marg = [ni: dimz*cx | nj: dimz*cy | nk: dimz].  Tables with many z slices are
 // processed one slice per thread in a single pass; small ones cell-parallel.
 __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int32_t *tab,
-                         int32_t *marg, Shared &sh)
+                         int32_t *marg, Shared &sh, const double *crit_lds, int n_crit)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cx = T.cx, cy = T.cy;
@@ -747,46 +759,48 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
         for (int32_t z = tid; z < dimz; z += kWG)
             zslice32(tab + (int64_t)z * cxy, cx, cy, ni + (int64_t)z * cx, nj + (int64_t)z * cy, df_loc, s, c, as);
     } else {
-        // marginals n_{x+z}, n_{+yz}
-        for (int32_t idx = tid; idx < dimz * cx; idx += kWG) {
-            const int32_t z = idx / cx, i = idx - z * cx;
-            const int32_t *row = tab + z * cxy + i * cy;
+        // one pass for the marginals: rows (z, i) sum their cy cells into n_{x+z}, columns
+        // (z, j) their cx cells into n_{+yz}, slices z their cx*cy cells into n_{++z}
+        const int32_t nr = dimz * cx, nc = dimz * cy;
+        const float rcx = 1.0f / (float)cx, rcy = 1.0f / (float)cy;
+        for (int32_t idx = tid; idx < nr + nc + dimz; idx += kWG) {
             int32_t v = 0;
-            for (int j = 0; j < cy; ++j) v += row[j];
-            ni[idx] = v;
-        }
-        for (int32_t idx = tid; idx < dimz * cy; idx += kWG) {
-            const int32_t z = idx / cy, j = idx - z * cy;
-            const int32_t *col = tab + z * cxy + j;
-            int32_t v = 0;
-            for (int i = 0; i < cx; ++i) v += col[i * cy];
-            nj[idx] = v;
+            if (idx < nr) {
+                const int32_t *row = tab + (int64_t)idx * cy;
+                for (int j = 0; j < cy; ++j) v += row[j];
+                ni[idx] = v;
+            } else if (idx < nr + nc) {
+                const int32_t k = idx - nr, z = small_div(k, rcy, cy), j = k - z * cy;
+                const int32_t *col = tab + (int64_t)z * cxy + j;
+                for (int i = 0; i < cx; ++i) v += col[i * cy];
+                nj[k] = v;
+            } else {
+                const int32_t z = idx - nr - nc;
+                const int32_t *tz = tab + (int64_t)z * cxy;
+                for (int k = 0; k < cxy; ++k) v += tz[k];
+                nk[z] = v;
+            }
         }
         __syncthreads();
-        // n_{++z} and the adjusted degrees of freedom
-        for (int32_t z = tid; z < dimz; z += kWG) {
-            int32_t v = 0;
+        // terms cell-parallel (row r = z * cx + i of cell r * cy + j)
+        for (int32_t cell = tid; cell < cells; cell += kWG) {
+            const int32_t n = tab[cell];
+            if (n == 0) continue;
+            const int32_t r = small_div(cell, rcy, cy), j = cell - r * cy;
+            const int32_t z = small_div(r, rcx, cx);
+            const float tm = term32(n, ni[r], nj[z * cy + j], nk[z]);
+            nadd(s, c, tm);
+            as += fabsf(tm);
+        }
+        // the adjusted degrees of freedom, one slice per thread from the last thread down
+        // (those count one cell fewer when the cells do not fill the last pass)
+        for (int32_t z = kWG - 1 - tid; z < dimz; z += kWG) {
             int alx = 0, aly = 0;
-            for (int i = 0; i < cx; ++i) {
-                const int32_t u = ni[z * cx + i];
-                v += u;
-                alx += (u > 0);
-            }
+            for (int i = 0; i < cx; ++i) alx += (ni[z * cx + i] > 0);
             for (int j = 0; j < cy; ++j) aly += (nj[z * cy + j] > 0);
             alx = alx >= 1 ? alx : 1;
             aly = aly >= 1 ? aly : 1;
             df_loc += (alx - 1) * (aly - 1);
-            nk[z] = v;
-        }
-        __syncthreads();
-        for (int32_t cell = tid; cell < cells; cell += kWG) {
-            const int32_t n = tab[cell];
-            if (n == 0) continue;
-            const int32_t z = cell / cxy, rem = cell - z * cxy;
-            const int32_t i = rem / cy, j = rem - i * cy;
-            const float tm = term32(n, ni[z * cx + i], nj[z * cy + j], nk[z]);
-            nadd(s, c, tm);
-            as += fabsf(tm);
         }
     }
     df_loc = wave_sum_i(df_loc);
@@ -810,7 +824,7 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
         }
         sh.df = df;
         DevResult R;
-        sh.need64 = decide32(a, T.flags, df, S + Cc, AS, R, sh.crit);
+        sh.need64 = decide32(a, T.flags, df, S + Cc, AS, R, crit_lds, n_crit);
         a.res[t] = R;
         sh.indep = R.indep;
     }
@@ -859,6 +873,11 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
 }
 
 template <int kMode>
+struct CritLds {
+    static constexpr int value = (kMode == kSinkShared) ? kCritLdsShared : kCritLds;
+};
+
+template <int kMode>
 struct LdsWords {
     static constexpr int value =
         (kMode == kSinkGlobal) ? 1
@@ -879,6 +898,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
     __shared__ int32_t small[kSmallCells];
 
     __shared__ Shared sh;
+    __shared__ double s_crit[CritLds<kMode>::value];  // critical values for df = 1..CritLds
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
     // XCD-aware dynamic order: workgroups are dispatched round-robin over the XCDs, so the
@@ -923,7 +943,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         draw();
         pk_next = resolve();
     }
-    for (int i = tid; i < kCritLds; i += kWG) sh.crit[i] = a.crit[i];
+    for (int i = tid; i < CritLds<kMode>::value; i += kWG) s_crit[i] = a.crit[i];
     for (int it = 0;; ++it) {
         if (tid == kQ) {
             s_pk[it & 1] = pk_next;
@@ -1065,7 +1085,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         if (tid == kQ) pk_next = resolve();  // published at the top of the next item
         if (tab) {
             if (T.flags & kTFLdsMarg) {
-                finalize(a, T, t, tab, const_cast<int32_t *>(tab) + cells, sh);
+                finalize(a, T, t, tab, const_cast<int32_t *>(tab) + cells, sh, s_crit, CritLds<kMode>::value);
                 continue;
             }
             for (int i = tid; i < (int)cells; i += kWG) {
@@ -1087,7 +1107,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         if (sh.flag) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            finalize(a, T, t, g, g + cells, sh);
+            finalize(a, T, t, g, g + cells, sh, s_crit, CritLds<kMode>::value);
             if (!(T.flags & kTFKeep)) {
                 // the workspace must read zero wherever the next launch places a table,
                 // so the marginal scratch behind the table is cleared as well
