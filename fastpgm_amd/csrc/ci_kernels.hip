@@ -1016,12 +1016,13 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             // column l (l & 31), key << 6 (5) from the lane's column, so the lanes of a wave
             // (of a half-wave) always hit distinct banks, however skewed the data (waves meet
             // only through the atomics, never within one instruction); then thread t sums
-            // cell t, reading its columns from (t + j) on: again one bank per lane.  Larger tables: as many histogram
-            // copies as fit, `sub` copies per wave, lane l counting into copy l % sub of
-            // its wave, so skewed data collides on one address in at most 64 / sub lanes;
-            // copies are offset by 4 banks from one another, so lanes of different copies
-            // hitting the same cell use different banks; tables too large for one copy per
-            // wave share copies between waves.  One row loop serves both (the key shift).
+            // cell t, reading its columns from (t + j) on: again one bank per lane.  Larger
+            // tables: k histogram copies shared by the waves (8 at most, 4 for items below
+            // kColMinRows rows, whose clearing and reduction the counting does not repay;
+            // fewer when they do not fit), lane l counting into copy l % k, so skewed data
+            // collides on one address in at most 64 / k lanes of an instruction; copies are
+            // offset by 4 banks from one another, so lanes of different copies hitting the
+            // same cell use different banks.  One row loop serves both (the key shift).
             const int c32 = (int)cells;
             // (the column layout clears and reduces twice the words of the copies: it is
             // used for items of enough rows to repay that)
@@ -1030,15 +1031,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             int copies = 1, my = 0, stride = c32, span = c32 * ncol;
             if (!col) {
                 stride = ((c32 + 63) & ~63) + 4;
-                int sub = 1;
-                while (sub < 16 && 4 * (2 * sub) * stride <= kLdsCells) sub *= 2;
-                if (4 * stride <= kLdsCells) {
-                    copies = 4 * sub;
-                    my = wave * sub + (lane & (sub - 1));
-                } else {
-                    copies = (2 * stride <= kLdsCells) ? 2 : 1;
-                    my = wave % copies;
-                }
+                const int kmax = (r_end - r_begin < kColMinRows) ? 4 : 8;
+                while (copies < kmax && 2 * copies * stride <= kLdsCells) copies *= 2;
+                my = lane & (copies - 1);
                 span = (copies > 1) ? copies * stride : c32;
             }
             for (int i = tid; i < span; i += kWG) lds[i] = 0;
