@@ -989,7 +989,12 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
                                                               : kLaneCells;
             const int c32 = (int)cells;
             uint32_t *pl = reinterpret_cast<uint32_t *>(lds);
-            for (int i = tid; i < 2 * kCells * 64; i += kWG) pl[i] = 0;
+            // only the cells of this table (keys stay below c32: values are checked against
+            // their cardinalities when the context is created, rows past the end are masked)
+            for (int i = tid; i < c32 * 64; i += kWG) {
+                pl[i] = 0;
+                pl[kCells * 64 + i] = 0;
+            }
             __syncthreads();
             count_rows_lane<kCells, kPacked>(I, T, sh, r_begin, r_end, pl);
             __syncthreads();
