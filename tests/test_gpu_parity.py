@@ -448,6 +448,35 @@ def test_mid_tables_many_steps_per_chunk():
     np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
 
 
+@pytest.mark.parametrize("n_rows,n_tests", [(30_001, 400), (300_003, 2000)])
+def test_shared_sink_copies_skewed_data(n_rows, n_tests):
+    # tables of 257..2000 cells count into LDS histogram copies, lane l in copy l % k: items
+    # below 65 536 rows (the first case) use at most 4 copies, longer ones (the second, one
+    # chunk per test) up to 8; nine rows in ten take value 0, so most lanes of an
+    # instruction hit one cell
+    rng = np.random.default_rng(n_rows)
+    s = datagen.random_sample(41, 12, n_rows, 6, 12)
+    for v in range(s.n_vars):
+        col = rng.integers(1, s.card[v], n_rows)
+        col[rng.random(n_rows) < 0.9] = 0
+        s.cols[v, :n_rows] = col
+    tests = []
+    while len(tests) < n_tests:
+        vs = rng.choice(s.n_vars, size=3, replace=False)
+        if 257 <= int(np.prod(s.card[vs])) <= 2000:
+            x, y = sorted(int(v) for v in vs[:2])
+            tests.append((x, y, (int(vs[2]),)))
+    D = O.Data.of(s)
+    arr = np.array([[x, y, 1, z[0]] for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
+    with _ctx(s) as ctx:
+        r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+        for (x, y, z) in tests[:3]:
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
+    assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+    np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+
+
 def test_sliced_and_device_memory_sinks():
     # card 8 throughout: |Z| = 2 -> 4096 cells (LDS), 3 -> 32768 (2 z slices),
     # 4 -> 262144 (16 slices: device-memory sink), 5 -> 2^21 (device-memory sink)
