@@ -31,7 +31,7 @@ EXPORTS = [
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats", "fpgm_last_packed_bytes",
     "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
-    "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear",
+    "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_ci_group",
 ]
 
 
@@ -48,6 +48,11 @@ class Test(C.Structure):
 class Result(C.Structure):
     _fields_ = [("g2", C.c_double), ("p_value", C.c_double), ("df", C.c_int32), ("indep", C.c_uint8),
                 ("in_band", C.c_uint8), ("exact", C.c_uint8), ("pad", C.c_uint8)]
+
+
+class GroupResult(C.Structure):
+    _fields_ = [("p_value", C.c_double), ("is_independent", C.c_uint8), ("pad", C.c_uint8 * 3),
+                ("first", C.c_int32)]
 
 
 class PcOpts(C.Structure):
@@ -132,10 +137,11 @@ def load(path: str = LIB_PATH):
     L.fpgm_autotune.argtypes = [vp, f64, i32, C.POINTER(PcOpts), vp, i32, C.POINTER(i32), C.POINTER(f64)]
     L.fpgm_tune_set.argtypes = [vp, vp, i32]
     L.fpgm_tune_clear.argtypes = [vp]
+    L.fpgm_ci_group.argtypes = [vp, i32, i32, vp, i32, i32, f64, C.POINTER(GroupResult), vp]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
                  "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
-                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes"):
+                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes", "fpgm_ci_group"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -237,6 +243,24 @@ class Context:
         return {"g2": a["g2"].copy(), "p_value": a["p_value"].copy(), "df": a["df"].copy(),
                 "indep": a["indep"].astype(bool), "in_band": a["in_band"].astype(bool),
                 "exact": a["exact"].astype(bool)}
+
+    def ci_group(self, x, y, zsets, alpha=0.05):
+        """IndependenceResult(x, y, Z, "g square", timer, c_size) with Z the concatenation of
+        the c_size sets in zsets (all of one size d; one empty set for d = 0).  Returns the
+        group's {"p_value", "is_independent", "first"} and the per-test records."""
+        zsets = [tuple(int(v) for v in zs) for zs in zsets]
+        d = len(zsets[0]) if zsets else 0
+        assert all(len(zs) == d for zs in zsets)
+        flat = np.ascontiguousarray(np.asarray([v for zs in zsets for v in zs] or [0], dtype=np.int32))
+        n = 1 if d == 0 else len(zsets)
+        res = GroupResult()
+        out = (Result * max(n, 1))()
+        self._check(self._L.fpgm_ci_group(self.h, int(x), int(y), flat.ctypes.data if d else None, d, len(zsets),
+                                          float(alpha), C.byref(res), C.cast(out, C.c_void_p)))
+        a = np.ctypeslib.as_array(out)[:n]
+        return ({"p_value": res.p_value, "is_independent": bool(res.is_independent), "first": res.first},
+                {"g2": a["g2"].copy(), "p_value": a["p_value"].copy(), "df": a["df"].copy(),
+                 "indep": a["indep"].astype(bool)})
 
     def counts(self, x, y, z=()):
         z = np.ascontiguousarray(np.asarray(z, dtype=np.int32))

@@ -158,5 +158,7 @@ hipError_t launch_pack_cols(const uint8_t *cols, uint8_t *pcols, int64_t stride,
                             hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);
+// Zeroes n 32-bit words at p (a fill kernel: no memset node in code a caller may capture).
+hipError_t launch_zero(uint32_t *p, int64_t n, hipStream_t s);
 
 }  // namespace fpgm

@@ -423,8 +423,8 @@ __device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
 
 // Counts the (up to) 4 rows whose keys are the bytes of K; rows >= nvalid are skipped.
 // A row's byte offset in the counters is pair * kCells * 256 + key * 256 + lane * 4: every
-// key byte already carries the wave pair's bias (0, or kCells: a bit no key uses; see
-// lane_key4), and one v_perm_b32 puts key byte j into byte 1 and lane * 4 (lane4, byte 0)
+// key byte already carries the wave pair's bias (0, or kCells added to the key; see
+// lane_bias), and one v_perm_b32 puts key byte j into byte 1 and lane * 4 (lane4, byte 0)
 // into byte 0.
 __device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, int nvalid)
 {
@@ -436,9 +436,21 @@ __device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc
         }
 }
 
+// The wave pair's bias enters every key byte in one of two ways (the key of a valid row
+// is below kCells, the biased key below 2 * kCells <= 256, so no byte carries into the
+// next; rows past the end only carry into later rows, which are past the end as well):
+//   kCells a power of two: ORed into the Y bytes at unpacking (fused with the unpacking
+//     mask).  y < c(Y) <= kCells, so y | kCells == y + kCells;
+//   otherwise (96 cells): added to the packed key word after the Horner steps.  An OR
+//     would not do there: y | 96 != y + 96 once y >= 32.
+template <int kCells>
+struct lane_bias {
+    static constexpr bool kOr = (kCells & (kCells - 1)) == 0;
+};
+
 // Packed keys of the 4 rows of word q, |Z| = D known at compile time.
-// c's Y bytes carry the wave pair's bias (ORed in at unpacking): adding it to every y adds
-// it to every key, which stays below 256.
+// c's Y bytes carry the wave pair's bias when it is ORed in at unpacking: adding it to
+// every y adds it to every key.
 template <int D>
 __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *zc2, uint32_t cx2, uint32_t cy2,
                                               int q)
@@ -451,11 +463,13 @@ __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *
     return pk_mad(K, cy2, word_of(c.v[X + 1], q));
 }
 
-template <int D, int NW>
+// kOr: the bias is ORed into the Y bytes, else added to each key word (lane_bias)
+template <int D, int NW, bool kOr>
 __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTest &T, const Shared &sh,
                                                   int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
                                                   uint32_t inc, uint32_t bias)
 {
+    const uint32_t ybias = kOr ? bias : 0u, kbias = kOr ? 0u : bias;
     const ColSet<D, NW> cs(I, T, sh);
     uint32_t zc2[D > 0 ? D : 1];
 #pragma unroll
@@ -464,15 +478,16 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
     int64_t r = r_begin + (int64_t)threadIdx.x * kRowsPerThread;
     auto step = [&](const Cols<D, NW> &raw, int64_t r) {
         Cols<D> cur;
-        cs.unpack(raw, cur, bias);
+        cs.unpack(raw, cur, ybias);
         const int64_t nv = r_end - r;
         if (nv >= kRowsPerThread) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), 4);
+            for (int q = 0; q < 4; ++q)
+                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, 4);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q), (int)nv - 4 * q);
+                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, (int)nv - 4 * q);
         }
     };
     Cols<D, NW> b0, b1;
@@ -490,10 +505,12 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
 }
 
 // Runtime |Z|: d > 3, or a column of the test kept at 8 bits.
+template <bool kOr>
 __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevTest &T, const Shared &sh,
                                                     int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
                                                     uint32_t inc, uint32_t bias)
 {
+    const uint32_t ybias = kOr ? bias : 0u, kbias = kOr ? 0u : bias;
     const uint8_t *colx = I.pcols + (int64_t)T.x * I.stride;
     const uint8_t *coly = I.pcols + (int64_t)T.y * I.stride;
     const uint32_t wx = uniform(sh.wx), wy = uniform(sh.wy);
@@ -509,11 +526,11 @@ __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevT
             for (int q = 0; q < 4; ++q) K[q] = pk_mad(K[q], c2, word_of(v, q));
         }
         const uint4 vx = unpackw(ldw<4>(colx, wshift(wx), r), wx);
-        const uint4 vy = unpackw(ldw<4>(coly, wshift(wy), r), wy, bias);
+        const uint4 vy = unpackw(ldw<4>(coly, wshift(wy), r), wy, ybias);
         const int64_t nv = r_end - r;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q));
+            const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q)) + kbias;
             lane_put4(lb, lane4, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
         }
     }
@@ -537,15 +554,16 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
     const uint32_t bias = (wave >> 1) ? (uint32_t)kCells * 0x01010101u : 0u;
     const uint32_t lane4 = lane * 4u;
     constexpr int NW = kPacked ? 2 : 8;
+    constexpr bool kOr = lane_bias<kCells>::kOr;
     if (T.d <= 3 && (!kPacked || (uniform(T.flags) & kTFNarrow))) {
         switch (T.d) {
-        case 0: count_rows_lane_d<0, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        case 1: count_rows_lane_d<1, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        case 2: count_rows_lane_d<2, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
-        default: count_rows_lane_d<3, NW>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 0: count_rows_lane_d<0, NW, kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 1: count_rows_lane_d<1, NW, kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        case 2: count_rows_lane_d<2, NW, kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
+        default: count_rows_lane_d<3, NW, kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias); return;
         }
     }
-    count_rows_lane_any(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
+    count_rows_lane_any<kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
 }
 
 // |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
@@ -1319,7 +1337,20 @@ __global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, doubl
     }
 }
 
+__global__ __launch_bounds__(kWG) void k_zero(uint32_t *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) p[i] = 0u;
+}
+
 }  // namespace
+
+hipError_t launch_zero(uint32_t *p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + kWG - 1) / kWG, 1024);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(kWG), 0, s, p, n);
+    return hipGetLastError();
+}
 
 template <bool kPacked>
 void launch_count_stat_form(const LaunchArgs &a, int mode, int grid, hipStream_t s)

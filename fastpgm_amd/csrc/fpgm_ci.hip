@@ -341,6 +341,34 @@ double packed_row_bytes(const fpgm_ctx *ctx, const HostTest &h, const int32_t *z
     return bits / 8.0;
 }
 
+// Sink group of a table of at most kLdsCells cells (marg: its marginal entries), the
+// launch it joins: 0 per-lane (<= 64 cells), 1 lane-wide (<= 96), kGroupPending (97..128:
+// lane-wide or shared, decided per launch), 2 shared (<= kLdsCells); -1 for larger tables.
+// Sets the table's kTF* bits; *fits: table and marginals fit the workgroup's LDS scratch.
+int lds_group(const fpgm_ctx *ctx, DevTest &T, int64_t cells, int64_t marg, bool *fits)
+{
+    if (cells <= kLaneCells) {
+        T.flags |= kTFLane;
+        *fits = cells + marg <= kSmallCells;
+        return 0;  // kSinkLane
+    }
+    if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
+        *fits = cells + marg <= kSmallCells;
+        return 1;  // kSinkLaneW
+    }
+    if (cells <= kLaneWide128Cells && !(ctx->sink_mask & 1)) {
+        *fits = cells + marg <= kSmallCells;
+        return kGroupPending;
+    }
+    if (cells <= kLdsCells) {
+        T.flags |= kTFLdsTable;
+        *fits = cells + marg <= kLdsCells;
+        return 2;  // kSinkShared
+    }
+    *fits = false;
+    return -1;
+}
+
 // batch_wait returns.  flags: FPGM_F_*, tflags: extra kTF* bits for every test.
 // keep_ws receives the workspace offset of test i0 when kTFKeep is set.
 fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t n, const int32_t *zpool,
@@ -412,22 +440,8 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         const int64_t marg = dimz * (T.cx + T.cy + 1);
         T.flags = tflags | (narrow ? kTFNarrow : 0);
         bool fits;
-        int gi;
-        if (cells <= kLaneCells) {
-            gi = 0;  // kSinkLane
-            T.flags |= kTFLane;
-            fits = cells + marg <= kSmallCells;
-        } else if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
-            gi = 1;  // kSinkLaneW
-            fits = cells + marg <= kSmallCells;
-        } else if (cells <= kLaneWide128Cells && !(ctx->sink_mask & 1)) {
-            gi = kGroupPending;  // lane-wide (128-cell instance) or shared, decided below
-            fits = cells + marg <= kSmallCells;
-        } else if (cells <= kLdsCells) {
-            gi = 2;  // kSinkShared
-            T.flags |= kTFLdsTable;
-            fits = cells + marg <= kLdsCells;
-        } else {
+        int gi = lds_group(ctx, T, cells, marg, &fits);
+        if (gi < 0) {
             gi = 4;  // kSinkGlobal
             fits = false;
             // large device-memory tables get their statistic from the grid-wide
@@ -1323,6 +1337,43 @@ fpgm_status fpgm_ci_batch(fpgm_ctx *ctx, const fpgm_test *tests, int64_t n, cons
     fpgm_status st = run_batch(ctx, h.data(), n, z_pool, z_pool_len, alpha, flags, 0.02f, 0, r);
     if (st != FPGM_OK) return st;
     for (int64_t i = 0; i < n; ++i) out[i] = to_result(r[i]);
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_ci_group(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, int32_t d, int32_t c_size,
+                          double alpha, fpgm_group_result *res, fpgm_result *out)
+{
+    if (!ctx || !res || c_size < 1) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    // IndependenceResult's dispatch (IndependenceTest.cpp:45-52): no Z -> one XY test, whatever
+    // c_size; c_size 1 -> one XYZ test; else the group of c_size sets of d nodes each
+    const int64_t n = (d == 0) ? 1 : c_size;
+    if (d > 0 && (int64_t)d * n > INT32_MAX) return fail(ctx, FPGM_EINVAL, "group too large");
+    std::vector<HostTest> h((size_t)n);
+    for (int64_t m = 0; m < n; ++m) {
+        fpgm_status st = check_test(ctx, x, y, d > 0 ? z + m * d : nullptr, d, alpha);
+        if (st != FPGM_OK) return st;
+        h[m] = HostTest{x, y, d, 0, m * d};
+    }
+    reset_kernel_stats(ctx);
+    std::vector<DevResult> r;
+    fpgm_status st = run_batch(ctx, h.data(), n, z, d * n, alpha, FPGM_F_EXACT, 0.02f, kTFExact, r);
+    if (st != FPGM_OK) return st;
+    if (out)
+        for (int64_t m = 0; m < n; ++m) out[m] = to_result(r[m]);
+    const fpgm_result r0 = to_result(r[0]);
+    if (d == 0) {  // ComputeGSquareXY's Result(p, indep): first keeps its default -1 (:349-363)
+        *res = fpgm_group_result{r0.p_value, r0.indep, {0, 0, 0}, -1};
+    } else if (n == 1) {  // ComputeGSquareXYZ's Result(p, indep, 0) (:140-154)
+        *res = fpgm_group_result{r0.p_value, r0.indep, {0, 0, 0}, 0};
+    } else {  // ComputeGSquareXYZGroup: the first independent set of the group (:280-286)
+        *res = fpgm_group_result{0.0, 0, {0, 0, 0}, -1};
+        for (int64_t m = 0; m < n; ++m)
+            if (r[m].indep) {
+                *res = fpgm_group_result{-1.0, 1, {0, 0, 0}, (int32_t)m};
+                break;
+            }
+    }
     return FPGM_OK;
 }
 

@@ -142,6 +142,32 @@ fpgm_status fpgm_ci_test(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, 
 fpgm_status fpgm_ci_batch(fpgm_ctx *ctx, const fpgm_test *tests, int64_t n, const int32_t *z_pool,
                           int64_t z_pool_len, double alpha, uint32_t flags, fpgm_result *out);
 
+/* IndependenceTest::Result (IndependenceTest.h:38-49) of a grouped call. */
+typedef struct {
+    double p_value;          /* as the reference returns it (see fpgm_ci_group)                 */
+    uint8_t is_independent;
+    uint8_t pad[3];
+    int32_t first;           /* index of the first independent test of the group, or -1        */
+} fpgm_group_result;
+
+/*
+ * A group of CI tests sharing (x, y): replaces
+ * IndependenceTest::IndependenceResult(x, y, z, "g square", timer, c_size)
+ * (IndependenceTest.cpp:35-58), the call PCStable::Testing makes with a group of
+ * conditioning sets (PCStable.cpp:539-569).  z holds c_size sets of d node ids each, set m
+ * at z[m*d .. m*d+d).  res receives the reference's Result for the dispatch it takes:
+ *   d == 0            ComputeGSquareXY (:295-364): one test, {p, independent, first = -1};
+ *   c_size == 1       ComputeGSquareXYZ (:65-155): {p, independent, first = 0};
+ *   c_size > 1        ComputeGSquareXYZGroup (:163-287): the first independent set m of the
+ *                     group gives {-1, true, m} (:280-284), none gives {0.0, false, -1}.
+ * The reference holds at most 8 results (bool results[8], :170); here c_size is unbounded.
+ * df == 0 counts as independent, as in ComputeGSquareXYZ (:140-142).  out (may be NULL;
+ * c_size entries, 1 when d == 0) receives every test's record, FP64-exact.  Tests of the
+ * group run in one launch sequence, so X and Y are read once into L2 for all of them.
+ */
+fpgm_status fpgm_ci_group(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z, int32_t d, int32_t c_size,
+                          double alpha, fpgm_group_result *res, fpgm_result *out);
+
 /*
  * Raw contingency table of one test, for parity checks: the dense
  * n[z][x][y] int32 table of Counts3D / Counts2D (CellTable.cpp:23-91, :174-205,

@@ -105,3 +105,32 @@ def brute_pc_stable(cols, n_rows, card, alpha, depth, test_fn):
         for j in adj[i]:
             a[i, j] = 1
     return a, sep, tests
+
+
+def sepset_digest(seps):
+    """SHA-256 of the sepset map in a canonical text form: one line "x y level z1,z2,..."
+    per record, records in ascending (x, y) order."""
+    import hashlib
+    h = hashlib.sha256()
+    for (x, y) in sorted(seps):
+        level, z = seps[(x, y)]
+        h.update(f"{x} {y} {level} {','.join(str(v) for v in z)}\n".encode())
+    return h.hexdigest()
+
+
+def stage_fixture(name):
+    """A committed oracle stage (tests/golden/make_stage_fixtures.py) as a dict of arrays."""
+    path = os.path.join(GOLDEN, f"stage_{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def ensemble_sha256(samples):
+    """SHA-256 over the valid bytes of every replicate, in order, and the cardinalities."""
+    import hashlib
+    h = hashlib.sha256()
+    for s in samples:
+        h.update(np.int64(s.n_rows).tobytes())
+        h.update(np.ascontiguousarray(s.cols[:, : s.n_rows]).tobytes())
+    h.update(samples[0].card.astype(np.int32).tobytes())
+    return h.hexdigest()

@@ -137,6 +137,56 @@ def test_lane_wide_instances_and_routing():
             np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
 
 
+def _lane_wide_sample(n_rows, seed):
+    # cardinalities chosen so that tables of 65..96 cells have c(Y) from 33 to 90: the
+    # 48 KiB per-lane sink, whose second wave pair's bias (96) must be added to the key,
+    # not ORed into Y.  Variables 4 and 5 depend on variable 0, so that some tables are
+    # far from independent.
+    s = datagen.random_sample(seed, 8, n_rows, 2, 2)
+    s.card[:] = [2, 1, 2, 33, 40, 48, 80, 90]
+    rng = np.random.default_rng(seed)
+    for v in range(8):
+        s.cols[v, :n_rows] = rng.integers(0, s.card[v], n_rows)
+    for v in (4, 5):
+        keep = rng.random(n_rows) < 0.5
+        s.cols[v, :n_rows] = np.where(keep, s.cols[0, :n_rows] * (s.card[v] // 2) + s.cols[v, :n_rows] % 3,
+                                      s.cols[v, :n_rows])
+    return s
+
+
+_LANE_WIDE_TESTS = [(0, 3, ()), (0, 4, ()), (0, 5, ()), (1, 6, ()), (1, 7, ()), (2, 5, ()), (2, 4, ()),
+                    (0, 4, (1,)), (1, 4, (0,)), (0, 5, (1,)), (1, 3, (2,)), (1, 5, (2,)), (2, 5, (1,))]
+
+
+@pytest.mark.parametrize("n_rows", [4099, 20_011, 262_147])
+def test_lane_wide_sink_large_y(n_rows):
+    # every test 65..96 cells (no 97..128-cell table in the launch, so the 48 KiB instance
+    # counts them), at least 4096 rows so that waves 2 and 3 count rows; through the
+    # counts, the batch and a two-instance level 0 on the counting kernel
+    s = _lane_wide_sample(n_rows, 17 + n_rows)
+    D = O.Data.of(s)
+    tests = _LANE_WIDE_TESTS
+    for (x, y, z) in tests:
+        assert 65 <= int(np.prod([s.card[v] for v in (x, y) + z])) <= 96
+    with _ctx(s) as ctx:
+        for (x, y, z) in tests:
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z)), (x, y, z)
+        r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+    arr = np.array([[x, y, len(z)] + list(z) + [0] * (1 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
+    assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+    np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+    assert not o["indep"].all()
+    # level 0 of two instances on the counting kernel (no Gram path for this context)
+    s2 = _lane_wide_sample(n_rows // 2 + 4096, 5 + n_rows)
+    with _lib.Context.of_many([s, s2]) as ctx:
+        g = ctx.pc_skeleton_multi(0.05, 1, flags=_lib.FPGM_F_NO_GRAM)
+    for i, si in enumerate((s, s2)):
+        oi = O.pc_skeleton(O.Data.of(si), 0.05, 1, n_threads=8)
+        assert np.array_equal(g["adj"][i], oi["adj"]), i
+        assert g["num_ci_test_per_instance"][i] == oi["num_ci_test"]
+
+
 def test_counts_large_tables_global_path():
     # card 8: d = 3 gives 8^3 * 64 = 32768 cells (beyond the LDS table)
     s = datagen.random_sample(5, 8, 200_003, 8, 8)
@@ -180,6 +230,41 @@ def test_ci_test_single_launch_matches_oracle():
             assert r["df"] == o["df"] and r["indep"] == o["indep"]
             assert r["g2"] == pytest.approx(o["g2"], rel=1e-11, abs=1e-9)
             assert r["p_value"] == pytest.approx(o["p_value"], rel=1e-9, abs=1e-12)
+
+
+def test_ci_group_first_independent():
+    # IndependenceResult with a group of conditioning sets (ComputeGSquareXYZGroup,
+    # IndependenceTest.cpp:163-287): the index of the first independent set, which
+    # PCStable::Testing turns into the sepset; per-test records equal the oracle's
+    s = alarm_sample()
+    D = O.Data.of(s)
+    rng = np.random.default_rng(12)
+    firsts = []
+    with _ctx(s) as ctx:
+        for _ in range(40):
+            d, c = int(rng.integers(1, 4)), int(rng.integers(2, 11))
+            x, y = sorted(int(v) for v in rng.choice(s.n_vars, 2, replace=False))
+            others = [v for v in range(s.n_vars) if v not in (x, y)]
+            zs = [tuple(sorted(int(v) for v in rng.choice(others, d, replace=False))) for _ in range(c)]
+            res, per = ctx.ci_group(x, y, zs, alpha=0.05)
+            arr = np.array([[x, y, d] + list(z) + [0] * (3 - d) for z in zs], dtype=np.int32)
+            o = O.ci_batch(D, arr, 3, 0.05, n_threads=8)
+            assert np.array_equal(per["indep"], o["indep"]) and np.array_equal(per["df"], o["df"])
+            np.testing.assert_allclose(per["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+            first = next((m for m in range(c) if o["indep"][m]), -1)
+            assert res == {"p_value": -1.0 if first >= 0 else 0.0, "is_independent": first >= 0, "first": first}
+            firsts.append(first)
+        # both outcomes, and independent sets after the first position, occur
+        assert -1 in firsts and max(firsts) >= 1
+        # the other dispatches: no Z (first stays -1), one set (first = 0, even when dependent)
+        for (x, y, zs) in ((0, 1, [()]), (4, 9, [(2,)]), (12, 30, [(5, 7)])):
+            res, per = ctx.ci_group(x, y, zs, alpha=0.05)
+            o = O.ci_test(D, x, y, zs[0], alpha=0.05)
+            assert res["is_independent"] == o["indep"]
+            assert res["first"] == (-1 if not zs[0] else 0)
+            assert res["p_value"] == pytest.approx(o["p_value"], rel=1e-9, abs=1e-12)
+        with pytest.raises(_lib.FpgmError):
+            ctx.ci_group(3, 3, [(1,), (2,)])
 
 
 def _assert_same(g, o):
@@ -426,8 +511,9 @@ def test_level0_gram_card_paths(card_hi):
 
 
 def test_mid_tables_many_steps_per_chunk():
-    # 300 003 rows and thousands of tests: every chunk spans tens of 4096-row steps, so the
-    # 8-bit per-lane counters (tables of 65..256 cells) go through several flushes
+    # 300 003 rows and thousands of tests of 65..256 cells: every chunk spans tens of
+    # 4096-row steps, through the 16-bit per-lane counters (65..96 cells, and 97..128 when
+    # the launch has no shared-sink table) and the shared sink's column layout (97..256)
     w = configs.get(3)
     net = w.network()
     s = datagen.forward_sample(net, 300_003, w.seed)
