@@ -31,7 +31,7 @@ EXPORTS = [
     "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts", "fpgm_pc_skeleton", "fpgm_sepset_count",
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats", "fpgm_last_packed_bytes",
     "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
-    "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_ci_group",
+    "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_ci_group", "fpgm_ci_batch_noalloc",
 ]
 
 
@@ -138,10 +138,12 @@ def load(path: str = LIB_PATH):
     L.fpgm_tune_set.argtypes = [vp, vp, i32]
     L.fpgm_tune_clear.argtypes = [vp]
     L.fpgm_ci_group.argtypes = [vp, i32, i32, vp, i32, i32, f64, C.POINTER(GroupResult), vp]
+    L.fpgm_ci_batch_noalloc.argtypes = [vp, vp, i64, vp, i64, f64, u32, vp]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
                  "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
-                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes", "fpgm_ci_group"):
+                 "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes", "fpgm_ci_group",
+                 "fpgm_ci_batch_noalloc"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -227,8 +229,8 @@ class Context:
         return {"g2": r.g2, "p_value": r.p_value, "df": r.df, "indep": bool(r.indep), "in_band": bool(r.in_band),
                 "exact": bool(r.exact)}
 
-    def ci_batch(self, tests, alpha=0.05, flags=0):
-        """tests: sequence of (x, y, z-tuple)."""
+    def ci_batch(self, tests, alpha=0.05, flags=0, noalloc=False):
+        """tests: sequence of (x, y, z-tuple).  noalloc: through fpgm_ci_batch_noalloc."""
         n = len(tests)
         arr = (Test * max(n, 1))()
         pool = []
@@ -237,8 +239,9 @@ class Context:
             pool.extend(int(v) for v in z)
         zp = np.ascontiguousarray(np.asarray(pool if pool else [0], dtype=np.int32))
         out = (Result * max(n, 1))()
-        self._check(self._L.fpgm_ci_batch(self.h, C.cast(arr, C.c_void_p), n, zp.ctypes.data, len(pool),
-                                          float(alpha), int(flags), C.cast(out, C.c_void_p)))
+        fn = self._L.fpgm_ci_batch_noalloc if noalloc else self._L.fpgm_ci_batch
+        self._check(fn(self.h, C.cast(arr, C.c_void_p), n, zp.ctypes.data, len(pool), float(alpha), int(flags),
+                       C.cast(out, C.c_void_p)))
         a = np.ctypeslib.as_array(out)[:n]
         return {"g2": a["g2"].copy(), "p_value": a["p_value"].copy(), "df": a["df"].copy(),
                 "indep": a["indep"].astype(bool), "in_band": a["in_band"].astype(bool),

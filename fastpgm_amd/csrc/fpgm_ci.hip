@@ -2373,3 +2373,261 @@ fpgm_status fpgm_last_kernel_stats(const fpgm_ctx *ctx, double *kernel_ms, doubl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free batch
+//
+// fpgm_ci_batch_noalloc: fpgm_ci_batch for callers that cannot allocate device memory.
+// Every buffer the counting launches need beyond the context's columns -- descriptors,
+// z pool, work items, results, completion and queue counters, critical values and the
+// workspace of tables split across workgroups -- lives in `scratch`, one static device
+// array per GPU, whose address is taken once per device with hipGetSymbolAddress.  The
+// host stages descriptors in static arrays.  Calls are serialised by na_mu within a
+// process; calls from several processes on one GPU would race on its scratch.
+
+namespace {
+
+constexpr int64_t kNaTests = 4096;          // tests per call
+constexpr int64_t kNaZ = kNaTests * 8;      // z pool entries per call
+constexpr int64_t kNaItems = 65536;         // work items per launch sequence
+constexpr int64_t kNaWs = (int64_t)1 << 22; // workspace int32 entries (16 MiB)
+constexpr int kNaDevices = 64;
+
+constexpr int64_t na_align(int64_t b)
+{
+    return (b + 255) / 256 * 256;
+}
+// scratch layout (byte offsets)
+constexpr int64_t kNaOffTests = 0;
+constexpr int64_t kNaOffRes = kNaOffTests + na_align(kNaTests * (int64_t)sizeof(DevTest));
+constexpr int64_t kNaOffZ = kNaOffRes + na_align(kNaTests * (int64_t)sizeof(DevResult));
+constexpr int64_t kNaOffItems = kNaOffZ + na_align(kNaZ * 4);
+constexpr int64_t kNaOffCrit = kNaOffItems + na_align(kNaItems * 8);
+constexpr int64_t kNaOffDone = kNaOffCrit + na_align(kCritTab * 8);
+constexpr int64_t kNaOffQueue = kNaOffDone + na_align(kNaTests * 4);
+constexpr int64_t kNaOffWs = kNaOffQueue + na_align(kGroups * (kXcds + 1) * 4);
+constexpr int64_t kNaBytes = kNaOffWs + kNaWs * 4;
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+std::mutex na_mu;
+uint8_t *na_ptr[kNaDevices] = {};                 // scratch's device address, per device
+double na_crit_alpha[kNaDevices];                 // alpha of the critical values in scratch
+bool na_crit_valid[kNaDevices] = {};
+// host staging (calls hold na_mu)
+DevTest na_tests[kNaTests];
+DevResult na_res[kNaTests];
+uint64_t na_items[kNaItems];
+int32_t na_gidx[kNaTests];
+uint8_t na_fit[kNaTests];
+double na_crit[kCritTab];
+
+// scratch's address on the context's device, taken on the first call there (with that
+// device current) and kept
+fpgm_status na_scratch(fpgm_ctx *ctx, uint8_t **out)
+{
+    if (ctx->dev < 0 || ctx->dev >= kNaDevices) return fail(ctx, FPGM_EDEVICE, "device index beyond the scratch table");
+    if (!na_ptr[ctx->dev]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != ctx->dev) HIP_TRY(hipSetDevice(ctx->dev), "hipSetDevice");
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != ctx->dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "scratch address");
+        na_ptr[ctx->dev] = (uint8_t *)p;
+    }
+    *out = na_ptr[ctx->dev];
+    return FPGM_OK;
+}
+
+// One launch sequence over tests [i0, i1) (descriptors already in na_tests[0, i1 - i0),
+// groups in na_gidx): uploads, clears the counters and the workspace it uses, counts,
+// takes the p-values and reads the results back into na_res.
+fpgm_status na_sequence(fpgm_ctx *ctx, uint8_t *base, int64_t nb, const int32_t *zpool, int64_t zlen,
+                        const int64_t *grp_begin, const int64_t *grp_len, const int64_t *rpc, bool lane_w128,
+                        int64_t ws_used, double alpha, uint32_t flags)
+{
+    DevTest *d_tests = (DevTest *)(base + kNaOffTests);
+    DevResult *d_res = (DevResult *)(base + kNaOffRes);
+    int32_t *d_z = (int32_t *)(base + kNaOffZ);
+    uint64_t *d_items = (uint64_t *)(base + kNaOffItems);
+    double *d_crit = (double *)(base + kNaOffCrit);
+    uint32_t *d_done = (uint32_t *)(base + kNaOffDone);
+    uint32_t *d_queue = (uint32_t *)(base + kNaOffQueue);
+    int32_t *d_ws = (int32_t *)(base + kNaOffWs);
+    hipStream_t s = ctx->stream;
+    int64_t ni = 0;
+    for (int g = 0; g < kGroups; ++g) ni += grp_len[g];
+    // the previous call's contents stay in scratch: counters and the workspace region this
+    // sequence uses start from zero (done and queue counters are adjacent)
+    HIP_TRY(launch_zero(d_done, (kNaOffWs - kNaOffDone) / 4, s), "clear counters");
+    HIP_TRY(launch_zero((uint32_t *)d_ws, ws_used, s), "clear workspace");
+    if (!na_crit_valid[ctx->dev] || na_crit_alpha[ctx->dev] != alpha) {
+        for (int df = 1; df <= kCritTab; ++df) na_crit[df - 1] = chisq_crit(df, alpha);
+        HIP_TRY(hipMemcpyAsync(d_crit, na_crit, sizeof(double) * kCritTab, hipMemcpyHostToDevice, s), "upload crit");
+        na_crit_alpha[ctx->dev] = alpha;
+        na_crit_valid[ctx->dev] = true;
+    }
+    if (zlen > 0) HIP_TRY(hipMemcpyAsync(d_z, zpool, sizeof(int32_t) * zlen, hipMemcpyHostToDevice, s), "upload z pool");
+    HIP_TRY(hipMemcpyAsync(d_tests, na_tests, sizeof(DevTest) * nb, hipMemcpyHostToDevice, s), "upload tests");
+    HIP_TRY(hipMemcpyAsync(d_items, na_items, sizeof(uint64_t) * ni, hipMemcpyHostToDevice, s), "upload items");
+    LaunchArgs a{};
+    a.inst = ctx->d_inst;
+    a.tests = d_tests;
+    a.zpool = d_z;
+    a.card = ctx->d_card;
+    a.colw = ctx->d_colw;
+    a.items = d_items;
+    a.ws = d_ws;
+    a.done = d_done;
+    a.crit = d_crit;
+    a.alpha = alpha;
+    a.z_alpha = (alpha > 0.0 && alpha < 1.0) ? norm_upper_quantile(alpha) : 0.0;
+    a.band = 0.02f;
+    a.flags = flags;
+    a.res = d_res;
+    a.edge_done = nullptr;
+    a.round = 0;
+    for (int gi = 0; gi < kGroups; ++gi) {
+        if (grp_len[gi] == 0) continue;
+        a.item_begin = grp_begin[gi];
+        a.n_items = grp_len[gi];
+        a.rows_per_chunk = rpc[gi];
+        a.queue = d_queue + gi * kXcds;
+        a.n_items_dev = nullptr;
+        const int sink = (gi == 1 && lane_w128) ? kSinkLaneW128 : kGroupSink[gi];
+        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * ctx->sched.wpc[gi]);
+        grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+        HIP_TRY(launch_count_stat(a, sink, ctx->d_pcols != nullptr, (int)grid, s), "launch count_stat");
+    }
+    HIP_TRY(launch_pvalue(d_res, nb, alpha, d_tests, nullptr, 0, d_queue, kGroups * (kXcds + 1), s), "launch pvalue");
+    HIP_TRY(hipMemcpyAsync(na_res, d_res, sizeof(DevResult) * nb, hipMemcpyDeviceToHost, s), "download results");
+    HIP_TRY(hipStreamSynchronize(s), "synchronize");
+    return FPGM_OK;
+}
+
+}  // namespace
+
+extern "C" fpgm_status fpgm_ci_batch_noalloc(fpgm_ctx *ctx, const fpgm_test *tests, int64_t n,
+                                             const int32_t *z_pool, int64_t z_pool_len, double alpha, uint32_t flags,
+                                             fpgm_result *out)
+{
+    if (!ctx || (n > 0 && (!tests || !out)) || n < 0) return FPGM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<std::mutex> lk2(na_mu);
+    if (n > kNaTests) return fail(ctx, FPGM_ETOOBIG, "more than 4096 tests in one allocation-free call");
+    if (z_pool_len > kNaZ) return fail(ctx, FPGM_ETOOBIG, "z pool beyond 32768 entries in an allocation-free call");
+    // checks and descriptors (fpgm_ci_batch's rules, tables of at most kLdsCells cells)
+    int64_t N = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const fpgm_test &t = tests[i];
+        if (t.d > 0 && (t.z_off < 0 || (int64_t)t.z_off + t.d > z_pool_len || !z_pool))
+            return fail(ctx, FPGM_EINVAL, "z_off/d outside the z pool");
+        fpgm_status st = check_test(ctx, t.x, t.y, t.d > 0 ? z_pool + t.z_off : nullptr, t.d, alpha);
+        if (st != FPGM_OK) return st;
+        DevTest T{};
+        T.x = t.x;
+        T.y = t.y;
+        T.d = t.d;
+        T.z_off = t.z_off;
+        T.inst = 0;
+        T.edge = -1;
+        T.cx = ctx->card[t.x];
+        T.cy = ctx->card[t.y];
+        int64_t dimz = 1;
+        bool narrow = ctx->colw[t.x] != 8 && ctx->colw[t.y] != 8;
+        for (int j = 0; j < t.d; ++j) {
+            dimz *= ctx->card[z_pool[t.z_off + j]];
+            narrow = narrow && ctx->colw[z_pool[t.z_off + j]] != 8;
+            if (dimz > kLdsCells) break;
+        }
+        const int64_t cells = dimz * T.cx * T.cy;
+        bool fits = false;
+        const int gi = cells <= kLdsCells ? lds_group(ctx, T, cells, dimz * (T.cx + T.cy + 1), &fits) : -1;
+        if (gi < 0) return fail(ctx, FPGM_ETOOBIG, "allocation-free calls take tables of at most 8192 cells");
+        T.dimz = (int32_t)dimz;
+        T.n_slices = 1;
+        T.slice_cells = (int32_t)cells;
+        T.flags |= narrow ? kTFNarrow : 0;
+        T.ws_off = -1;
+        na_tests[i] = T;
+        na_gidx[i] = gi;
+        na_fit[i] = fits ? 1 : 0;
+    }
+    if (n == 0) return FPGM_OK;
+    uint8_t *base = nullptr;
+    fpgm_status st = na_scratch(ctx, &base);
+    if (st != FPGM_OK) return st;
+    N = ctx->inst[0].n_rows;
+    const int64_t steps = std::max<int64_t>(1, (N + kRowsPerStep - 1) / kRowsPerStep);
+    const int64_t target_items = (int64_t)ctx->n_cu * ctx->sched.items_per_cu;
+    // launch sequences: tests in order while their workspace and work items fit scratch
+    for (int64_t i0 = 0; i0 < n;) {
+        const int64_t m = n - i0;
+        // rows per chunk: one value for the LDS sinks, enough items over all tests to fill the CUs
+        const int64_t cpt = std::min(steps, std::max<int64_t>(1, (target_items + m - 1) / m));
+        const int64_t r = std::min<int64_t>((steps + cpt - 1) / cpt * kRowsPerStep, kLaneMaxRowsPerChunk);
+        const int64_t n_chunks = std::max<int64_t>(1, (N + r - 1) / r);
+        // 97..128-cell tables join the shared launch when there is one (batch_submit's rule)
+        bool any_shared = false;
+        for (int64_t k = i0; k < n; ++k) any_shared = any_shared || na_gidx[k] == 2;
+        int64_t ws_used = 0, items = 0, nb = 0;
+        int64_t grp_len[kGroups] = {0, 0, 0, 0, 0};
+        for (; i0 + nb < n; ++nb) {
+            const int64_t k = i0 + nb;
+            DevTest T = na_tests[k];
+            int gi = na_gidx[k];
+            bool fit = na_fit[k] != 0;
+            if (gi == kGroupPending) {
+                if (any_shared) {
+                    const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
+                    T.flags |= kTFLdsTable;
+                    fit = cells + (int64_t)T.dimz * (T.cx + T.cy + 1) <= kLdsCells;
+                    gi = 2;
+                } else {
+                    gi = 1;
+                }
+            }
+            const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
+            const bool need_ws = n_chunks > 1 || !fit;
+            const int64_t need = need_ws ? (cells + (int64_t)T.dimz * (T.cx + T.cy + 1) + 63) / 64 * 64 : 0;
+            if (ws_used + need > kNaWs || items + n_chunks > kNaItems) break;
+            T.n_chunks = (int32_t)n_chunks;
+            if (need_ws) {
+                T.ws_off = ws_used;
+                ws_used += need;
+            } else {
+                T.flags |= kTFLdsMarg;
+            }
+            T.item0 = (int32_t)grp_len[gi];
+            grp_len[gi] += n_chunks;
+            items += n_chunks;
+            na_tests[k] = T;
+            na_gidx[k] = gi;
+        }
+        if (nb == 0) return fail(ctx, FPGM_ETOOBIG, "a test's workspace exceeds the allocation-free scratch");
+        // descriptors of the sequence to the front of the staging arrays, items by group
+        int64_t grp_begin[kGroups], acc = 0, rpc[kGroups];
+        for (int g = 0; g < kGroups; ++g) {
+            grp_begin[g] = acc;
+            acc += grp_len[g];
+            rpc[g] = r;
+        }
+        for (int64_t k = 0; k < nb; ++k) {
+            DevTest &T = na_tests[i0 + k];
+            T.item0 += (int32_t)grp_begin[na_gidx[i0 + k]];
+            for (int64_t j = 0; j < T.n_chunks; ++j) na_items[T.item0 + j] = ((uint64_t)k << 32) | (uint64_t)j;
+            if (i0 > 0) na_tests[k] = T;
+        }
+        st = na_sequence(ctx, base, nb, z_pool, z_pool_len, grp_begin, grp_len, rpc, !any_shared, ws_used, alpha,
+                         flags);
+        if (st != FPGM_OK) return st;
+        for (int64_t k = 0; k < nb; ++k) out[i0 + k] = to_result(na_res[k]);
+        i0 += nb;
+    }
+    return FPGM_OK;
+}

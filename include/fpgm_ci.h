@@ -169,6 +169,21 @@ fpgm_status fpgm_ci_group(fpgm_ctx *ctx, int32_t x, int32_t y, const int32_t *z,
                           double alpha, fpgm_group_result *res, fpgm_result *out);
 
 /*
+ * fpgm_ci_batch without any allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and results as fpgm_ci_batch; the library's buffers for the
+ * call -- descriptors, work items, results, counters, critical values, the workspace of
+ * tables split across workgroups -- live in one static device array of 17.7 MB per GPU
+ * (held by every GPU the library runs on until the process ends), and nothing is
+ * allocated on the host or the device.  Limits: at most 4096 tests and 32768 z-pool
+ * entries per call, tables of at most 8192 cells (FPGM_ETOOBIG beyond).  The context
+ * (created beforehand) holds the columns.  Calls within one process are serialised;
+ * calls from several processes on one GPU race on its array.
+ */
+fpgm_status fpgm_ci_batch_noalloc(fpgm_ctx *ctx, const fpgm_test *tests, int64_t n, const int32_t *z_pool,
+                                  int64_t z_pool_len, double alpha, uint32_t flags, fpgm_result *out);
+
+/*
  * Raw contingency table of one test, for parity checks: the dense
  * n[z][x][y] int32 table of Counts3D / Counts2D (CellTable.cpp:23-91, :174-205,
  * :226-291, :430-455), z the mixed-radix index with the first Z most significant.
