@@ -32,6 +32,7 @@ EXPORTS = [
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats", "fpgm_last_packed_bytes",
     "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
     "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_ci_group", "fpgm_ci_batch_noalloc",
+    "fpgm_last_group_stats",
 ]
 
 
@@ -139,11 +140,12 @@ def load(path: str = LIB_PATH):
     L.fpgm_tune_clear.argtypes = [vp]
     L.fpgm_ci_group.argtypes = [vp, i32, i32, vp, i32, i32, f64, C.POINTER(GroupResult), vp]
     L.fpgm_ci_batch_noalloc.argtypes = [vp, vp, i64, vp, i64, f64, u32, vp]
+    L.fpgm_last_group_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
                  "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
                  "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes", "fpgm_ci_group",
-                 "fpgm_ci_batch_noalloc"):
+                 "fpgm_ci_batch_noalloc", "fpgm_last_group_stats"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -383,6 +385,11 @@ class Context:
         ms, ops, pms, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
         self._check(self._L.fpgm_last_gram_stats(self.h, C.byref(ms), C.byref(ops), C.byref(pms), C.byref(n)))
         return {"gram_ms": ms.value, "int8_ops": ops.value, "pairs_ms": pms.value, "launches": n.value}
+
+    def group_stats(self):
+        ms, ops, t, n = C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
+        self._check(self._L.fpgm_last_group_stats(self.h, C.byref(ms), C.byref(ops), C.byref(t), C.byref(n)))
+        return {"ms": ms.value, "int8_ops": ops.value, "tests": t.value, "sequences": n.value}
 
     def kernel_stats(self):
         ms, b, n = C.c_double(), C.c_double(), C.c_int64()

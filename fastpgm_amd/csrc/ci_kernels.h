@@ -143,6 +143,40 @@ struct CompactArgs {
     uint32_t *count;   // [5], zero before the launch
 };
 
+// Grouped level-1 counting on the MFMA units.  The tests (x, y | z) of a round that share
+// the walking endpoint u (x for the x side of an edge's walk, y for its y side) and the
+// conditioning node z form a group; with `other` the remaining endpoint of each test,
+// the group's tables are the blocks of C = A^T B, A the one-hot of (u, z) over the rows
+// (row index c * c(u) + a for u = a, z = c) and B the one-hot of every other (columns
+// concatenated in test order): test j's table n[z][x][y] is C[z * c(u) + u][col_j + other]
+// read with x, y in their order.  A chunk holds the tests of one group whose columns fit
+// kGrpB 32-column fragments; one work item = (chunk, row split).
+constexpr int kGrpB = 4;                    // B fragments (32 one-hot columns each) per chunk
+constexpr int kGrpCols = 32 * kGrpB;        // one-hot columns per chunk
+constexpr int kGrpRows = 64;                // A rows per chunk: c(u) * c(z) <= 64 (two fragments)
+constexpr int kGrpMaxOther = 32;            // largest c(other)
+constexpr int kGrpMaxTests = 64;            // tests per chunk
+
+struct GroupChunk {
+    int32_t inst, u, z;
+    int32_t cu, cz;
+    int32_t t0, n_t;        // its tests: gtest[t0 .. t0 + n_t), indices into the launch's tests
+    int32_t n_cols;         // one-hot columns: the sum of the others' cardinalities (<= kGrpCols)
+    int32_t n_splits;       // row splits (work items) of the chunk
+    int32_t done;           // its completion counter in LaunchArgs::done when split
+    int64_t ws_off;         // kGrpRows x kGrpCols int32 partial table in the workspace when split, else -1
+};
+
+struct GroupArgs {
+    LaunchArgs a;           // tests, results, critical values, skip state, workspace, counters
+    const GroupChunk *chunks;
+    const int32_t *gtest;
+    const uint64_t *items;  // chunk << 32 | split
+    int64_t n_items;
+    int64_t rows_per_split; // multiple of 128
+    int32_t max_card;       // largest cardinality of the launch's columns (<= 8: table lookup)
+};
+
 // packed: some column of the context is packed (the kernels' packed-column form)
 hipError_t launch_count_stat(const LaunchArgs &a, int mode, bool packed, int grid, hipStream_t s);
 hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s);
@@ -158,6 +192,9 @@ hipError_t launch_pack_cols(const uint8_t *cols, uint8_t *pcols, int64_t stride,
                             hipStream_t s);
 hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars,
                                const int32_t *card, uint32_t *bad, hipStream_t s);
+// Grouped level-1 chunks on the MFMA units (fused with each test's statistic).
+// n_a: A fragments the launch's chunks need (1: c(u) * c(z) <= 32, else 2).
+hipError_t launch_group_count(const GroupArgs &g, int n_a, hipStream_t s);
 // Zeroes n 32-bit words at p (a fill kernel: no memset node in code a caller may capture).
 hipError_t launch_zero(uint32_t *p, int64_t n, hipStream_t s);
 

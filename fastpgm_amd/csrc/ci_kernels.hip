@@ -22,6 +22,7 @@
 #include "ci_kernels.h"
 #include "chisq.h"
 #include "fpgm_ci.h"
+#include "onehot.h"
 
 namespace fpgm {
 
@@ -1337,12 +1338,267 @@ __global__ __launch_bounds__(kWG) void k_pvalue(DevResult *res, int64_t n, doubl
     }
 }
 
+// ---------------------------------------------------------------- grouped level 1 (MFMA)
+
+// Statistic of one test of a group chunk (one thread per test) from the chunk's table
+// tab (row stride kGrpCols): cell (c, i, j) of n[z][x][y] is tab[(c * cu + i) * kGrpCols
+// + col0 + j] when u is x, tab[(c * cu + j) * kGrpCols + col0 + i] when u is y.  Same
+// marginals, adjusted df, FP32 terms, decision and FP64 recheck (terms in the reference's
+// z, x, y order, IndependenceTest.cpp:112-137) as finalize.  nj: the thread's marginal
+// scratch, entry j at nj[j * kGrpMaxTests].
+__device__ void group_stat(const LaunchArgs &a, int t, const DevTest &T, const int32_t *tab, int col0, int cu,
+                           int cz, bool u_is_x, int32_t *nj)
+{
+    const int cx = T.cx, cy = T.cy;
+    auto cell = [&](int c, int i, int j) -> int32_t {
+        const int ur = u_is_x ? i : j, oc = u_is_x ? j : i;
+        return tab[(c * cu + ur) * kGrpCols + col0 + oc];
+    };
+    // n_{+yz} of slice c into nj; returns n_{++z}, with the count of non-empty columns
+    auto col_sums = [&](int c, int &aly) -> int32_t {
+        int32_t nk = 0;
+        aly = 0;
+        for (int j = 0; j < cy; ++j) {
+            int32_t v = 0;
+            for (int i = 0; i < cx; ++i) v += cell(c, i, j);
+            nj[j * kGrpMaxTests] = v;
+            nk += v;
+            aly += (v > 0);
+        }
+        return nk;
+    };
+    int df = 0;
+    float s = 0.f, cc = 0.f, as = 0.f;
+    for (int c = 0; c < cz; ++c) {
+        int aly;
+        const int32_t nk = col_sums(c, aly);
+        int alx = 0;
+        for (int i = 0; i < cx; ++i) {
+            int32_t ni = 0;
+            for (int j = 0; j < cy; ++j) ni += cell(c, i, j);
+            alx += (ni > 0);
+            if (ni == 0) continue;
+            for (int j = 0; j < cy; ++j) {
+                const int32_t n = cell(c, i, j);
+                if (n == 0) continue;
+                const float tm = term32(n, ni, nj[j * kGrpMaxTests], nk);
+                nadd(s, cc, tm);
+                as += fabsf(tm);
+            }
+        }
+        alx = alx >= 1 ? alx : 1;
+        aly = aly >= 1 ? aly : 1;
+        df += (alx - 1) * (aly - 1);
+    }
+    DevResult R;
+    if (decide32(a, T.flags, df, s + cc, as, R)) {
+        double G = 0.0;
+        for (int c = 0; c < cz; ++c) {
+            int aly;
+            const int32_t nk = col_sums(c, aly);
+            if (nk == 0) continue;
+            for (int i = 0; i < cx; ++i) {
+                int32_t ni = 0;
+                for (int j = 0; j < cy; ++j) ni += cell(c, i, j);
+                if (ni == 0) continue;
+                for (int j = 0; j < cy; ++j) {
+                    const int32_t n = cell(c, i, j);
+                    if (n) G += term64(n, ni, nj[j * kGrpMaxTests], nk);
+                }
+            }
+        }
+        R.g2_64 = G;
+        R.exact = 1;
+        R.need_p = (df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
+    }
+    a.res[t] = R;
+    mark_edge(a, T, R.indep != 0, R.need_p != 0);
+}
+
+// One work item = (chunk, row split) of grouped level-1 tests.  The four waves take the
+// split's 32-row steps in turn; each forms the one-hot A rows of (u, z) -- the byte-wise
+// AND of u's and z's one-hot bytes -- and the B columns of the chunk's others from
+// 16-byte loads of the uint8 columns, and accumulates kNA x kGrpB fragments of 32 x 32
+// with v_mfma_i32_32x32x32_i8 (exact int8 products, int32 sums).  The waves' fragments
+// are summed in LDS; a split chunk adds its partial table to the workspace and the
+// workgroup retiring the last split computes the statistics, one thread per test.
+template <bool kPerm, int kNA>
+__global__ __launch_bounds__(kWG) void k_group_count(GroupArgs g)
+{
+    __shared__ int32_t s_tab[kGrpRows * kGrpCols];     // 32 KiB
+    __shared__ int32_t s_nj[kGrpRows * kGrpMaxTests];  // 16 KiB of marginal scratch
+    __shared__ int32_t s_colvar[kGrpCols];
+    __shared__ uint8_t s_colstate[kGrpCols];
+    __shared__ int32_t s_col0[kGrpMaxTests + 1];
+    __shared__ int32_t s_flag;
+    const LaunchArgs &a = g.a;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, l32 = lane & 31;
+    // XCD-aware order: consecutive items (a chunk's splits, a group's chunks) run on one
+    // XCD, whose L2 then serves their shared u and z columns
+    const int64_t per = (g.n_items + kXcds - 1) / kXcds;
+    const int64_t item = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (item >= g.n_items) return;
+    const uint64_t w = g.items[item];
+    const GroupChunk C = g.chunks[(int64_t)(w >> 32)];
+    const int split = (int)(uint32_t)w;
+    const DevInst I = a.inst[C.inst];
+    if (tid == 0) {
+        // a chunk whose tests were all decided in earlier rounds is not counted
+        int live = 0, off = 0;
+        for (int j = 0; j < C.n_t; ++j) {
+            const DevTest T = a.tests[g.gtest[C.t0 + j]];
+            live |= skipped(a, T) ? 0 : 1;
+            s_col0[j] = off;
+            off += (T.x == C.u) ? T.cy : T.cx;
+        }
+        s_col0[C.n_t] = off;
+        s_flag = live;
+    }
+    for (int k = tid; k < kGrpCols; k += kWG) {
+        s_colvar[k] = C.u;  // padding columns: a valid column with a state no byte holds
+        s_colstate[k] = 255;
+    }
+    for (int i = tid; i < kGrpRows * kGrpCols; i += kWG) s_tab[i] = 0;
+    __syncthreads();
+    if (!s_flag) {
+        if (split == 0 && tid < C.n_t) a.res[g.gtest[C.t0 + tid]] = skipped_result();
+        return;
+    }
+    if (tid < C.n_t) {
+        const DevTest T = a.tests[g.gtest[C.t0 + tid]];
+        const int o = (T.x == C.u) ? T.y : T.x;
+        for (int k = s_col0[tid], b = 0; k < s_col0[tid + 1]; ++k, ++b) {
+            s_colvar[k] = o;
+            s_colstate[k] = (uint8_t)b;
+        }
+    }
+    __syncthreads();
+    // this lane's A rows 32 f + l32 = (a, c) with row = c * cu + a, and B columns 32 e + l32
+    uint32_t lau[kNA], hau[kNA], laz[kNA], haz[kNA];
+#pragma unroll
+    for (int f = 0; f < kNA; ++f) {
+        const int r = 32 * f + l32;
+        const bool real = r < C.cu * C.cz;
+        onehot_lut<kPerm>(real ? (uint8_t)(r % C.cu) : (uint8_t)255, lau[f], hau[f]);
+        onehot_lut<kPerm>(real ? (uint8_t)(r / C.cu) : (uint8_t)255, laz[f], haz[f]);
+    }
+    const uint8_t *pu = I.cols + (int64_t)C.u * I.stride + h * 16;
+    const uint8_t *pz = I.cols + (int64_t)C.z * I.stride + h * 16;
+    const uint8_t *pb[kGrpB];
+    uint32_t lb[kGrpB], hb[kGrpB];
+#pragma unroll
+    for (int e = 0; e < kGrpB; ++e) {
+        const int k = 32 * e + l32;
+        pb[e] = I.cols + (int64_t)s_colvar[k] * I.stride + h * 16;
+        onehot_lut<kPerm>(s_colstate[k], lb[e], hb[e]);
+    }
+    v16i acc[kNA][kGrpB];
+#pragma unroll
+    for (int f = 0; f < kNA; ++f)
+#pragma unroll
+        for (int e = 0; e < kGrpB; ++e) acc[f][e] = v16i{0};
+    auto mma = [&](const uint4 &qu, const uint4 &qz, const uint4 *qb, const uint4 *mask) {
+        v4i bo[kGrpB];
+#pragma unroll
+        for (int e = 0; e < kGrpB; ++e) bo[e] = onehot<kPerm>(qb[e], lb[e], hb[e]);
+#pragma unroll
+        for (int f = 0; f < kNA; ++f) {
+            const v4i ou = onehot<kPerm>(qu, lau[f], hau[f]), oz = onehot<kPerm>(qz, laz[f], haz[f]);
+            v4i ao = ou & oz;
+            if (mask) ao = and4(ao, *mask);
+#pragma unroll
+            for (int e = 0; e < kGrpB; ++e) acc[f][e] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ao, bo[e], acc[f][e], 0, 0, 0);
+        }
+    };
+    const int64_t r0 = (int64_t)split * g.rows_per_split;
+    const int64_t r1 = (r0 + g.rows_per_split < I.n_rows) ? r0 + g.rows_per_split : I.n_rows;
+    const int64_t full_end = r0 + (r1 - r0) / 32 * 32;
+    for (int64_t k0 = r0 + 32 * wave; k0 < full_end; k0 += 128) {
+        uint4 qb[kGrpB];
+        const uint4 qu = ld16(pu + k0), qz = ld16(pz + k0);
+#pragma unroll
+        for (int e = 0; e < kGrpB; ++e) qb[e] = ld16(pb[e] + k0);
+        mma(qu, qz, qb, nullptr);
+    }
+    if (full_end < r1 && wave == 0) {
+        // last partial step: bytes past r1 are masked out (their values are arbitrary)
+        const int64_t valid = r1 - (full_end + h * 16);
+        uint4 m;
+        m.x = byte_mask_word(valid, 0);
+        m.y = byte_mask_word(valid, 1);
+        m.z = byte_mask_word(valid, 2);
+        m.w = byte_mask_word(valid, 3);
+        uint4 qb[kGrpB];
+        const uint4 qu = ld16(pu + full_end), qz = ld16(pz + full_end);
+#pragma unroll
+        for (int e = 0; e < kGrpB; ++e) qb[e] = ld16(pb[e] + full_end);
+        mma(qu, qz, qb, &m);
+    }
+    // the waves' fragments summed in LDS
+#pragma unroll
+    for (int f = 0; f < kNA; ++f)
+#pragma unroll
+        for (int e = 0; e < kGrpB; ++e)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = 32 * f + (q >> 2) * 8 + h * 4 + (q & 3);
+                const int32_t v = acc[f][e][q];
+                if (v) atomicAdd(&s_tab[row * kGrpCols + 32 * e + l32], v);
+            }
+    __syncthreads();
+    const int32_t *tab = s_tab;
+    int32_t *gw = (C.n_splits > 1) ? a.ws + C.ws_off : nullptr;
+    if (gw) {
+        for (int i = tid; i < kGrpRows * kGrpCols; i += kWG) {
+            const int32_t v = s_tab[i];
+            if (v) atomicAdd(&gw[i], v);
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t prev = atomicAdd(&a.done[C.done], 1u);
+            s_flag = (prev + 1 == (uint32_t)C.n_splits) ? 1 : 0;
+        }
+        __syncthreads();
+        if (!s_flag) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        tab = gw;
+    }
+    if (tid < C.n_t) {
+        const int t = g.gtest[C.t0 + tid];
+        const DevTest T = a.tests[t];
+        if (skipped(a, T)) a.res[t] = skipped_result();
+        else group_stat(a, t, T, tab, s_col0[tid], C.cu, C.cz, T.x == C.u, s_nj + tid);
+    }
+    if (gw) {
+        // the workspace reads zero again for the next launch
+        __syncthreads();
+        for (int i = tid; i < kGrpRows * kGrpCols; i += kWG) gw[i] = 0;
+        if (tid == 0) a.done[C.done] = 0;
+    }
+}
+
 __global__ __launch_bounds__(kWG) void k_zero(uint32_t *p, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) p[i] = 0u;
 }
 
 }  // namespace
+
+hipError_t launch_group_count(const GroupArgs &g, int n_a, hipStream_t s)
+{
+    if (g.n_items <= 0) return hipSuccess;
+    const int64_t grid = (g.n_items + kXcds - 1) / kXcds * kXcds;
+    const bool perm = g.max_card <= 8;
+    if (n_a <= 1) {
+        if (perm) hipLaunchKernelGGL((k_group_count<true, 1>), dim3((unsigned)grid), dim3(kWG), 0, s, g);
+        else hipLaunchKernelGGL((k_group_count<false, 1>), dim3((unsigned)grid), dim3(kWG), 0, s, g);
+    } else {
+        if (perm) hipLaunchKernelGGL((k_group_count<true, 2>), dim3((unsigned)grid), dim3(kWG), 0, s, g);
+        else hipLaunchKernelGGL((k_group_count<false, 2>), dim3((unsigned)grid), dim3(kWG), 0, s, g);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_zero(uint32_t *p, int64_t n, hipStream_t s)
 {

@@ -121,6 +121,7 @@ constexpr int kGroups = 5;
 constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
 constexpr int64_t kPackMinRows = 262144;  // mean rows per instance from which columns are packed
 constexpr int kGroupPending = kGroups;     // a 97..128-cell table whose group is decided per launch
+constexpr int kGroupMfma = kGroups + 1;    // a level-1 test counted by the grouped MFMA kernel
 
 // Buffers of one in-flight batch.  Two slots let the host build one batch while the
 // device runs the other; the workspace and completion counters are shared, since both
@@ -134,6 +135,14 @@ struct Slot {
     DevBuf<WideItem> d_wide_items;
     DevBuf<WidePart> d_wide_parts;
     DevBuf<WideTest> d_wide_tests;
+    DevBuf<GroupChunk> d_gchunks;  // grouped level-1 chunks, their tests and work items
+    DevBuf<int32_t> d_gtest;
+    DevBuf<uint64_t> d_gitems;
+    HostBuf<GroupChunk> h_gchunks;
+    HostBuf<int32_t> h_gtest;
+    HostBuf<uint64_t> h_gitems;
+    std::vector<int32_t> g_bucket;  // grouping scratch: bucket starts, test order
+    std::vector<int32_t> g_order;
     HostBuf<DevTest> h_tests;
     HostBuf<int32_t> h_zpool;
     HostBuf<uint64_t> h_items;  // test << 32 | (slice * n_chunks + chunk)
@@ -147,6 +156,9 @@ struct Slot {
     hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing: start of each group
     hipEvent_t ev_gend[kGroups] = {};     // and its end (groups may run concurrently)
     hipEvent_t ev_fork = nullptr, ev_join[kGroups] = {};  // concurrent sink launches
+    hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;           // grouped MFMA launches
+    int64_t g_tests = 0, g_chunks = 0, g_items = 0;        // ... of the pending sequence
+    double g_ops = 0.0;
     bool pending = false;
     int64_t n = 0;       // tests of the pending launch sequence
     double bytes = 0.0;  // their algorithmic bytes
@@ -165,6 +177,8 @@ struct Slot {
             if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
         for (hipEvent_t &ev : ev_gend)
             if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+        if ((e = hipEventCreate(&ev_g0)) != hipSuccess) return e;
+        if ((e = hipEventCreate(&ev_g1)) != hipSuccess) return e;
         if ((e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming)) != hipSuccess) return e;
         for (hipEvent_t &ev : ev_join)
             if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
@@ -181,6 +195,12 @@ struct Slot {
         d_wide_items.release();
         d_wide_parts.release();
         d_wide_tests.release();
+        d_gchunks.release();
+        d_gtest.release();
+        d_gitems.release();
+        h_gchunks.release();
+        h_gtest.release();
+        h_gitems.release();
         h_tests.release();
         h_items.release();
         h_zpool.release();
@@ -194,6 +214,8 @@ struct Slot {
         for (hipEvent_t ev : ev_gend)
             if (ev) (void)hipEventDestroy(ev);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_g0) (void)hipEventDestroy(ev_g0);
+        if (ev_g1) (void)hipEventDestroy(ev_g1);
         for (hipEvent_t ev : ev_join)
             if (ev) (void)hipEventDestroy(ev);
     }
@@ -204,6 +226,7 @@ struct HostTest {
     int32_t inst;   // dataset instance
     int64_t z_off;  // into the host z pool
     int32_t edge = -1;  // index of the test's edge in the level (device skip checks), -1 for none
+    int32_t side = 0;   // walk side that produced it: 0 subsets of adj(x), 1 of adj(y)
 };
 
 struct InstHost {
@@ -252,6 +275,17 @@ struct fpgm_ctx {
     const fpgm_level_tune *tune_try = nullptr;
     int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
     uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices
+    // grouped level-1 tests on the MFMA units (k_group_count) for instances of at least
+    // group_min_rows rows: off unless FPGM_GROUP=1.  Measured slower than the LDS sinks on
+    // level 1 at 1000 variables x 10^6 rows (1856 against 122 ms: in the best-grouped
+    // round 22 tests per 128-column chunk, 0.71 POP/s on fragments whose A rows are half
+    // padding, against 0.18 us per test on the lane sinks; later rounds' groups fragment as
+    // edges reach their y side at different times) and on the bootstrap ensemble (2364
+    // against 494 ms); FPGM_GROUP_MIN_ROWS sets the bound
+    int group = 0;
+    int64_t group_min_rows = 0;
+    double grp_ms = 0.0, grp_ops = 0.0;  // grouped launches of the last call: time, int8 operations
+    int64_t grp_tests = 0, grp_launches = 0;
 
     double crit_alpha = -1.0;
     double z_alpha = 0.0;
@@ -406,6 +440,9 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     need.resize((size_t)m);
     fit.resize((size_t)m);
     int too_big = 0;
+    // grouped level-1 counting: instances long enough, no table kept for a parity read, and
+    // the walk sides known (speculative rounds of the level driver)
+    const bool grp_on = ctx->group && ctx->skip_round > 0 && !(tflags & kTFKeep) && N >= ctx->group_min_rows;
     int64_t units[kGroups] = {0, 0, 0, 0, 0};  // per group: the tests' z slices (work items per chunk)
     int64_t units_pending = 0;
     const int nth = (m >= kParallelTests) ? ctx->host_threads : 1;
@@ -441,6 +478,12 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.flags = tflags | (narrow ? kTFNarrow : 0);
         bool fits;
         int gi = lds_group(ctx, T, cells, marg, &fits);
+        if (grp_on && h.d == 1) {
+            // a level-1 test whose (u, z) rows and other columns fit the grouped kernel
+            const int32_t u = h.side ? h.y : h.x, o = h.side ? h.x : h.y;
+            const int64_t cu = ctx->card[u], cz = ctx->card[zpool[h.z_off]];
+            if (cu * cz <= kGrpRows && ctx->card[o] <= kGrpMaxOther) gi = kGroupMfma;
+        }
         if (gi < 0) {
             gi = 4;  // kSinkGlobal
             fits = false;
@@ -462,8 +505,8 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         T.ws_off = -1;
         fit[k] = fits ? 1 : 0;
         gidx[k] = gi;
-        if (gi != kGroupPending) units[gi] += T.n_slices;
-        else ++units_pending;
+        if (gi == kGroupPending) ++units_pending;
+        else if (gi != kGroupMfma) units[gi] += T.n_slices;
         dt[k] = T;
     }
     if (too_big) return fail(ctx, FPGM_ETOOBIG, "contingency table exceeds 2^31 cells");
@@ -509,6 +552,11 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
 #pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
     for (int64_t k = 0; k < m; ++k) {
         DevTest &T = dt[k];
+        if (gidx[k] == kGroupMfma) {  // counted by the grouped kernel: no work items of its own
+            T.n_chunks = 0;
+            need[k] = 0;
+            continue;
+        }
         const int64_t r = rpc[gidx[k]];
         const int64_t n_chunks = std::max<int64_t>(1, (ctx->inst[T.inst].n_rows + r - 1) / r);
         T.n_chunks = (int32_t)n_chunks;
@@ -524,6 +572,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     S.pbytes = 0.0;
     for (; nb < m; ++nb) {
         DevTest &T = dt[nb];
+        if (gidx[nb] == kGroupMfma) {
+            T.item0 = -1;
+            continue;
+        }
         if (need[nb] > 0) {
             if (ws_used + need[nb] > kWsMax && nb > 0) break;
             if (need[nb] > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test workspace exceeds 4 GiB");
@@ -563,6 +615,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     const int nth2 = (nb >= kParallelTests) ? ctx->host_threads : 1;
 #pragma omp parallel for if (nth2 > 1) num_threads(nth2) schedule(static)
     for (int64_t k = 0; k < nb; ++k) {
+        if (gidx[k] == kGroupMfma) continue;
         dt[k].item0 += (int32_t)grp_begin[gidx[k]];
         const int64_t cnt = (int64_t)dt[k].n_chunks * dt[k].n_slices;
         for (int64_t j = 0; j < cnt; ++j) S.h_items.p[dt[k].item0 + j] = ((uint64_t)k << 32) | (uint64_t)j;
@@ -598,6 +651,120 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                                ctx->stream),
                 "upload finalize tests");
     }
+    // grouped level-1 tests: (instance, u, z) groups cut into chunks of at most kGrpCols
+    // one-hot columns and kGrpMaxTests tests, the tests of a group in launch order
+    int64_t n_gc = 0, n_gi = 0, n_gi1 = 0, g_rps = 0;
+    S.g_tests = S.g_chunks = S.g_items = 0;
+    S.g_ops = 0.0;
+    if (grp_on) {
+        std::vector<int32_t> &order = S.g_order;
+        order.clear();
+        for (int64_t k = 0; k < nb; ++k)
+            if (gidx[k] == kGroupMfma) order.push_back((int32_t)k);
+        const int64_t ng = (int64_t)order.size();
+        if (ng > 0) {
+            auto u_of = [&](int32_t k) {
+                const HostTest &h = tests[i0 + k];
+                return h.side ? h.y : h.x;
+            };
+            auto o_of = [&](int32_t k) {
+                const HostTest &h = tests[i0 + k];
+                return h.side ? h.x : h.y;
+            };
+            auto z_of = [&](int32_t k) { return zpool[tests[i0 + k].z_off]; };
+            auto bucket = [&](int32_t k) { return (int64_t)tests[i0 + k].inst * ctx->n_vars + u_of(k); };
+            // counting sort by (instance, u), stable in launch order; then each bucket by z
+            const int64_t nbk = (int64_t)ctx->inst.size() * ctx->n_vars;
+            std::vector<int32_t> &bs = S.g_bucket;
+            bs.assign((size_t)nbk + 1, 0);
+            for (int32_t k : order) ++bs[bucket(k) + 1];
+            for (int64_t b = 0; b < nbk; ++b) bs[b + 1] += bs[b];
+            HIP_TRY(S.h_gtest.ensure(ng, ctx->stream), "alloc pinned group tests");
+            int32_t *sorted = S.h_gtest.p;
+            {
+                std::vector<int32_t> cur(bs.begin(), bs.end() - 1);
+                for (int32_t k : order) sorted[cur[bucket(k)]++] = k;
+            }
+            const int nthg = (ng >= kParallelTests) ? ctx->host_threads : 1;
+#pragma omp parallel for if (nthg > 1) num_threads(nthg) schedule(dynamic, 64)
+            for (int64_t b = 0; b < nbk; ++b)
+                if (bs[b + 1] - bs[b] > 1)
+                    std::stable_sort(sorted + bs[b], sorted + bs[b + 1],
+                                     [&](int32_t p, int32_t q) { return z_of(p) < z_of(q); });
+            // chunks, those with one A fragment (c(u) c(z) <= 32) first
+            HIP_TRY(S.h_gchunks.ensure(ng, ctx->stream), "alloc pinned group chunks");
+            GroupChunk *gc = S.h_gchunks.p;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int64_t p = 0; p < ng;) {
+                    const int32_t k0 = sorted[p];
+                    const int64_t b = bucket(k0);
+                    const int32_t z = z_of(k0), u = u_of(k0);
+                    const int32_t cu = ctx->card[u], cz = ctx->card[z];
+                    int64_t q = p;  // end of the (instance, u, z) group
+                    while (q < ng && bucket(sorted[q]) == b && z_of(sorted[q]) == z) ++q;
+                    if ((cu * cz > 32) != (pass == 1)) {
+                        p = q;
+                        continue;
+                    }
+                    while (p < q) {
+                        GroupChunk G{};
+                        G.inst = tests[i0 + k0].inst;
+                        G.u = u;
+                        G.z = z;
+                        G.cu = cu;
+                        G.cz = cz;
+                        G.t0 = (int32_t)p;
+                        G.ws_off = -1;
+                        while (p < q && G.n_t < kGrpMaxTests && G.n_cols + ctx->card[o_of(sorted[p])] <= kGrpCols) {
+                            G.n_cols += ctx->card[o_of(sorted[p])];
+                            ++G.n_t;
+                            ++p;
+                        }
+                        gc[n_gc++] = G;
+                    }
+                    if (pass == 0) n_gi1 = n_gc;  // chunks with one A fragment so far
+                }
+            // row splits: enough work items to fill the CUs, at least 4096 rows each; split
+            // chunks keep a partial table in the workspace (one done counter each)
+            const int64_t target = (int64_t)ctx->n_cu * ctx->sched.items_per_cu;
+            int64_t splits = std::max<int64_t>(1, (target + n_gc - 1) / n_gc);
+            splits = std::min<int64_t>(splits, std::max<int64_t>(1, N / 4096));
+            if (splits > 1 && ws_used + n_gc * (int64_t)(kGrpRows * kGrpCols) > kWsMax) splits = 1;
+            g_rps = std::max<int64_t>(128, ((N + splits - 1) / splits + 127) / 128 * 128);
+            HIP_TRY(S.h_gitems.ensure(n_gc * splits + 1, ctx->stream), "alloc pinned group items");
+            const int64_t chunks1 = n_gi1;
+            for (int64_t c = 0; c < n_gc; ++c) {
+                GroupChunk &G = gc[c];
+                const int64_t rows = ctx->inst[G.inst].n_rows;
+                G.n_splits = (int32_t)std::max<int64_t>(1, (rows + g_rps - 1) / g_rps);
+                G.done = (int32_t)(nb + c);
+                if (G.n_splits > 1) {
+                    G.ws_off = ws_used;
+                    ws_used += kGrpRows * kGrpCols;
+                }
+                for (int32_t sp = 0; sp < G.n_splits; ++sp) {
+                    S.h_gitems.p[n_gi++] = ((uint64_t)c << 32) | (uint64_t)sp;
+                    const int64_t r = std::min<int64_t>(g_rps, rows - (int64_t)sp * g_rps);
+                    S.g_ops += 2.0 * (double)std::max<int64_t>(r, 0) * (G.cu * G.cz > 32 ? 64 : 32) * kGrpCols;
+                }
+                if (c + 1 == chunks1) n_gi1 = n_gi;
+            }
+            if (chunks1 == 0) n_gi1 = 0;
+            S.g_tests = ng;
+            S.g_chunks = n_gc;
+            S.g_items = n_gi;
+            HIP_TRY(S.d_gtest.ensure(ng, false, ctx->stream), "alloc group tests");
+            HIP_TRY(S.d_gchunks.ensure(n_gc, false, ctx->stream), "alloc group chunks");
+            HIP_TRY(S.d_gitems.ensure(n_gi, false, ctx->stream), "alloc group items");
+            HIP_TRY(hipMemcpyAsync(S.d_gtest.p, S.h_gtest.p, sizeof(int32_t) * ng, hipMemcpyHostToDevice, ctx->stream),
+                    "upload group tests");
+            HIP_TRY(hipMemcpyAsync(S.d_gchunks.p, gc, sizeof(GroupChunk) * n_gc, hipMemcpyHostToDevice, ctx->stream),
+                    "upload group chunks");
+            HIP_TRY(hipMemcpyAsync(S.d_gitems.p, S.h_gitems.p, sizeof(uint64_t) * n_gi, hipMemcpyHostToDevice,
+                                   ctx->stream),
+                    "upload group items");
+        }
+    }
     HIP_TRY(S.h_zpool.ensure(std::max<int64_t>(zlen, 1), ctx->stream), "alloc pinned z pool");
     if (zlen > 0) {
         const int nthz = (zlen >= 4 * kParallelTests) ? ctx->host_threads : 1;
@@ -612,7 +779,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     HIP_TRY(S.d_items.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
     if (ctx->skip_round > 0) HIP_TRY(S.d_items_live.ensure(std::max<int64_t>(ni, 1), false, ctx->stream), "alloc items");
     HIP_TRY(ctx->d_ws.ensure(std::max<int64_t>(ws_used, kWsInit), true, ctx->stream), "alloc workspace");
-    HIP_TRY(ctx->d_done.ensure(std::max<int64_t>(nb, 1024), true, ctx->stream), "alloc counters");
+    HIP_TRY(ctx->d_done.ensure(std::max<int64_t>(nb + n_gc, 1024), true, ctx->stream), "alloc counters");
     HIP_TRY(S.d_res.ensure(nb, false, ctx->stream), "alloc results");
     if (!S.d_queue.p) {  // zero once here; afterwards every sequence's k_pvalue re-zeroes them
         HIP_TRY(S.d_queue.ensure(kGroups * (kXcds + 1), false, ctx->stream), "alloc work queues");
@@ -664,6 +831,23 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         c.count = live_count;
         HIP_TRY(launch_compact(c, max_len, ctx->stream), "launch compact");
         a.items = S.d_items_live.p;
+    }
+    if (n_gc > 0) {
+        GroupArgs ga{};
+        ga.a = a;
+        ga.chunks = S.d_gchunks.p;
+        ga.gtest = S.d_gtest.p;
+        ga.rows_per_split = g_rps;
+        ga.max_card = *std::max_element(ctx->card.begin(), ctx->card.end());
+        HIP_TRY(hipEventRecord(S.ev_g0, ctx->stream), "event");
+        ga.items = S.d_gitems.p;
+        ga.n_items = n_gi1;
+        HIP_TRY(launch_group_count(ga, 1, ctx->stream), "launch group count");
+        ga.items = S.d_gitems.p + n_gi1;
+        ga.n_items = n_gi - n_gi1;
+        HIP_TRY(launch_group_count(ga, 2, ctx->stream), "launch group count");
+        HIP_TRY(hipEventRecord(S.ev_g1, ctx->stream), "event");
+        ctx->k_launches += (n_gi1 > 0) + (n_gi > n_gi1);
     }
     // one launch per sink group.  The LDS sinks (lane, lane-wide, shared) run one after
     // another on the context stream: run together they compete for the same CUs' LDS and
@@ -725,6 +909,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             G.items = grp_len[gi];
         }
         for (int64_t k = 0; k < nb; ++k) {
+            if (gidx[k] == kGroupMfma) continue;
             Slot::GroupInfo &G = S.grp[gidx[k]];
             const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
             G.tests += 1;
@@ -753,6 +938,18 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
     ctx->k_ms += ms;
     ctx->k_bytes += S.bytes;
     ctx->k_pbytes += S.pbytes;
+    if (S.g_chunks > 0) {
+        float gms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&gms, S.ev_g0, S.ev_g1), "event time");
+        ctx->grp_ms += gms;
+        ctx->grp_ops += S.g_ops;
+        ctx->grp_tests += S.g_tests;
+        ctx->grp_launches += 1;
+        if (ctx->trace)
+            fprintf(stderr, "[fpgm]   launch grouped   tests %7lld chunks %7lld items %8lld: %8.3f ms, %6.1f TOP/s int8\n",
+                    (long long)S.g_tests, (long long)S.g_chunks, (long long)S.g_items, gms,
+                    gms > 0 ? S.g_ops / (gms * 1e9) : 0.0);
+    }
     if (ctx->trace) {
         static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
         for (int gi = 0; gi < kGroups; ++gi) {
@@ -802,6 +999,10 @@ void reset_kernel_stats(fpgm_ctx *ctx)
     ctx->g_ops = 0.0;
     ctx->g_pair_ms = 0.0;
     ctx->g_launches = 0;
+    ctx->grp_ms = 0.0;
+    ctx->grp_ops = 0.0;
+    ctx->grp_tests = 0;
+    ctx->grp_launches = 0;
 }
 
 // Level 0 of one instance on the MFMA units: C = O^T O, then one thread per pair, every
@@ -1090,6 +1291,8 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
             for (int g = 0; g < kGroups; ++g) ctx->sched.wpc[g] = std::min<int64_t>(64, std::max<int64_t>(1, v[g]));
     }
     if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_GROUP")) ctx->group = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1668,7 +1871,7 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
                 int64_t We = W;
                 if (!plain_round && ctx->sched.promote > 0 && w.tests >= ctx->sched.promote) We = INT64_MAX;
                 for (int64_t k = 0; k < We && w.side < 2; ++k) {
-                    G.tv.push_back(HostTest{w.x, w.y, d, active[g].first, (int64_t)G.z.size(), (int32_t)g});
+                    G.tv.push_back(HostTest{w.x, w.y, d, active[g].first, (int64_t)G.z.size(), (int32_t)g, w.side});
                     for (int i = 0; i < d; ++i) G.z.push_back(I.F.at(w.u, w.skip, c[i]));
                     if (!next_choice(w.a, d, c)) start_side(w, I.F, d, w.side + 1, c);
                 }
@@ -2353,6 +2556,17 @@ fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *i
     if (int8_ops) *int8_ops = ctx->g_ops;
     if (pairs_ms) *pairs_ms = ctx->g_pair_ms;
     if (launches) *launches = ctx->g_launches;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_last_group_stats(const fpgm_ctx *ctx, double *ms, double *int8_ops, int64_t *tests,
+                                  int64_t *sequences)
+{
+    if (!ctx) return FPGM_EINVAL;
+    if (ms) *ms = ctx->grp_ms;
+    if (int8_ops) *int8_ops = ctx->grp_ops;
+    if (tests) *tests = ctx->grp_tests;
+    if (sequences) *sequences = ctx->grp_launches;
     return FPGM_OK;
 }
 

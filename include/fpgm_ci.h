@@ -286,6 +286,13 @@ fpgm_status fpgm_last_packed_bytes(const fpgm_ctx *ctx, double *packed_bytes);
 fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *int8_ops, double *pairs_ms,
                                  int64_t *launches);
 
+/* Grouped level-1 counting of the last fpgm_pc_skeleton call (tests sharing the walking
+ * endpoint and the conditioning node, counted as one-hot int8 products on the MFMA units):
+ * kernel time, int8 operations of the padded 32 x 32 x 32 fragments, tests counted there,
+ * launch sequences that had such tests.  No counterpart in the reference. */
+fpgm_status fpgm_last_group_stats(const fpgm_ctx *ctx, double *ms, double *int8_ops, int64_t *tests,
+                                  int64_t *sequences);
+
 #ifdef __cplusplus
 }
 #endif

@@ -349,6 +349,35 @@ def test_skeleton_concurrent_sink_launches_same_results(monkeypatch, key, alpha,
         _assert_same(g, o)
 
 
+@pytest.mark.parametrize("case", ["dag-1000v-30k", "cards-6-12", "cards-2-8-small"])
+def test_grouped_level1_mfma_path(monkeypatch, case):
+    # level 1's tests that share the walking endpoint u and the conditioning node z are
+    # counted as one-hot int8 products on the MFMA units (FPGM_GROUP, on by default); the
+    # stage equals the oracle's with the path on and off.  The cases: the 1000-variable
+    # network (chunks of 128 columns cut from large groups, row splits), cardinalities 6-12
+    # (byte-comparison one-hot form, 36..64 A rows: two fragments), and a small sample whose
+    # rows end inside a 32-row step
+    if case == "dag-1000v-30k":
+        w = configs.get(3)
+        s, alpha, depth = w.sample(30_011), w.alpha, 2
+    elif case == "cards-6-12":
+        net = datagen.make_network(77, 40, 90, 4, ("uniform", 6, 12), 0.4)
+        s, alpha, depth = datagen.forward_sample(net, 50_003, 77), 0.05, 3
+    else:
+        net = datagen.make_network(78, 30, 60, 4, ("uniform", 2, 8), 0.4)
+        s, alpha, depth = datagen.forward_sample(net, 4_093, 78), 0.05, 3
+    o = O.pc_skeleton(O.Data.of(s), alpha, depth, n_threads=8)
+    out = {}
+    for on in ("1", "0"):
+        monkeypatch.setenv("FPGM_GROUP", on)
+        with _ctx(s) as ctx:
+            out[on] = ctx.pc_skeleton(alpha, depth)
+            out[on]["group"] = ctx.group_stats()
+        _assert_same(out[on], o)
+    assert out["1"]["group"]["tests"] > 0 and out["0"]["group"]["tests"] == 0
+    assert out["1"]["tests_per_level"][1] > 0
+
+
 def test_skeleton_depth_and_alpha():
     w = configs.get(2)
     s = w.sample(20_000)
