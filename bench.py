@@ -7,18 +7,33 @@ size 1, speculative tests excluded) over the timed steps.
 
     python bench.py [--steps K] [--warmup W] [--workload 3]
 
-Prints one JSON line.  Beside the metric it reports the plain one-launch-per-test form
-(first tests of every level, sampled, the rest batched so the stage stays exact), the
-counting kernel's achieved bandwidth against HBM peak, and the CPU restatement timed on
-a bounded sample of the same levels on the host cores.
+Prints one JSON line.  Beside the metric it reports:
+  roofline   the dominant counting kernel against HBM: bytes from rocprofv3 FETCH_SIZE /
+             WRITE_SIZE counters read in two child runs of this bench (one CI stage each,
+             before this process touches the GPU), scaled by a 1 GiB copy; the algorithmic
+             rate ((|Z|+2) N bytes per test, columns shared through L2) kept apart as
+             "effective"; and every sink against the resource that bounds it (LDS atomics,
+             device atomics, int8 MFMA), with peaks measured by probe kernels;
+  plain      the one-launch-per-test form (first tests of every level, sampled);
+  cpu_baseline  the CPU restatement on a bounded sample of every level on the host cores,
+             with one thread, and under the reference's 128-edge schedule;
+  g2_error   FP32 against FP64 G^2 over a random sample of tests of the stage's shape;
+  oracle_fixture  the stage against the oracle's committed full-size result.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
+from collections import defaultdict
 
 import numpy as np
 
@@ -28,6 +43,16 @@ sys.path.insert(0, ROOT)
 from fastpgm_amd import _lib, configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_OPS = 5.0e15   # dense int8 MFMA, MI355X
+SINKS = ["lane", "lane-wide", "shared", "slice", "device"]
+# k_count_stat's template argument (the sink kind) -> the sink the library reports it under
+SINK_OF_KIND = {"2": "lane", "3": "lane-wide", "5": "lane-wide", "0": "shared", "4": "slice", "1": "device"}
+SINK_BOUND = {"lane": ("lds_atomics", "LDS atomic adds into lane-private 16-bit counters, one per row and test"),
+              "lane-wide": ("lds_atomics", "LDS atomic adds into lane-private 16-bit counters, one per row and test"),
+              "shared": ("lds_atomics", "LDS atomic adds into shared histogram copies, one per row and test"),
+              "slice": ("lds_atomics", "LDS atomic adds into a z slice, one per row and test"),
+              "device": ("dev_atomics", "device-memory atomic adds, one per row and test")}
+TUNE_DIR = os.path.join(ROOT, "profiles")
 
 
 def log(msg):
@@ -47,10 +72,31 @@ def adjacency_before_levels(n, sepsets, n_levels):
     return out
 
 
+def cpu_info():
+    """Host CPU: logical CPUs of the machine, those this process may run on, the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
 def cpu_baseline(sample, w, g, seconds, threads):
     """The oracle (CPU restatement) on a bounded sample of every level: the first edges of
     the level's frozen adjacency, sized to about seconds / levels each; the stage's CPU
-    time is then extrapolated level by level from the measured rates."""
+    time is then extrapolated level by level from the measured rates.  On the level with
+    the most tests the same edges are also run with one thread and under the reference's
+    own schedule (128-edge batches of one test per edge, PCStable.cpp:275-324), and the
+    stage estimate is scaled by each variant's rate there."""
     from oracle import oracle as O
     D = O.Data.of(sample)
     L = len(g["tests_per_level"])
@@ -60,6 +106,7 @@ def cpu_baseline(sample, w, g, seconds, threads):
     # the level with the most tests gets what the others leave of the budget
     main = int(np.argmax(g["tests_per_level"]))
     order = [l for l in range(L) if l != main] + [main]
+    main_run = None
     for lvl in order:
         tests_l = g["tests_per_level"][lvl]
         if tests_l == 0:
@@ -81,12 +128,37 @@ def cpu_baseline(sample, w, g, seconds, threads):
         est += tests_l / rate
         sampled += k
         sampled_s += dt
+        if lvl == main:
+            main_run = (m, k, dt)
         log(f"cpu level {lvl}: {k} tests on {m} edges in {dt:.2f} s ({rate:.0f} tests/s)")
     value = g["num_ci_test"] / est if est > 0 else None
-    return {"value": value, "unit": "CI tests/s", "cores": threads, "kind": "port",
-            "sample": f"oracle CheckEdge on the first edges of each level's frozen adjacency: {sampled} tests "
-                      f"in {sampled_s:.1f} s on {threads} threads, stage time extrapolated per level "
-                      f"(est. {est:.1f} s for {g['num_ci_test']} tests)"}
+    out = {"value": value, "unit": "CI tests/s", "cores": threads, "kind": "port",
+           "sample": f"oracle CheckEdge on the first edges of each level's frozen adjacency: {sampled} tests "
+                     f"in {sampled_s:.1f} s on {threads} threads (dynamic schedule over edges), stage time "
+                     f"extrapolated per level (est. {est:.1f} s for {g['num_ci_test']} tests)",
+           "host": cpu_info()}
+    if main_run and main > 0 and value:
+        m, k, dt = main_run
+        rate_all = k / dt
+        variants = {}
+        # one thread: about a quarter of the budget, on the main level's first edges
+        m1 = max(1, min(m, int(m * 0.25 * seconds / max(dt * threads, 1e-3))))
+        t0 = time.perf_counter()
+        k1, _ = O.level_sample(D, w.alpha, main, adjs[main], m1, 1)
+        dt1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        kr, _ = O.level_sample_ref128(D, w.alpha, main, adjs[main], m, threads)
+        dtr = time.perf_counter() - t0
+        for name, kk, dd, th, what in (("threads_1", k1, dt1, 1, f"{m1} edges"),
+                                       ("reference_schedule", kr, dtr, threads,
+                                        f"{m} edges in 128-edge batches of one test per edge")):
+            r = kk / dd if dd > 0 and kk > 0 else None
+            variants[name] = {"value": value * r / rate_all if r else None, "cores": th,
+                              "sample": f"level {main}: {kk} tests on {what} in {dd:.2f} s ({r:.0f} tests/s against "
+                                        f"{rate_all:.0f} with {threads} threads, dynamic schedule)"}
+            log(f"cpu level {main}, {name}: {kk} tests in {dd:.2f} s ({r:.0f} tests/s)")
+        out.update(variants)
+    return out
 
 
 def bench_ensemble(args, w):
@@ -104,7 +176,8 @@ def bench_ensemble(args, w):
     for i in range(args.warmup):
         g = ctx.pc_skeleton_multi(w.alpha, depth, want_adj=False)
         log(f"warmup {i}: {g['num_ci_test']} tests in {g['ci_seconds']:.3f} s")
-    total, kernel_ms, alg_bytes, launches = 0, 0.0, 0.0, 0
+    total, kernel_ms, alg_bytes, packed_bytes, launches = 0, 0.0, 0.0, 0.0, 0
+    sinks = {k: defaultdict(float) for k in SINKS}
     t_start = time.perf_counter()
     for i in range(args.steps):
         g = ctx.pc_skeleton_multi(w.alpha, depth, want_adj=False)
@@ -112,12 +185,17 @@ def bench_ensemble(args, w):
         total += g["num_ci_test"]
         kernel_ms += ks["kernel_ms"]
         alg_bytes += ks["alg_bytes"]
+        packed_bytes += ks["packed_bytes"]
         launches += ks["launches"]
+        for k, v in ctx.sink_stats().items():
+            for f, x in v.items():
+                sinks[k][f] += x
     elapsed = time.perf_counter() - t_start
     value = total / elapsed
     freq = g["edge_freq"]
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {value:.0f} tests/s; per level {g['tests_per_level']}")
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    roof = build_roofline(sinks, args.steps, None, probe_peaks(), None, kernel_ms, alg_bytes, packed_bytes, launches)
+    roof["traffic_source"] = "HBM counters not read for the ensemble (its columns, 1.3 GB in all, are read through L2)"
     out = {
         "metric": "CI tests/sec (reference-equivalent, PC-stable CI stage, G2)",
         "value": value, "unit": "CI tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -130,10 +208,7 @@ def bench_ensemble(args, w):
         "num_ci_test": g["num_ci_test"], "tests_per_level": g["tests_per_level"],
         "executed_per_level": g["executed_per_level"],
         "edge_freq_sum": int(freq.sum()), "edge_freq_max": int(freq.max()),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_count_stat",
-                     "kernel_ms_per_step": kernel_ms / args.steps, "launches_per_step": launches / args.steps,
-                     "alg_bytes_per_step": alg_bytes / args.steps},
+        "roofline": roof,
     }
     if not args.no_plain:
         gp = ctx.pc_skeleton_multi(w.alpha, depth, mode=_lib.FPGM_MODE_PLAIN, plain_sample=args.plain_sample,
@@ -163,84 +238,237 @@ def bench_ensemble(args, w):
     print(json.dumps(out), flush=True)
 
 
-def hbm_traffic_from_counters(fetch_csv, write_csv, stages, copy_bytes=1 << 30):
-    """HBM bytes moved by the counting kernels per CI stage, from rocprofv3 --pmc
-    FETCH_SIZE and WRITE_SIZE counter_collection.csv files (KiB per dispatch) of a
-    bench run made with --copy-calibration that ran `stages` CI stages: each counter
-    is scaled by copy_bytes over what it read for the largest copy dispatch (the
-    calibration copy), then divided by the stage count."""
-    import csv
-
-    def totals(path):
-        rows = list(csv.DictReader(open(path)))
-        copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
-        kern = [float(r["Counter_Value"]) for r in rows if "k_count_stat" in r["Kernel_Name"]]
-        scale = copy_bytes / (copy * 1024.0)
-        return sum(kern) * 1024.0 * scale / stages, len(kern) / stages, scale
-
-    f, n, ff = totals(fetch_csv)
-    wr, _, wf = totals(write_csv)
-    return {"fetch_bytes": f, "write_bytes": wr, "dispatches": n, "fetch_scale": ff, "write_scale": wf,
-            "bytes_per_launch": (f + wr) / max(n, 1)}
-
-
-def summarize_pmc(pmc_dir):
-    """Write the counting kernels' per-stage HBM traffic into profiles/hbm_traffic.json
-    from a directory holding fetch/ and write/ (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE
-    output) and run.json (the JSON line the profiled bench run printed).  No GPU use."""
-    import glob
-    run = json.load(open(os.path.join(pmc_dir, "run.json")))
-    fc = glob.glob(os.path.join(pmc_dir, "fetch", "*counter_collection.csv"))
-    wc = glob.glob(os.path.join(pmc_dir, "write", "*counter_collection.csv"))
-    if not fc or not wc:
-        raise SystemExit(f"no counter_collection.csv under {pmc_dir}/fetch and {pmc_dir}/write")
-    t = hbm_traffic_from_counters(fc[0], wc[0], run["stages_in_run"])
-    rl = run["roofline"]
-    alg, kms = rl["alg_bytes_per_step"], rl["kernel_ms_per_step"]
-    t.update({"alg_bytes": alg, "traffic_over_alg": (t["fetch_bytes"] + t["write_bytes"]) / alg,
-              "kernel_ms_step": kms,
-              "hbm_gbs_measured": (t["fetch_bytes"] + t["write_bytes"]) / (kms * 1e-3) / 1e9})
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    rec = json.load(open(path)) if os.path.exists(path) else {}
-    name = run["config"]["workload"]
-    rec[name] = t
-    with open(path, "w") as fh:
-        json.dump(rec, fh, indent=1)
-        fh.write("\n")
-    lines = [
-        f"k_count_stat -- HBM traffic, workload {name}, per CI stage",
-        "Commands (GPU box, repository root, TMPDIR=/tmp), one counter per run, tuning records from a",
-        "bench run with --save-tune:",
-        "  rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/pmc_w3/fetch -o w3 -- python bench.py --steps 1 \\",
-        "      --warmup 0 --no-cpu --no-plain --copy-calibration --tune-file gpurun_out/tune_w3.json",
-        "  rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_w3/write -o w3 -- (the same bench command)",
-        "  (the FETCH run's JSON line saved as gpurun_out/pmc_w3/run.json), then, on any host:",
-        "  python bench.py --summarize-pmc gpurun_out/pmc_w3     -> profiles/hbm_traffic.json and this file",
-        f"Each profiled run executes {run['stages_in_run']} CI stages; totals are divided by that.",
-        "Calibration: --copy-calibration runs one 1 GiB device-to-device hipMemcpy (__amd_rocclr_copyBuffer);",
-        f"  each counter is scaled by 1 GiB over what it read for that copy: fetch scale "
-        f"{t['fetch_scale']:.4f}, write scale {t['write_scale']:.4f}.",
-        f"Counting kernels (k_count_stat<0..4>), {t['dispatches']:.0f} dispatches per stage:",
-        f"  fetch  {t['fetch_bytes']:.4g} bytes, write {t['write_bytes']:.4g} bytes (scaled)",
-        f"  total  {t['fetch_bytes'] + t['write_bytes']:.4g} bytes = {t['traffic_over_alg']:.3f} x the algorithmic "
-        f"bytes {alg:.4g}",
-        f"  per dispatch {t['bytes_per_launch']:.4g} bytes",
-        f"  kernel time {kms:.1f} ms per stage -> measured HBM rate {t['hbm_gbs_measured']:.0f} GB/s of 8000.",
-        "Reading: the traffic is below the algorithmic bytes because the kernels read the packed",
-        f"columns ({run['roofline'].get('packed', {}).get('bytes_per_step', 0):.4g} bytes per stage at 2, 4 "
-        "or 8 bits per value by cardinality) and tests",
-        "launched together share columns (every level-1 test of an edge scans x and y; concurrent tests",
-        "share z columns): the second read of a column comes from L2 / MALL.  HBM does not bound these",
-        "kernels: the per-row work (unpacking, key arithmetic and one LDS read-modify-write per row and",
-        "test) does -- the SQ counter profile has the instruction mix.",
-    ]
-    with open(os.path.join(ROOT, "profiles", f"k_count_stat_{name}_hbm_traffic.txt"), "w") as fh:
-        fh.write("\n".join(lines) + "\n")
-    print(json.dumps({name: t}, indent=1))
-
-
 def _kernel_short(name):
     return name.replace("void ", "").replace("fpgm::(anonymous namespace)::", "").split("(")[0]
+
+
+def kernel_family(name):
+    """The sink (or other kernel) a profiled kernel name belongs to."""
+    k = _kernel_short(name)
+    if k.startswith("k_count_stat<"):
+        return SINK_OF_KIND.get(k[len("k_count_stat<"):].split(",")[0].strip(), "other")
+    if k.startswith("k_fin_wide"):
+        return "device"
+    if "copyBuffer" in name:
+        return "copy"
+    return k.split("<")[0]
+
+
+def pmc_by_family(csv_path, copy_bytes=1 << 30):
+    """Bytes per kernel family from a rocprofv3 --pmc counter_collection.csv (one counter,
+    KiB per dispatch), scaled by copy_bytes over what the counter read for the calibration
+    copy (the largest __amd_rocclr_copyBuffer dispatch)."""
+    rows = list(csv.DictReader(open(csv_path)))
+    copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+    scale = copy_bytes / (copy * 1024.0)
+    fam = defaultdict(lambda: [0.0, 0])
+    for r in rows:
+        f = kernel_family(r["Kernel_Name"])
+        fam[f][0] += float(r["Counter_Value"]) * 1024.0 * scale
+        fam[f][1] += 1
+    return {k: {"bytes": v[0], "dispatches": v[1]} for k, v in fam.items()}, scale
+
+
+def tune_file_for(w):
+    return os.path.join(TUNE_DIR, f"tune_{w.name}.json")
+
+
+def traffic_child(args, w):
+    """Child of measure_traffic, run under rocprofv3 --pmc: the calibration copy, then one CI
+    stage with the committed tuning records; prints the per-sink figures as JSON."""
+    copy_calibration()
+    sample = w.sample()
+    ctx = _lib.Context.of(sample)
+    if args.tune_file and os.path.exists(args.tune_file):
+        with open(args.tune_file) as f:
+            ctx.tune_set(json.load(f)["levels"])
+    _, st = ctx.run_stage(w.alpha, w.depth)
+    out = {"stages": 1, "num_ci_test": int(st.num_ci_test), "sinks": ctx.sink_stats(), "gram": ctx.gram_stats()}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def measure_traffic(args, w):
+    """HBM bytes per kernel family of one CI stage, measured with rocprofv3 --pmc FETCH_SIZE
+    and then WRITE_SIZE in two child runs of this script (--traffic-child).  Runs before this
+    process makes any GPU call.  Returns (result, None) or (None, reason)."""
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None, "rocprofv3 not found"
+    base = tempfile.mkdtemp(prefix="fpgm_pmc_", dir="/tmp")
+    tune = tune_file_for(w)
+    fams, scales, child = {}, {}, None
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(base, counter)
+            cmd = [exe, "--pmc", counter, "-f", "csv", "-d", d, "-o", "run", "--", sys.executable,
+                   os.path.abspath(__file__), "--traffic-child", "--workload", str(w.key)]
+            if os.path.exists(tune):
+                cmd += ["--tune-file", tune]
+            log(f"traffic: rocprofv3 --pmc {counter}, one CI stage in a child process")
+            t0 = time.perf_counter()
+            p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                                 env={**os.environ, "TMPDIR": "/tmp"}, start_new_session=True)
+            try:
+                out, err = p.communicate(timeout=args.traffic_timeout)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                return None, f"rocprofv3 --pmc {counter} exceeded {args.traffic_timeout} s"
+            lines = [ln for ln in out.splitlines() if ln.startswith("{")]
+            csvs = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            # rocprofv3 may end with a fault of its own after the program has finished: the
+            # program's last line and the counter file decide
+            if not lines or not csvs:
+                tail = (err or "").strip().splitlines()[-3:]
+                return None, f"rocprofv3 --pmc {counter}: no result (rc {p.returncode}): {' | '.join(tail)}"
+            child = json.loads(lines[-1])
+            fams[counter], scales[counter] = pmc_by_family(csvs[0])
+            log(f"traffic: {counter} done in {time.perf_counter() - t0:.1f} s")
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+    fam = {}
+    for k in set(fams["FETCH_SIZE"]) | set(fams["WRITE_SIZE"]):
+        f = fams["FETCH_SIZE"].get(k, {"bytes": 0.0, "dispatches": 0})
+        wr = fams["WRITE_SIZE"].get(k, {"bytes": 0.0, "dispatches": 0})
+        fam[k] = {"fetch_bytes": f["bytes"], "write_bytes": wr["bytes"], "bytes": f["bytes"] + wr["bytes"],
+                  "dispatches": max(f["dispatches"], wr["dispatches"])}
+    return {"per_stage": fam, "fetch_scale": scales["FETCH_SIZE"], "write_scale": scales["WRITE_SIZE"],
+            "child_sinks": child["sinks"], "tune_file": os.path.relpath(tune, ROOT) if os.path.exists(tune) else None,
+            "source": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each over one CI stage in a child run of "
+                      "this bench; every counter scaled by 1 GiB over what it read for a 1 GiB device copy"}, None
+
+
+def probe_peaks():
+    """Measured peaks of the resources the sinks are bound by (fastpgm_amd/libfpgm_probe.so)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "fastpgm_amd", "libfpgm_probe.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.fpgm_probe_lds_atomics.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.fpgm_probe_dev_atomics.argtypes = [C.c_int64, C.POINTER(C.c_double)]
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    if L.fpgm_probe_lds_atomics(C.byref(a), C.byref(b)) or L.fpgm_probe_dev_atomics(1 << 20, C.byref(c)):
+        return None
+    return {"lds_atomics": a.value, "lds_atomics_per_cu_clock": b.value, "dev_atomics": c.value,
+            "probes": "k_probe_lds_atomics (lane-private [cell][lane] words, 16 adds per step), "
+                      "k_probe_dev_atomics (scattered adds into a 4 MiB table)"}
+
+
+def build_roofline(sinks, steps, traffic, peaks, gram, kernel_ms, alg_bytes, packed_bytes, launches):
+    """The roofline object: the dominant counting kernel against HBM with measured bytes, its
+    algorithmic rate apart, and each sink against the resource that bounds it."""
+    lines = {}
+    for name in SINKS:
+        S = sinks.get(name)
+        if not S or S["launches"] == 0 or S["ms"] <= 0:
+            continue
+        ms = S["ms"] / steps
+        sec = ms * 1e-3
+        L = {"kernel": f"k_count_stat ({name} sink)", "launches": S["launches"] / steps, "ms": ms,
+             "tests": S["tests"] / steps, "alg_bytes": S["alg_bytes"] / steps,
+             "effective_gbs": S["alg_bytes"] / steps / sec / 1e9,
+             "packed_gbs": S["packed_bytes"] / steps / sec / 1e9}
+        fam = (traffic or {}).get("per_stage", {}).get(name)
+        if fam:
+            L["hbm_bytes"] = fam["bytes"]
+            L["hbm_gbs"] = fam["bytes"] / sec / 1e9
+            L["hbm_frac"] = L["hbm_gbs"] / HBM_PEAK_GBS
+        res, what = SINK_BOUND[name]
+        rate = S["row_tests"] / steps / sec
+        peak = (peaks or {}).get(res)
+        L["bound"] = {"resource": what, "achieved": rate, "unit": "adds/s", "peak": peak,
+                      "frac": rate / peak if peak else None}
+        lines[name] = L
+    if gram and gram.get("launches"):
+        g_ms = gram["gram_ms"]
+        lines["gram"] = {"kernel": "k_gram_l0 (level 0)", "launches": gram["launches"], "ms": g_ms,
+                         "bound": {"resource": "int8 MFMA (v_mfma_i32_32x32x32_i8)",
+                                   "achieved": gram["int8_ops"] / (g_ms * 1e-3), "unit": "OP/s",
+                                   "peak": INT8_PEAK_OPS, "frac": gram["int8_ops"] / (g_ms * 1e-3) / INT8_PEAK_OPS}}
+    dom = max((k for k in lines if k != "gram"), key=lambda k: lines[k]["ms"], default=None)
+    roof = {"bound": "hbm", "kernel": lines[dom]["kernel"] if dom else "k_count_stat", "unit": "GB/s",
+            "peak": HBM_PEAK_GBS, "achieved": None, "frac": None, "traffic": None}
+    if dom:
+        D = lines[dom]
+        per_launch = D["ms"] / max(D["launches"], 1e-9)
+        roof["avg_launch_ms"] = per_launch
+        roof["alg_bytes_per_launch"] = D["alg_bytes"] / max(D["launches"], 1e-9)
+        if "hbm_bytes" in D:
+            fam = traffic["per_stage"][dom]
+            roof["traffic"] = fam["bytes"] / max(fam["dispatches"], 1)  # the profiled run's own dispatches
+            roof["achieved"] = D["hbm_gbs"]
+            roof["frac"] = D["hbm_frac"]
+        roof["effective"] = {"achieved": D["effective_gbs"], "frac": D["effective_gbs"] / HBM_PEAK_GBS,
+                             "note": "algorithmic (|Z|+2) N bytes per test over the launch time; tests launched "
+                                     "together read their columns through L2, so this is not an HBM rate"}
+        roof["limit"] = D["bound"]
+    total_hbm = sum(v["bytes"] for k, v in (traffic or {}).get("per_stage", {}).items() if k in SINKS)
+    roof["stage"] = {"counting_ms": kernel_ms / steps, "launches": launches / steps,
+                     "alg_bytes": alg_bytes / steps, "packed_bytes": packed_bytes / steps,
+                     "hbm_bytes": total_hbm or None,
+                     "hbm_gbs": total_hbm / (kernel_ms / steps * 1e-3) / 1e9 if total_hbm else None}
+    roof["sinks"] = lines
+    roof["traffic_source"] = (traffic or {}).get("source")
+    roof["peaks"] = peaks
+    return roof
+
+
+def g2_error_sample(ctx, sample, w, g, n=20000, seed=7):
+    """FP32 G^2 against FP64 over a random sample of tests shaped like the stage's: a level
+    drawn in proportion to its executed tests, an edge of that level's frozen adjacency, a
+    random conditioning set of that size from adj(x) minus y.  The same tests run once as
+    the stage runs them (FP32, FP64 only inside the recheck band) and once in FP64."""
+    rng = np.random.default_rng(seed)
+    L = len(g["tests_per_level"])
+    adjs = adjacency_before_levels(sample.n_vars, g["sepsets"], L)
+    ex = np.asarray(g["executed_per_level"], dtype=np.float64)
+    p = ex / ex.sum()
+    edges = [np.argwhere(np.triu(a, 1)) for a in adjs]
+    tests = []
+    while len(tests) < n:
+        d = int(rng.choice(L, p=p))
+        if len(edges[d]) == 0:
+            continue
+        x, y = (int(v) for v in edges[d][rng.integers(len(edges[d]))])
+        nb = [v for v in np.flatnonzero(adjs[d][x]) if v != y]
+        if len(nb) < d:
+            continue
+        tests.append((x, y, tuple(sorted(int(v) for v in rng.choice(nb, d, replace=False)))))
+    r32 = ctx.ci_batch(tests, alpha=w.alpha, flags=0)
+    r64 = ctx.ci_batch(tests, alpha=w.alpha, flags=_lib.FPGM_F_EXACT)
+    m = (~r32["exact"]) & (r64["g2"] > 0)
+    rel = np.abs(r32["g2"][m] - r64["g2"][m]) / r64["g2"][m]
+    return {"tests": n, "fp32_decided": int(m.sum()), "in_band": int(r32["in_band"].sum()),
+            "max_rel_err": float(rel.max()) if rel.size else 0.0, "mean_rel_err": float(rel.mean()) if rel.size else 0.0,
+            "p99_rel_err": float(np.quantile(rel, 0.99)) if rel.size else 0.0,
+            "decisions_identical": bool(np.array_equal(r32["indep"], r64["indep"])),
+            "sample": "levels in proportion to executed tests, random edge and conditioning set per test"}
+
+
+def oracle_fixture_check(w, sample, g):
+    """The stage against the oracle's committed result for this workload (tests/golden), when
+    the regenerated data has the fixture's SHA-256."""
+    path = os.path.join(ROOT, "tests", "golden", f"stage_{w.name}.npz")
+    if not os.path.exists(path):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import sepset_digest
+    with np.load(path, allow_pickle=False) as z:
+        f = {k: z[k] for k in z.files}
+    rel = os.path.relpath(path, ROOT)
+    if sample.sha256() != str(f["data_sha256"]):
+        return {"identical": False, "fixture": rel, "reason": "data SHA-256 differs from the fixture's"}
+    n = sample.n_vars
+    edges = np.array([(i, j) for i in range(n) for j in range(i + 1, n) if g["adj"][i, j]], dtype=np.int32)
+    same = (g["tests_per_level"] == f["tests_per_level"].tolist()
+            and g["edges_after_level"] == f["edges_after_level"].tolist()
+            and g["num_ci_test"] == int(f["num_ci_test"])
+            and np.array_equal(edges.reshape(-1, 2), f["edges"])
+            and sepset_digest(g["sepsets"]) == str(f["sep_digest"]))
+    return {"identical": bool(same), "fixture": rel,
+            "checked": "tests and edges per level, num_ci_test, final skeleton, SHA-256 of every sepset record"}
 
 
 def summarize_trace(trace_dir, out_path):
@@ -360,9 +588,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--summarize-pmc", default="", metavar="DIR",
-                    help="only write profiles/hbm_traffic.json from DIR/{fetch,write}/ rocprofv3 "
-                         "--pmc CSVs and DIR/run.json (the profiled run's output line); no GPU use")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure HBM bytes")
+    ap.add_argument("--traffic-timeout", type=float, default=240.0)
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-trace", default="", metavar="DIR",
                     help="only write the kernel-trace profile from DIR (rocprofv3 --kernel-trace --stats output "
                          "and run.json); no GPU use")
@@ -376,9 +605,6 @@ def main():
                     help="apply per-level tuning records saved by --save-tune instead of tuning (profiled runs)")
     ap.add_argument("--save-tune", default="", metavar="JSON", help="write the autotuner's per-level records")
     args = ap.parse_args()
-    if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc)
-        return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace,
                         os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_kernel_trace.txt"))
@@ -392,9 +618,19 @@ def main():
               "this build (see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    w = configs.get(args.workload)
+    if args.traffic_child:
+        traffic_child(args, w)
+        return
+    # HBM bytes by counters, in child processes, before this process makes any GPU call
+    traffic, traffic_err = None, "not measured (--no-traffic)"
+    if not args.no_traffic and not w.n_replicates:
+        traffic, traffic_err = measure_traffic(args, w)
+        if traffic_err:
+            log(f"traffic: {traffic_err}")
+
     if args.copy_calibration:
         copy_calibration()
-    w = configs.get(args.workload)
     if w.n_replicates:
         bench_ensemble(args, w)
         return
@@ -423,6 +659,7 @@ def main():
         if args.save_tune:
             with open(args.save_tune, "w") as f:
                 json.dump({"workload": w.name, **tune}, f, indent=1)
+                f.write("\n")
 
     for i in range(args.warmup):
         g = ctx.pc_skeleton(w.alpha, w.depth)
@@ -431,6 +668,8 @@ def main():
     # timed region: the CI stage only (fpgm_pc_skeleton), nothing read back but the
     # adjacency and the stats struct
     total_tests, kernel_ms, alg_bytes, packed_bytes, launches = 0, 0.0, 0.0, 0.0, 0
+    sinks = {k: defaultdict(float) for k in SINKS}
+    gram = defaultdict(float)
     adjs = [np.zeros((sample.n_vars, sample.n_vars), dtype=np.uint8) for _ in range(args.steps)]
     n_sep = []
     t_start = time.perf_counter()
@@ -443,6 +682,11 @@ def main():
         packed_bytes += ks["packed_bytes"]
         launches += ks["launches"]
         n_sep.append(st.n_sepsets)
+        for k, v in ctx.sink_stats().items():
+            for f, x in v.items():
+                sinks[k][f] += x
+        for f, x in ctx.gram_stats().items():
+            gram[f] += x
     elapsed = time.perf_counter() - t_start
     value = total_tests / elapsed
     g = ctx.pc_skeleton(w.alpha, w.depth)   # untimed: full read-back for the checks below
@@ -451,8 +695,24 @@ def main():
         raise RuntimeError("skeleton differs between steps")
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {value:.0f} tests/s; per level "
         f"{g['tests_per_level']} executed {g['executed_per_level']}")
+    fixture = oracle_fixture_check(w, sample, g)
+    if fixture:
+        log(f"oracle fixture {fixture['fixture']}: identical {fixture['identical']}")
+    g2err = g2_error_sample(ctx, sample, w, g)
+    log(f"G^2 FP32 vs FP64 on {g2err['tests']} sampled tests: max rel {g2err['max_rel_err']:.2e}, "
+        f"decisions identical {g2err['decisions_identical']}")
+    peaks = probe_peaks()
+    gram_step = {k: v / args.steps for k, v in gram.items()}
+    gram_step["launches"] = gram["launches"] / args.steps
+    roof = build_roofline(sinks, args.steps, traffic, peaks, gram_step, kernel_ms, alg_bytes, packed_bytes, launches)
+    if traffic is None:
+        roof["traffic_source"] = traffic_err
+    if traffic and traffic.get("tune_file"):
+        roof["traffic_tune_file"] = traffic["tune_file"]
+    log(f"roofline: {roof['kernel']} at {roof['achieved'] or 0:.0f} GB/s measured HBM "
+        f"(frac {roof['frac'] or 0:.3f}), effective {roof.get('effective', {}).get('achieved', 0):.0f} GB/s; "
+        f"bound {roof.get('limit', {}).get('resource')}: frac {roof.get('limit', {}).get('frac')}")
 
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     out = {
         "metric": "CI tests/sec (reference-equivalent, PC-stable CI stage, G2)",
         "value": value,
@@ -476,30 +736,14 @@ def main():
         "executed_per_level": g["executed_per_level"],
         "in_band_per_level": g["in_band_per_level"],
         "edges_after_level": g["edges_after_level"],
-        "max_rel_err_g2_fp32_vs_fp64": g["max_rel_err_g2_fp32"],
+        "max_rel_err_g2_fp32_vs_fp64_in_band": g["max_rel_err_g2_fp32"],
+        "g2_error": g2err,
+        "oracle_fixture": fixture,
         "untuned_stage_ms": untuned * 1e3 if untuned else None,
         "autotune": ({"seconds": tune.get("seconds"), "source": args.tune_file or "fpgm_autotune",
                       "levels": tune["levels"]} if tune else None),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_count_stat", "kernel_ms_per_step": kernel_ms / args.steps,
-                     "launches_per_step": launches / args.steps,
-                     "alg_bytes_per_step": alg_bytes / args.steps,
-                     # the same tests in the layout the kernels read (columns packed at 2, 4 or 8
-                     # bits per value by cardinality): the bytes they cannot avoid reading
-                     "packed": {"bytes_per_step": packed_bytes / args.steps,
-                                "achieved": packed_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0,
-                                "frac": (packed_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-                                         if kernel_ms > 0 else 0.0)}},
+        "roofline": roof,
     }
-
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(w.name)
-    if traffic:
-        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
-        out["roofline"]["traffic_detail"] = traffic
 
     if not args.no_plain:
         gp = ctx.pc_skeleton(w.alpha, w.depth, mode=_lib.FPGM_MODE_PLAIN, plain_sample=args.plain_sample)

@@ -32,7 +32,7 @@ EXPORTS = [
     "fpgm_sepset_get", "fpgm_pchisq", "fpgm_chisq_crit", "fpgm_last_kernel_stats", "fpgm_last_packed_bytes",
     "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
     "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_ci_group", "fpgm_ci_batch_noalloc",
-    "fpgm_last_group_stats",
+    "fpgm_last_group_stats", "fpgm_last_sink_stats",
 ]
 
 
@@ -54,6 +54,15 @@ class Result(C.Structure):
 class GroupResult(C.Structure):
     _fields_ = [("p_value", C.c_double), ("is_independent", C.c_uint8), ("pad", C.c_uint8 * 3),
                 ("first", C.c_int32)]
+
+
+class SinkStats(C.Structure):
+    _fields_ = [("sink", C.c_int32), ("pad", C.c_int32), ("launches", C.c_int64), ("tests", C.c_int64),
+                ("items", C.c_int64), ("ms", C.c_double), ("alg_bytes", C.c_double), ("packed_bytes", C.c_double),
+                ("row_tests", C.c_double)]
+
+
+SINK_NAMES = ["lane", "lane-wide", "shared", "slice", "device"]
 
 
 class PcOpts(C.Structure):
@@ -141,11 +150,13 @@ def load(path: str = LIB_PATH):
     L.fpgm_ci_group.argtypes = [vp, i32, i32, vp, i32, i32, f64, C.POINTER(GroupResult), vp]
     L.fpgm_ci_batch_noalloc.argtypes = [vp, vp, i64, vp, i64, f64, u32, vp]
     L.fpgm_last_group_stats.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]
+    L.fpgm_last_sink_stats.argtypes = [vp, vp, i32, C.POINTER(i32)]
     for name in ("fpgm_ctx_create", "fpgm_ctx_create_device", "fpgm_ci_test", "fpgm_ci_batch", "fpgm_ci_counts",
                  "fpgm_pc_skeleton", "fpgm_sepset_count", "fpgm_sepset_get", "fpgm_last_kernel_stats",
                  "fpgm_ctx_create_multi", "fpgm_pc_skeleton_multi", "fpgm_last_gram_stats", "fpgm_mle_counts",
                  "fpgm_autotune", "fpgm_tune_set", "fpgm_tune_clear", "fpgm_last_packed_bytes", "fpgm_ci_group",
-                 "fpgm_ci_batch_noalloc", "fpgm_last_group_stats"):
+                 "fpgm_ci_batch_noalloc", "fpgm_last_group_stats",
+                 "fpgm_last_sink_stats"):
         getattr(L, name).restype = i32
     _LIB = L
     return L
@@ -390,6 +401,19 @@ class Context:
         ms, ops, t, n = C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
         self._check(self._L.fpgm_last_group_stats(self.h, C.byref(ms), C.byref(ops), C.byref(t), C.byref(n)))
         return {"ms": ms.value, "int8_ops": ops.value, "tests": t.value, "sequences": n.value}
+
+    def sink_stats(self):
+        """Per counting sink of the last call: launches, tests, work items, ms, bytes, row-tests."""
+        arr = (SinkStats * 8)()
+        n = C.c_int32()
+        self._check(self._L.fpgm_last_sink_stats(self.h, C.cast(arr, C.c_void_p), 8, C.byref(n)))
+        out = {}
+        for k in range(min(n.value, 8)):
+            r = arr[k]
+            out[SINK_NAMES[r.sink]] = {"launches": r.launches, "tests": r.tests, "items": r.items, "ms": r.ms,
+                                       "alg_bytes": r.alg_bytes, "packed_bytes": r.packed_bytes,
+                                       "row_tests": r.row_tests}
+        return out
 
     def kernel_stats(self):
         ms, b, n = C.c_double(), C.c_double(), C.c_int64()

@@ -14,6 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfpgm_ci.so")
 GEN_LIB = os.path.join(HERE, "libfpgm_gen.so")
+PROBE_LIB = os.path.join(HERE, "libfpgm_probe.so")
 HOST_LIB = os.path.join(HERE, "libfpgm_host.so")
 HOST_CLI = os.path.join(HERE, "fpgm_pcstable")
 HOST_SOURCES = ["dataset.cpp", "network.cpp", "pc_stable.cpp", "bnsl_comparison.cpp", "param_learning.cpp", "host_c_api.cpp"]
@@ -53,8 +54,21 @@ def build_gen(verbose: bool = False) -> str:
     return GEN_LIB
 
 
+def build_probe(verbose: bool = False) -> str:
+    """Throughput probes of the resources the counting kernels are bound by (csrc/probe.hip),
+    loaded by bench.py only."""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "probe.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
+
+
 def build(verbose: bool = False) -> str:
     build_gen(verbose)
+    build_probe(verbose)
     objs = []
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
     bdir = os.path.join(HERE, "_build")

@@ -165,7 +165,7 @@ struct Slot {
     double pbytes = 0.0; // and the bytes of their packed columns
     struct GroupInfo {
         int64_t tests = 0, items = 0, dsum = 0;
-        double csum = 0.0, cmax = 0.0, bytes = 0.0;
+        double csum = 0.0, cmax = 0.0, bytes = 0.0, pbytes = 0.0, rows = 0.0;
     } grp[kGroups];
 
     hipError_t init()
@@ -284,6 +284,7 @@ struct fpgm_ctx {
     // against 494 ms); FPGM_GROUP_MIN_ROWS sets the bound
     int group = 0;
     int64_t group_min_rows = 0;
+    fpgm_sink_stats sink[kGroups] = {};  // per counting sink of the last call
     double grp_ms = 0.0, grp_ops = 0.0;  // grouped launches of the last call: time, int8 operations
     int64_t grp_tests = 0, grp_launches = 0;
 
@@ -872,7 +873,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             HIP_TRY(hipStreamWaitEvent(gs, S.ev_fork, 0), "stream wait");
             joined[gi] = true;
         }
-        if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_grp[gi], gs), "event");
+        HIP_TRY(hipEventRecord(S.ev_grp[gi], gs), "event");
         a.item_begin = grp_begin[gi];
         a.n_items = grp_len[gi];
         a.rows_per_chunk = rpc[gi];
@@ -889,7 +890,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                     "launch finalize");
             ctx->k_launches += 2;
         }
-        if (ctx->trace) HIP_TRY(hipEventRecord(S.ev_gend[gi], gs), "event");
+        HIP_TRY(hipEventRecord(S.ev_gend[gi], gs), "event");
         if (joined[gi]) HIP_TRY(hipEventRecord(S.ev_join[gi], gs), "event");
     }
     for (int gi = 0; gi < kGroups; ++gi)
@@ -902,22 +903,24 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             "download results");
     HIP_TRY(hipEventRecord(S.ev_done, ctx->stream), "event");
     S.n = nb;
-    if (ctx->trace) {
-        for (int gi = 0; gi < kGroups; ++gi) {
-            Slot::GroupInfo &G = S.grp[gi];
-            G = Slot::GroupInfo{};
-            G.items = grp_len[gi];
-        }
-        for (int64_t k = 0; k < nb; ++k) {
-            if (gidx[k] == kGroupMfma) continue;
-            Slot::GroupInfo &G = S.grp[gidx[k]];
-            const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
-            G.tests += 1;
-            G.dsum += dt[k].d;
-            G.csum += c;
-            G.cmax = std::max(G.cmax, c);
-            G.bytes += (double)(dt[k].d + 2) * (double)ctx->inst[dt[k].inst].n_rows;
-        }
+    // per-sink figures of the sequence (fpgm_last_sink_stats, trace lines)
+    for (int gi = 0; gi < kGroups; ++gi) {
+        Slot::GroupInfo &G = S.grp[gi];
+        G = Slot::GroupInfo{};
+        G.items = grp_len[gi];
+    }
+    for (int64_t k = 0; k < nb; ++k) {
+        if (gidx[k] == kGroupMfma) continue;
+        Slot::GroupInfo &G = S.grp[gidx[k]];
+        const double c = (double)dt[k].dimz * dt[k].cx * dt[k].cy;
+        const double rows = (double)ctx->inst[dt[k].inst].n_rows;
+        G.tests += 1;
+        G.dsum += dt[k].d;
+        G.csum += c;
+        G.cmax = std::max(G.cmax, c);
+        G.bytes += (double)(dt[k].d + 2) * rows;
+        G.pbytes += packed_row_bytes(ctx, tests[i0 + k], zpool) * rows;
+        G.rows += rows;
     }
     S.pending = true;
     if (keep_ws) *keep_ws = dt[0].ws_off;
@@ -950,13 +953,22 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
                     (long long)S.g_tests, (long long)S.g_chunks, (long long)S.g_items, gms,
                     gms > 0 ? S.g_ops / (gms * 1e9) : 0.0);
     }
-    if (ctx->trace) {
+    {
         static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
         for (int gi = 0; gi < kGroups; ++gi) {
             const Slot::GroupInfo &G = S.grp[gi];
             if (G.items == 0) continue;
             float gms = 0.f;
             HIP_TRY(hipEventElapsedTime(&gms, S.ev_grp[gi], S.ev_gend[gi]), "event time");
+            fpgm_sink_stats &A = ctx->sink[gi];
+            A.launches += 1;
+            A.tests += G.tests;
+            A.items += G.items;
+            A.ms += gms;
+            A.alg_bytes += G.bytes;
+            A.packed_bytes += G.pbytes;
+            A.row_tests += G.rows;
+            if (!ctx->trace) continue;
             fprintf(stderr,
                     "[fpgm]   launch %-9s tests %7lld items %8lld mean |Z| %.2f cells mean %.0f max %.0f: "
                     "%8.3f ms, %7.0f GB/s\n",
@@ -999,6 +1011,10 @@ void reset_kernel_stats(fpgm_ctx *ctx)
     ctx->g_ops = 0.0;
     ctx->g_pair_ms = 0.0;
     ctx->g_launches = 0;
+    for (int g = 0; g < kGroups; ++g) {
+        ctx->sink[g] = fpgm_sink_stats{};
+        ctx->sink[g].sink = g;
+    }
     ctx->grp_ms = 0.0;
     ctx->grp_ops = 0.0;
     ctx->grp_tests = 0;
@@ -2567,6 +2583,14 @@ fpgm_status fpgm_last_group_stats(const fpgm_ctx *ctx, double *ms, double *int8_
     if (int8_ops) *int8_ops = ctx->grp_ops;
     if (tests) *tests = ctx->grp_tests;
     if (sequences) *sequences = ctx->grp_launches;
+    return FPGM_OK;
+}
+
+fpgm_status fpgm_last_sink_stats(const fpgm_ctx *ctx, fpgm_sink_stats *out, int32_t cap, int32_t *n)
+{
+    if (!ctx || !n || (cap > 0 && !out)) return FPGM_EINVAL;
+    *n = kGroups;
+    for (int g = 0; g < kGroups && g < cap; ++g) out[g] = ctx->sink[g];
     return FPGM_OK;
 }
 

@@ -286,6 +286,27 @@ fpgm_status fpgm_last_packed_bytes(const fpgm_ctx *ctx, double *packed_bytes);
 fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *int8_ops, double *pairs_ms,
                                  int64_t *launches);
 
+/* Per counting sink of the last fpgm_pc_skeleton / fpgm_ci_batch call (no counterpart in the
+ * reference; measurement only).  sink: 0 per-lane 16-bit LDS counters (tables <= 64 cells),
+ * 1 the same with 48 or 64 KiB of counters (<= 96 / 128 cells), 2 shared LDS histogram
+ * (<= 8192 cells), 3 z-sliced LDS histogram, 4 device-memory table.  ms: summed HIP-event
+ * durations of its launches; alg_bytes: (|Z| + 2) * rows per test; packed_bytes: the same
+ * tests' packed column bytes; row_tests: rows summed over its tests (each row of each test
+ * is one histogram increment). */
+typedef struct {
+    int32_t sink;
+    int32_t pad;
+    int64_t launches;
+    int64_t tests;
+    int64_t items;
+    double ms;
+    double alg_bytes;
+    double packed_bytes;
+    double row_tests;
+} fpgm_sink_stats;
+
+fpgm_status fpgm_last_sink_stats(const fpgm_ctx *ctx, fpgm_sink_stats *out, int32_t cap, int32_t *n);
+
 /* Grouped level-1 counting of the last fpgm_pc_skeleton call (tests sharing the walking
  * endpoint and the conditioning node, counted as one-hot int8 products on the MFMA units):
  * kernel time, int8 operations of the padded 32 x 32 x 32 fragments, tests counted there,

@@ -69,6 +69,9 @@ def lib():
                                       C.POINTER(OrcStats)]
         L.orc_level0_sample.restype = C.c_int64
         L.orc_level0_sample.argtypes = [C.POINTER(OrcData), C.c_double, C.c_int64, C.c_int, C.c_void_p]
+        L.orc_level_sample_ref128.restype = C.c_int64
+        L.orc_level_sample_ref128.argtypes = [C.POINTER(OrcData), C.c_double, C.c_int, C.c_void_p, C.c_int64, C.c_int,
+                                              C.POINTER(C.c_int64)]
         L.orc_level_sample.restype = C.c_int64
         L.orc_level_sample.argtypes = [C.POINTER(OrcData), C.c_double, C.c_int, C.c_void_p, C.c_int64, C.c_int,
                                        C.POINTER(C.c_int64)]
@@ -203,6 +206,15 @@ def level_sample(data: Data, alpha: float, d: int, adj_before: np.ndarray, max_e
     rem = C.c_int64()
     t = lib().orc_level_sample(data.ref, float(alpha), int(d), a.ctypes.data, int(max_edges), int(n_threads),
                                C.byref(rem))
+    return int(t), int(rem.value)
+
+
+def level_sample_ref128(data: Data, alpha: float, d: int, adj_before: np.ndarray, max_edges: int, n_threads: int):
+    """level_sample under the reference's 128-edge batch schedule (PCStable.cpp:275-324)."""
+    a = np.ascontiguousarray(adj_before, dtype=np.uint8)
+    rem = C.c_int64()
+    t = lib().orc_level_sample_ref128(data.ref, float(alpha), int(d), a.ctypes.data, int(max_edges),
+                                      int(n_threads), C.byref(rem))
     return int(t), int(rem.value)
 
 

@@ -603,6 +603,117 @@ int64_t orc_level_sample(const orc_data *D, double alpha, int d, const uint8_t *
     return tests;
 }
 
+/*
+ * The same bounded sample under the reference's own schedule of a level (SearchAtDepth,
+ * PCStable.cpp:275-324, group size 1): the edges on a stack, 128 popped at a time and
+ * checked under one `omp parallel for` (static schedule), each check running one CI test
+ * of its edge -- the next conditioning set in CheckEdge's order (x side, then y side) --
+ * and the unfinished edges pushed back, until the stack is empty; the last 1..127 edges
+ * with min(size, threads) threads.  Test counts and removals equal orc_level_sample's;
+ * only the schedule (a barrier every 128 tests) differs.  Level 0 has no such loop.
+ */
+typedef struct {
+    int side;      /* 0: subsets of adj(x)\{y}; 1: of adj(y)\{x}; 2: exhausted */
+    int started;
+    int a;
+    int32_t c[64];
+    int32_t *s;    /* the current side's neighbour set */
+} orc_walk;
+
+static int orc_walk_side(orc_walk *w, int d, int x, int y, const int32_t *nb_off, const int32_t *nb, int side)
+{
+    for (; side < 2; ++side) {
+        const int u = side == 0 ? x : y, other = side == 0 ? y : x;
+        int a = 0;
+        for (int32_t k = nb_off[u]; k < nb_off[u + 1]; ++k)
+            if (nb[k] != other) w->s[a++] = nb[k];
+        if (a >= d) {
+            w->side = side;
+            w->a = a;
+            for (int i = 0; i < d; ++i) w->c[i] = i;
+            return 1;
+        }
+    }
+    w->side = 2;
+    return 0;
+}
+
+int64_t orc_level_sample_ref128(const orc_data *D, double alpha, int d, const uint8_t *adj_before,
+                                int64_t max_edges, int n_threads, int64_t *removed)
+{
+    if (d == 0) return orc_level_sample(D, alpha, d, adj_before, max_edges, n_threads, removed);
+    const int n = D->n_vars, nt = n_threads > 0 ? n_threads : 1;
+    const int64_t cap = max_edges > 0 ? max_edges : 1;
+    int32_t *ex = (int32_t *)malloc(sizeof(int32_t) * cap), *ey = (int32_t *)malloc(sizeof(int32_t) * cap);
+    int64_t ne = 0;
+    for (int i = 0; i < n && ne < max_edges; ++i)
+        for (int j = i + 1; j < n && ne < max_edges; ++j)
+            if (adj_before[(int64_t)i * n + j]) {
+                ex[ne] = i;
+                ey[ne] = j;
+                ++ne;
+            }
+    int32_t *nb_off = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+    int32_t *nb = (int32_t *)malloc(sizeof(int32_t) * ((int64_t)n * n + 1));
+    nb_off[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        int32_t k = nb_off[i];
+        for (int j = 0; j < n; ++j)
+            if (adj_before[(int64_t)i * n + j]) nb[k++] = j;
+        nb_off[i + 1] = k;
+    }
+    orc_walk *wk = (orc_walk *)calloc((size_t)cap, sizeof(orc_walk));
+    for (int64_t e = 0; e < ne; ++e) wk[e].s = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+    orc_buf *tb = (orc_buf *)calloc((size_t)nt, sizeof(orc_buf));
+    int64_t *stack = (int64_t *)malloc(sizeof(int64_t) * cap), top = 0;
+    for (int64_t e = ne - 1; e >= 0; --e) stack[top++] = e;  /* edge 0 on top */
+    int64_t proc[128];
+    int need[128];
+    int64_t tests = 0, rem = 0;
+    while (top > 0) {
+        const int size = top >= 128 ? 128 : (int)top;
+        for (int i = 0; i < size; ++i) proc[i] = stack[--top];
+        const int p = (size < 128 && size < nt) ? size : nt;
+#pragma omp parallel for num_threads(p) reduction(+ : tests, rem)
+        for (int i = 0; i < size; ++i) {
+            orc_walk *w = &wk[proc[i]];
+            const int x = ex[proc[i]], y = ey[proc[i]];
+            need[i] = 0;
+            if (!w->started) {
+                w->started = 1;
+                if (!orc_walk_side(w, d, x, y, nb_off, nb, 0)) continue;
+            }
+            if (w->side >= 2) continue;
+            int32_t zz[64];
+            for (int k = 0; k < d; ++k) zz[k] = w->s[w->c[k]];
+            orc_buf *b = &tb[omp_get_thread_num()];
+            int32_t *tbl = orc_buf_get(b, orc_table_cells(D, x, y, zz, d));
+            const orc_result r = orc_ci_test_buf(D, x, y, zz, d, alpha, tbl);
+            tests += 1;
+            if (r.indep) {
+                rem += 1;
+                w->side = 2;
+                continue;
+            }
+            if (!orc_next_choice(w->a, d, w->c) && !orc_walk_side(w, d, x, y, nb_off, nb, w->side + 1)) continue;
+            need[i] = 1;
+        }
+        for (int i = size - 1; i >= 0; --i)
+            if (need[i]) stack[top++] = proc[i];
+    }
+    for (int i = 0; i < nt; ++i) free(tb[i].p);
+    for (int64_t e = 0; e < ne; ++e) free(wk[e].s);
+    free(tb);
+    free(wk);
+    free(stack);
+    free(nb);
+    free(nb_off);
+    free(ey);
+    free(ex);
+    if (removed) *removed = rem;
+    return tests;
+}
+
 /* ------------------------------------------------------------------ MLE counts */
 
 /*

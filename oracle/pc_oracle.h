@@ -88,6 +88,9 @@ int64_t orc_level0_sample(const orc_data *D, double alpha, int64_t n_pairs, int 
 /* Bounded timing sample of level d over the first max_edges edges of adj_before. */
 int64_t orc_level_sample(const orc_data *D, double alpha, int d, const uint8_t *adj_before, int64_t max_edges,
                          int n_threads, int64_t *removed);
+/* The same sample under the reference's 128-edge batch schedule (PCStable.cpp:275-324). */
+int64_t orc_level_sample_ref128(const orc_data *D, double alpha, int d, const uint8_t *adj_before,
+                                int64_t max_edges, int n_threads, int64_t *removed);
 
 /*
  * Maximum-likelihood tables of a known structure (ParameterLearning.cpp:24-69): node i
