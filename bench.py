@@ -239,7 +239,8 @@ def bench_ensemble(args, w):
 
 
 def _kernel_short(name):
-    return name.replace("void ", "").replace("fpgm::(anonymous namespace)::", "").split("(")[0]
+    return (name.replace("void ", "").replace("fpgm::", "").replace("(anonymous namespace)::", "")
+            .split("(")[0])
 
 
 def kernel_family(name):
@@ -383,7 +384,10 @@ def build_roofline(sinks, steps, traffic, peaks, gram, kernel_ms, alg_bytes, pac
         lines[name] = L
     if gram and gram.get("launches"):
         g_ms = gram["gram_ms"]
+        gfam = (traffic or {}).get("per_stage", {}).get("k_gram_l0")
         lines["gram"] = {"kernel": "k_gram_l0 (level 0)", "launches": gram["launches"], "ms": g_ms,
+                         **({"hbm_bytes": gfam["bytes"], "hbm_gbs": gfam["bytes"] / (g_ms * 1e-3) / 1e9,
+                             "hbm_frac": gfam["bytes"] / (g_ms * 1e-3) / 1e9 / HBM_PEAK_GBS} if gfam else {}),
                          "bound": {"resource": "int8 MFMA (v_mfma_i32_32x32x32_i8)",
                                    "achieved": gram["int8_ops"] / (g_ms * 1e-3), "unit": "OP/s",
                                    "peak": INT8_PEAK_OPS, "frac": gram["int8_ops"] / (g_ms * 1e-3) / INT8_PEAK_OPS}}
@@ -473,61 +477,104 @@ def oracle_fixture_check(w, sample, g):
 
 def summarize_trace(trace_dir, out_path):
     """Kernel-trace profile text from DIR/*kernel_stats.csv (rocprofv3 --kernel-trace --stats) and
-    DIR/run.json (the profiled run's JSON line).  No GPU use."""
-    import csv
-    import glob
+    DIR/run.json (the profiled run's JSON line, the same command without the profiler is the bench
+    itself).  No GPU use."""
+    import glob as _glob
     run = json.load(open(os.path.join(trace_dir, "run.json")))
-    stats = glob.glob(os.path.join(trace_dir, "*kernel_stats.csv"))[0]
+    stats = _glob.glob(os.path.join(trace_dir, "**", "*kernel_stats.csv"), recursive=True)[0]
     rows = list(csv.DictReader(open(stats)))
     stages = run["stages_in_run"]
-    tot = {_kernel_short(r["Name"]): (float(r["TotalDurationNs"]) / 1e6, int(r["Calls"])) for r in rows}
-    counting = [k for k in tot if k.startswith(("k_count_stat", "k_fin_wide", "k_compact"))]
-    cnt_ms = sum(tot[k][0] for k in counting) / stages
-    cnt_calls = sum(tot[k][1] for k in counting) / stages
     rl = run["roofline"]
-    gram = next((v for k, v in tot.items() if k.startswith("k_gram_l0")), (0.0, 0))
+    fam = defaultdict(lambda: [0.0, 0])
+    for r in rows:
+        f = kernel_family(r["Name"])
+        fam[f][0] += float(r["TotalDurationNs"]) / 1e6
+        fam[f][1] += int(r["Calls"])
     lines = [
         f"k_count_stat / k_gram_l0 -- kernel trace, workload {run['config']['workload']}",
-        "Command (GPU box, repository root, TMPDIR=/tmp; tuning records from a bench run with --save-tune):",
-        "  rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_kt -o w3 -- \\",
-        "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --copy-calibration \\",
-        "      --tune-file gpurun_out/tune_w3.json",
-        f"  then: python bench.py --summarize-trace gpurun_out/prof_kt   (run.json = that run's JSON line)",
-        f"The run executes the CI stage {stages} times (timed step + read-back run); per-stage figures are",
-        "the trace totals divided by that.",
-        "  (every counting kernel reads the packed columns: 2, 4 or 8 bits per value by cardinality)",
-        "  k_count_stat<2>  per-lane 16-bit LDS sink, tables <= 64 cells",
-        "  k_count_stat<3>  per-lane 16-bit LDS sink, tables <= 96 cells (48 KiB LDS)",
-        "  k_count_stat<0>  shared LDS histogram, tables <= 8192 cells ([cell][32] columns up to 256)",
-        "  k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
-        "  k_count_stat<1>  device-memory sink (larger tables)",
-        "  k_fin_wide_*     grid-wide finalize of device-memory tables",
-        "  k_compact        per-round removal of tests whose edge an earlier round decided",
-        "  k_gram_l0        level 0 as the int8 one-hot Gram matrix (v_mfma_i32_32x32x32_i8)",
-        "  k_pairs_l0, k_pack_l0  level-0 G^2 / decision and result packing",
-        f"Counting launches (k_count_stat + k_fin_wide + k_compact): {cnt_ms:.2f} ms per stage summed over "
-        f"{cnt_calls:.0f} dispatches per stage;",
-        f"bench.py's HIP-event figure for the same run: {rl['kernel_ms_per_step']:.2f} ms per stage (elapsed",
-        "  from each round's first to its last counting launch: the slice and device-memory launches,",
-        "  and LDS groups smaller than the CU count, run on streams of their own beside the others, so",
-        "  the trace's sum of durations exceeds the elapsed time).",
-        f"Algorithmic bytes per stage (sum over executed tests of (|Z|+2) * N): {rl['alg_bytes_per_step']:.4g}",
-        f"  (in the packed layout the kernels read: {rl.get('packed', {}).get('bytes_per_step', 0):.4g} bytes, "
-        f"{rl.get('packed', {}).get('bytes_per_step', 0) / (cnt_ms * 1e-3) / 1e9:.0f} GB/s by the trace)",
-        f"  -> achieved {rl['alg_bytes_per_step'] / (cnt_ms * 1e-3) / 1e9:.0f} GB/s by the trace, "
-        f"{rl['achieved']:.0f} GB/s by the events (peak {rl['peak']:.0f}); the effective rate exceeds the",
-        "     HBM peak because tests launched together share columns through L2 / MALL (measured HBM",
-        "     traffic: the hbm_traffic profile).",
+        "Command (GPU box, repository root, TMPDIR=/tmp):",
+        "  rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_kt -o w3 -- python bench.py --steps 1 \\",
+        "      --warmup 0 --no-cpu --no-plain --no-traffic --no-g2 --tune-file profiles/tune_dag-1000v-1500e-1m.json",
+        "  (its JSON line saved as gpurun_out/prof_kt/run.json), then, on any host:",
+        "  python bench.py --summarize-trace gpurun_out/prof_kt",
+        f"The run executes the CI stage {stages} times (timed step + read-back run); per-stage figures",
+        "below are the trace totals divided by the stage count.",
+        "Sinks (every counting kernel reads the packed columns: 2, 4 or 8 bits per value by cardinality):",
+        "  lane       k_count_stat<2>  per-lane 16-bit LDS counters, tables <= 64 cells",
+        "  lane-wide  k_count_stat<3>, <5>  the same with 48 / 64 KiB of counters, <= 96 / 128 cells",
+        "  shared     k_count_stat<0>  shared LDS histogram, tables <= 8192 cells",
+        "  slice      k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
+        "  device     k_count_stat<1> + k_fin_wide_*  device-memory table",
+        "  k_gram_l0  level 0 as the int8 one-hot Gram matrix (v_mfma_i32_32x32x32_i8)",
+        "",
+        "family        calls/stage   ms/stage   avg us/call   bench (HIP events): ms/stage  launches/stage",
     ]
-    if gram[1]:
-        g_ms = gram[0] / gram[1]
-        lines.append(f"Level 0 (k_gram_l0): {g_ms:.2f} ms per launch; 1.73e13 int8 ops per launch at "
-                     f"1000 vars (W = 4075 one-hot columns padded to 4096, 528 tiles of 128 x 128, 2 N each)")
-        lines.append(f"  -> {1.7301504e13 / (g_ms * 1e-3) / 1e15:.2f} POP/s of the 5.0 POP/s dense int8 peak.")
+    for k in SINKS + ["k_gram_l0", "k_pairs_l0", "k_compact", "k_pvalue"]:
+        if k not in fam:
+            continue
+        ms, calls = fam[k]
+        b = rl["sinks"].get(k) or (rl["sinks"].get("gram") if k == "k_gram_l0" else None)
+        bench_col = f"{b['ms']:10.2f}  {b['launches']:8.1f}" if b else ""
+        lines.append(f"{k:12s} {calls / stages:10.1f} {ms / stages:10.2f} {ms / max(calls, 1) * 1e3:12.1f}   {bench_col}")
+    dom = next((k for k in SINKS if rl["sinks"].get(k, {}).get("kernel") == rl["kernel"]), None)
+    if dom and dom in fam:
+        ms, calls = fam[dom]
+        lines += ["",
+                  f"Dominant kernel {rl['kernel']}: average launch {ms / calls:.3f} ms by the trace, "
+                  f"{rl['avg_launch_ms']:.3f} ms by the bench's HIP events;",
+                  f"  algorithmic bytes per launch {rl['alg_bytes_per_launch']:.4g} -> effective "
+                  f"{rl['alg_bytes_per_launch'] / (ms / calls * 1e-3) / 1e9:.0f} GB/s by the trace (columns shared "
+                  "through L2: not an HBM rate);",
+                  f"  bound: {rl['limit']['resource']}: {rl['limit']['achieved']:.4g} adds/s of a measured "
+                  f"{rl['limit']['peak']:.4g} (frac {rl['limit']['frac']:.3f})."]
+    g = rl["sinks"].get("gram")
+    if g and "k_gram_l0" in fam:
+        ms, calls = fam["k_gram_l0"]
+        ops = g["bound"]["achieved"] * g["ms"] * 1e-3
+        lines.append(f"Level 0 (k_gram_l0): {ms / calls:.2f} ms per launch, {ops:.4g} int8 operations -> "
+                     f"{ops / (ms / calls * 1e-3) / 1e15:.2f} POP/s of the 5.0 POP/s dense int8 peak.")
     lines += ["", open(stats).read()]
     with open(out_path, "w") as f:
         f.write("\n".join(lines))
-    print("\n".join(lines[:30]))
+    print("\n".join(lines[:40]))
+
+
+def summarize_traffic(bench_json, out_path):
+    """HBM-traffic profile text from a bench run's JSON line (its roofline was read from
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in child runs of the same command).  No GPU use."""
+    run = json.load(open(bench_json))
+    rl = run["roofline"]
+    lines = [
+        f"k_count_stat -- HBM traffic by counters, workload {run['config']['workload']}, per CI stage",
+        "Command (GPU box, repository root): python bench.py   (the default run: before touching the GPU",
+        "  it runs, as child processes, for counter in FETCH_SIZE WRITE_SIZE:",
+        "    rocprofv3 --pmc $counter -f csv -d /tmp/fpgm_pmc_*/$counter -o run -- python bench.py \\",
+        "        --traffic-child --workload 3 --tune-file profiles/tune_dag-1000v-1500e-1m.json",
+        "  each a 1 GiB device copy and one CI stage; counters in KiB per dispatch, scaled by 1 GiB over",
+        "  what they read for the copy);  then: python bench.py --summarize-traffic BENCH_JSON",
+        f"Source: {rl.get('traffic_source')}",
+        "",
+        "sink         ms/stage  launches  HBM bytes    HBM GB/s  frac of 8000   effective GB/s   bound: achieved / peak (frac)",
+    ]
+    for k, L in rl["sinks"].items():
+        b = L["bound"]
+        hb = L.get("hbm_bytes")
+        lines.append(f"{k:10s} {L['ms']:9.2f} {L['launches']:8.1f}  "
+                     f"{(f'{hb:10.4g}' if hb else '         -')}  {(L.get('hbm_gbs') or 0):9.0f}  "
+                     f"{(L.get('hbm_frac') or 0):11.3f}  {L.get('effective_gbs', 0):14.0f}   "
+                     f"{b['achieved']:.3g} / {b['peak']:.3g} {b['unit']} ({b['frac']:.3f})")
+    st = rl["stage"]
+    lines += ["",
+              f"Counting kernels per stage: {st['counting_ms']:.1f} ms elapsed (events), {st['launches']:.0f} launches,",
+              f"  algorithmic bytes {st['alg_bytes']:.4g}, packed-column bytes {st['packed_bytes']:.4g},",
+              f"  measured HBM bytes {st['hbm_bytes']:.4g} -> {st['hbm_gbs']:.0f} GB/s over the counting time.",
+              f"Dominant kernel {rl['kernel']}: {rl['traffic']:.4g} HBM bytes per launch, {rl['avg_launch_ms']:.3f} ms per "
+              f"launch -> {rl['achieved']:.0f} GB/s = {rl['frac']:.3f} of the HBM peak;",
+              f"  its own bound: {rl['limit']['resource']}, {rl['limit']['frac']:.3f} of the measured peak.",
+              f"Peaks measured by the probes: {json.dumps(rl['peaks'])}"]
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def summarize_sq(sq_dir, out_path):
@@ -535,7 +582,7 @@ def summarize_sq(sq_dir, out_path):
     import collections
     import csv
     import glob
-    path = glob.glob(os.path.join(sq_dir, "*counter_collection.csv"))[0]
+    path = glob.glob(os.path.join(sq_dir, "**", "*counter_collection.csv"), recursive=True)[0]
     agg = collections.defaultdict(lambda: collections.defaultdict(float))
     disp = collections.defaultdict(set)
     for r in csv.DictReader(open(path)):
@@ -548,12 +595,19 @@ def summarize_sq(sq_dir, out_path):
              "Command (GPU box, repository root, TMPDIR=/tmp):",
              "  rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES \\",
              "      SQ_INSTS_VMEM_RD SQ_LDS_BANK_CONFLICT SQ_INSTS_SALU -f csv -d gpurun_out/pmc_sq -o sq -- \\",
-             "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --tune-file gpurun_out/tune_w3.json",
+             "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --no-traffic --no-g2 \\",
+             "      --tune-file profiles/tune_dag-1000v-1500e-1m.json",
              "  then: python bench.py --summarize-sq gpurun_out/pmc_sq",
              "kernel                 disp " + " ".join(f"{n[3:]:>17s}" for n in names)]
     for k in sorted(agg, key=lambda k: -agg[k]["SQ_BUSY_CYCLES"]):
         d = agg[k]
         lines.append(f"{k:22s} {len(disp[k]):4d} " + " ".join(f"{d[n]:17.4g}" for n in names))
+    lines += ["", "kernel                 VALU per LDS instr   LDS bank-conflict cycles per LDS instr"]
+    for k in sorted(agg, key=lambda k: -agg[k]["SQ_BUSY_CYCLES"]):
+        d = agg[k]
+        if d["SQ_INSTS_LDS"] > 0:
+            lines.append(f"{k:22s} {d['SQ_INSTS_VALU'] / d['SQ_INSTS_LDS']:18.2f}   "
+                         f"{d['SQ_LDS_BANK_CONFLICT'] / d['SQ_INSTS_LDS']:10.3f}")
     with open(out_path, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -591,10 +645,13 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM bytes")
     ap.add_argument("--traffic-timeout", type=float, default=240.0)
+    ap.add_argument("--no-g2", action="store_true", help="skip the G^2 FP32-vs-FP64 sample (profiled runs)")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-trace", default="", metavar="DIR",
                     help="only write the kernel-trace profile from DIR (rocprofv3 --kernel-trace --stats output "
                          "and run.json); no GPU use")
+    ap.add_argument("--summarize-traffic", default="", metavar="JSON",
+                    help="only write the HBM-traffic profile from a bench run's JSON line; no GPU use")
     ap.add_argument("--summarize-sq", default="", metavar="DIR",
                     help="only write the SQ counter profile from DIR (rocprofv3 --pmc SQ_* output); no GPU use")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -608,6 +665,10 @@ def main():
     if args.summarize_trace:
         summarize_trace(args.summarize_trace,
                         os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_kernel_trace.txt"))
+        return
+    if args.summarize_traffic:
+        summarize_traffic(args.summarize_traffic,
+                          os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_hbm_traffic.txt"))
         return
     if args.summarize_sq:
         summarize_sq(args.summarize_sq, os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_sq_counters.txt"))
@@ -698,9 +759,10 @@ def main():
     fixture = oracle_fixture_check(w, sample, g)
     if fixture:
         log(f"oracle fixture {fixture['fixture']}: identical {fixture['identical']}")
-    g2err = g2_error_sample(ctx, sample, w, g)
-    log(f"G^2 FP32 vs FP64 on {g2err['tests']} sampled tests: max rel {g2err['max_rel_err']:.2e}, "
-        f"decisions identical {g2err['decisions_identical']}")
+    g2err = None if args.no_g2 else g2_error_sample(ctx, sample, w, g)
+    if g2err:
+        log(f"G^2 FP32 vs FP64 on {g2err['tests']} sampled tests: max rel {g2err['max_rel_err']:.2e}, "
+            f"decisions identical {g2err['decisions_identical']}")
     peaks = probe_peaks()
     gram_step = {k: v / args.steps for k, v in gram.items()}
     gram_step["launches"] = gram["launches"] / args.steps

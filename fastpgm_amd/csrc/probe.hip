@@ -4,9 +4,9 @@
 // row and test into lane-private counters, the device-memory sink one device-memory
 // atomic add per row.  These kernels measure what the device sustains for exactly those
 // access patterns, so that each sink's rate can be stated as a fraction of a measured peak:
-//   k_probe_lds_atomics: every lane adds into its own 32-bit word of a [cell][lane] LDS
-//     array (64 distinct words in 64 distinct banks per wave instruction, 32 KiB per
-//     workgroup as in the lane sink), 16 adds per step, no global memory traffic;
+//   k_probe_lds_atomics: every lane adds into its own 32-bit words of a [cell][lane] LDS
+//     array (64 distinct words per wave instruction, 32 KiB per workgroup as in the lane
+//     sink), 16 adds per step to addresses formed before the loop: no other work;
 //   k_probe_dev_atomics: every lane adds 1 to scattered words of a device-memory table
 //     (a multiplicative hash of its index), as the device-memory sink does.
 #include <hip/hip_runtime.h>
@@ -23,16 +23,16 @@ __global__ __launch_bounds__(kWG) void k_probe_lds_atomics(int steps, uint32_t *
     const int tid = threadIdx.x;
     for (int i = tid; i < kCells * kWG; i += kWG) h[i] = 0;
     __syncthreads();
-    uint32_t key = (uint32_t)tid * 2654435761u;
+    // 16 distinct cells per lane, addresses formed once: the loop is LDS atomics only
+    uint32_t *p[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) p[j] = &h[(((uint32_t)tid * 7u + (uint32_t)j * 5u) & (kCells - 1)) * kWG + tid];
     for (int s = 0; s < steps; ++s) {
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            key = key * 1664525u + 1013904223u;
-            atomicAdd(&h[((key >> 27) & (kCells - 1)) * kWG + tid], 1u);
-        }
+        for (int j = 0; j < 16; ++j) atomicAdd(p[j], 1u);
     }
     __syncthreads();
-    // one word per workgroup keeps the adds observable
+    // the sums stay observable
     uint32_t t = 0;
     for (int c = 0; c < kCells; ++c) t += h[c * kWG + tid];
     if (t == 0xFFFFFFFFu) sink[blockIdx.x] = t;
