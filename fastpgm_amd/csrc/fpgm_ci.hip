@@ -1021,12 +1021,13 @@ void reset_kernel_stats(fpgm_ctx *ctx)
     ctx->grp_launches = 0;
 }
 
-// Level 0 of one instance on the MFMA units: C = O^T O, then one thread per pair, every
-// pair x < y in lexicographic order.  Returns false in *used when the one-hot width is too
-// large for a dense C (the caller then counts pair by pair).  Results: one flag byte per
-// pair (bit 0 independent, bit 1 in the recheck band) in ctx->h_l0_flags, and the summary
-// in ctx->h_l0_sum.
-fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flags, float band, bool *used)
+// Level 0 of every instance on the MFMA units: C_i = O_i^T O_i, then one thread per
+// (instance, pair), every pair x < y in lexicographic order.  Returns false in *used when
+// the one-hot width is too large for dense C blocks (the caller then counts pair by pair).
+// Results: one flag byte per (instance, pair) (bit 0 independent, bit 1 in the recheck
+// band) in ctx->h_l0_flags, instance i's pairs at i * n(n-1)/2, and the summary in
+// ctx->h_l0_sum.
+fpgm_status run_level0_gram(fpgm_ctx *ctx, double alpha, uint32_t flags, float band, bool *used)
 {
     *used = false;
     Slot &S = ctx->slot[0];
@@ -1035,15 +1036,16 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
         if (st != FPGM_OK) return st;
     }
     const int n = ctx->n_vars;
+    const int64_t n_inst = (int64_t)ctx->inst.size();
     std::vector<int32_t> off(n + 1, 0);
     for (int v = 0; v < n; ++v) off[v + 1] = off[v] + ctx->card[v];
     const int64_t W = off[n];
     const int64_t Wp = std::max<int64_t>(kGramTile, (W + kGramTile - 1) / kGramTile * kGramTile);
-    if (Wp > 32768) return FPGM_OK;
+    // dense C blocks of every instance: at most 32768 one-hot columns and 8 GiB in all
+    if (Wp > 32768 || (double)n_inst * Wp * Wp * 4.0 > 8.0 * (1 << 30)) return FPGM_OK;
     *used = true;
     fpgm_status st = ensure_crit(ctx, alpha);
     if (st != FPGM_OK) return st;
-    const InstHost &I = ctx->inst[inst];
     std::vector<int32_t> rowvar((size_t)Wp, 0);
     std::vector<uint8_t> rowstate((size_t)Wp, 255);
     for (int v = 0; v < n; ++v)
@@ -1059,20 +1061,29 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
             tiles.push_back(j);
         }
     const int64_t n_tp = (int64_t)tiles.size() / 2;
-    const int64_t ksteps = std::max<int64_t>(1, (I.n_rows + 31) / 32);
+    int64_t N = 0;
+    double rows_all = 0.0;
+    for (const InstHost &I : ctx->inst) {
+        N = std::max(N, I.n_rows);
+        rows_all += (double)I.n_rows;
+    }
+    // row splits of the longest instance: enough items to fill the CUs (splits past a
+    // shorter instance's rows end at once)
+    const int64_t ksteps = std::max<int64_t>(1, (N + 31) / 32);
     const int64_t target = (int64_t)ctx->n_cu * ctx->sched.gram_items_per_cu;
-    const int64_t splits = std::min<int64_t>(ksteps, std::max<int64_t>(1, (target + n_tp - 1) / n_tp));
+    const int64_t splits = std::min<int64_t>(ksteps, std::max<int64_t>(1, (target + n_tp * n_inst - 1) / (n_tp * n_inst)));
     const int64_t kps = (ksteps + splits - 1) / splits * 32;
-    const int64_t n_items = n_tp * splits;
+    const int64_t n_items = n_inst * n_tp * splits;
+    if (n_items > INT32_MAX) return fail(ctx, FPGM_ETOOBIG, "too many level-0 Gram work items");
     const int64_t np = (int64_t)n * (n - 1) / 2;
-    HIP_TRY(ctx->d_C.ensure(Wp * Wp, false, ctx->stream), "alloc gram");
+    HIP_TRY(ctx->d_C.ensure(n_inst * Wp * Wp, false, ctx->stream), "alloc gram");
     HIP_TRY(ctx->d_rowvar.ensure(Wp, false, ctx->stream), "alloc gram rows");
     HIP_TRY(ctx->d_rowstate.ensure(Wp, false, ctx->stream), "alloc gram rows");
     HIP_TRY(ctx->d_tiles.ensure((int64_t)tiles.size(), false, ctx->stream), "alloc gram tiles");
     HIP_TRY(ctx->d_off.ensure(n + 1, false, ctx->stream), "alloc gram offsets");
-    HIP_TRY(S.d_res.ensure(np, false, ctx->stream), "alloc results");
-    HIP_TRY(ctx->d_l0_flags.ensure(np, false, ctx->stream), "alloc level-0 flags");
-    HIP_TRY(ctx->h_l0_flags.ensure(np, ctx->stream), "alloc pinned level-0 flags");
+    HIP_TRY(S.d_res.ensure(np * n_inst, false, ctx->stream), "alloc results");
+    HIP_TRY(ctx->d_l0_flags.ensure(np * n_inst, false, ctx->stream), "alloc level-0 flags");
+    HIP_TRY(ctx->h_l0_flags.ensure(np * n_inst, ctx->stream), "alloc pinned level-0 flags");
     HIP_TRY(ctx->d_l0_sum.ensure(1, false, ctx->stream), "alloc level-0 summary");
     HIP_TRY(ctx->h_l0_sum.ensure(1, ctx->stream), "alloc pinned level-0 summary");
     HIP_TRY(hipMemcpyAsync(ctx->d_rowvar.p, rowvar.data(), sizeof(int32_t) * Wp, hipMemcpyHostToDevice, ctx->stream),
@@ -1084,11 +1095,15 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
             "upload gram tiles");
     HIP_TRY(hipMemcpyAsync(ctx->d_off.p, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream),
             "upload offsets");
-    HIP_TRY(hipMemsetAsync(ctx->d_C.p, 0, sizeof(int32_t) * Wp * Wp, ctx->stream), "clear gram");
+    HIP_TRY(hipMemsetAsync(ctx->d_C.p, 0, sizeof(int32_t) * n_inst * Wp * Wp, ctx->stream), "clear gram");
     GramArgs g{};
-    g.cols = I.cols;
-    g.stride = I.stride;
-    g.n_rows = I.n_rows;
+    const InstHost &I0 = ctx->inst[0];
+    g.cols = I0.cols;
+    g.stride = I0.stride;
+    g.n_rows = I0.n_rows;
+    g.insts = n_inst > 1 ? ctx->d_inst : nullptr;
+    g.n_tp = (int32_t)n_tp;
+    g.c_stride = Wp * Wp;
     g.row_var = ctx->d_rowvar.p;
     g.row_state = ctx->d_rowstate.p;
     g.tiles = ctx->d_tiles.p;
@@ -1106,6 +1121,8 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
     p.py = nullptr;
     p.n_vars = n;
     p.n_pairs = np;
+    p.n_inst = (int32_t)n_inst;
+    p.c_stride = Wp * Wp;
     p.crit = ctx->d_crit;
     p.alpha = alpha;
     p.z_alpha = ctx->z_alpha;
@@ -1119,10 +1136,12 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
     HIP_TRY(hipEventRecord(e_mid, ctx->stream), "event");
     HIP_TRY(launch_pairs_l0(p, ctx->stream), "launch pairs");
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
-    HIP_TRY(launch_pvalue(S.d_res.p, np, alpha, nullptr, nullptr, 0, nullptr, 0, ctx->stream), "launch pvalue");
+    HIP_TRY(launch_pvalue(S.d_res.p, np * n_inst, alpha, nullptr, nullptr, 0, nullptr, 0, ctx->stream),
+            "launch pvalue");
     HIP_TRY(hipMemsetAsync(ctx->d_l0_sum.p, 0, sizeof(PairSummary), ctx->stream), "clear level-0 summary");
-    HIP_TRY(launch_pack_l0(S.d_res.p, np, ctx->d_l0_flags.p, ctx->d_l0_sum.p, ctx->stream), "launch pack");
-    HIP_TRY(hipMemcpyAsync(ctx->h_l0_flags.p, ctx->d_l0_flags.p, (size_t)np, hipMemcpyDeviceToHost, ctx->stream),
+    HIP_TRY(launch_pack_l0(S.d_res.p, np * n_inst, ctx->d_l0_flags.p, ctx->d_l0_sum.p, ctx->stream), "launch pack");
+    HIP_TRY(hipMemcpyAsync(ctx->h_l0_flags.p, ctx->d_l0_flags.p, (size_t)(np * n_inst), hipMemcpyDeviceToHost,
+                           ctx->stream),
             "download level-0 flags");
     HIP_TRY(hipMemcpyAsync(ctx->h_l0_sum.p, ctx->d_l0_sum.p, sizeof(PairSummary), hipMemcpyDeviceToHost, ctx->stream),
             "download level-0 summary");
@@ -1133,7 +1152,7 @@ fpgm_status run_level0_gram(fpgm_ctx *ctx, int inst, double alpha, uint32_t flag
     (void)hipEventDestroy(e_mid);
     ctx->g_ms += ms_g;
     ctx->g_pair_ms += ms_p;
-    ctx->g_ops += 2.0 * (double)n_tp * kGramTile * kGramTile * (double)I.n_rows;
+    ctx->g_ops += 2.0 * (double)n_tp * kGramTile * kGramTile * rows_all;
     ctx->g_launches += 1;
     return FPGM_OK;
 }
@@ -2163,37 +2182,39 @@ fpgm_status run_pc(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm_pc_opt
         const double tl = now_s();
         const int64_t np = (int64_t)n * (n - 1) / 2;
         bool gram = false;
-        if (!plain && n_inst == 1 && !(o.flags & FPGM_F_NO_GRAM)) {
+        if (!plain && !(o.flags & FPGM_F_NO_GRAM)) {
             const double ms0 = ctx->g_ms + ctx->g_pair_ms;
             const int64_t gram_default = ctx->sched.gram_items_per_cu;
             if (!ctx->tune.empty() && ctx->tune[0].level == 0 && ctx->tune[0].items_per_cu > 0)
                 ctx->sched.gram_items_per_cu = ctx->tune[0].items_per_cu;
-            st = run_level0_gram(ctx, 0, alpha, o.flags, band, &gram);
+            st = run_level0_gram(ctx, alpha, o.flags, band, &gram);
             ctx->sched.gram_items_per_cu = gram_default;
             if (st != FPGM_OK) return st;
             S.kernel_seconds_per_level[0] += (ctx->g_ms + ctx->g_pair_ms - ms0) * 1e-3;
         }
         int64_t inb = 0, left = 0;
         if (gram) {
-            PcInst &I = P[0];
-            const uint8_t *f = ctx->h_l0_flags.p;
-            int64_t k = 0;
-            for (int x = 0; x < n; ++x)
-                for (int y = x + 1; y < n; ++y, ++k) {
-                    if (f[k] & 1) {
-                        I.seps.push_back(PcInst::Sep{x, y, 0, 0, (int64_t)I.sepz.size()});
-                    } else {
-                        I.ex.push_back(x);
-                        I.ey.push_back(y);
+            for (int32_t i = 0; i < n_inst; ++i) {
+                PcInst &I = P[i];
+                const uint8_t *f = ctx->h_l0_flags.p + (int64_t)i * np;
+                int64_t k = 0;
+                for (int x = 0; x < n; ++x)
+                    for (int y = x + 1; y < n; ++y, ++k) {
+                        if (f[k] & 1) {
+                            I.seps.push_back(PcInst::Sep{x, y, 0, 0, (int64_t)I.sepz.size()});
+                        } else {
+                            I.ex.push_back(x);
+                            I.ey.push_back(y);
+                        }
                     }
-                }
+                I.num_ci_test += np;
+                I.n_sep0 = I.seps.size();
+                left += (int64_t)I.ex.size();
+            }
             inb = ctx->h_l0_sum.p->in_band;
             float err;
             std::memcpy(&err, &ctx->h_l0_sum.p->max_rel_err_bits, sizeof(float));
             S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, (double)err);
-            I.num_ci_test += np;
-            I.n_sep0 = I.seps.size();
-            left = (int64_t)I.ex.size();
         } else {
             tv.clear();
             for (int32_t i = 0; i < n_inst; ++i)
@@ -2355,7 +2376,7 @@ fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm
                 ctx->sched.gram_items_per_cu = ipc;
                 const double ms0 = ctx->g_ms + ctx->g_pair_ms;
                 bool used = false;
-                const fpgm_status s2 = run_level0_gram(ctx, 0, alpha, o.flags, o.band > 0.f ? o.band : 0.02f, &used);
+                const fpgm_status s2 = run_level0_gram(ctx, alpha, o.flags, o.band > 0.f ? o.band : 0.02f, &used);
                 ctx->sched.gram_items_per_cu = keep;
                 if (s2 != FPGM_OK) return -1.0;
                 best = std::min(best, ctx->g_ms + ctx->g_pair_ms - ms0);

@@ -23,6 +23,11 @@ struct GramArgs {
     int64_t k_per_split;       // rows per split, multiple of 32
     int32_t *C;                // [Wp][Wp] int32, zero before the launch
     int32_t max_card;          // largest cardinality (<= 8: one-hot bytes by table lookup)
+    // several instances (insts != nullptr): item = (instance * n_tp + tile pair) * k_splits
+    // + split, instance i's columns from insts[i] and its C at C + i * c_stride
+    const DevInst *insts;
+    int32_t n_tp;              // tile pairs per instance
+    int64_t c_stride;
 };
 
 struct PairArgs {
@@ -32,7 +37,9 @@ struct PairArgs {
     const int32_t *card;
     const int32_t *px, *py;    // pairs x < y; nullptr: all pairs of n_vars in lexicographic order
     int32_t n_vars;
-    int64_t n_pairs;
+    int64_t n_pairs;           // pairs per instance
+    int32_t n_inst;            // instances: thread t takes pair t % n_pairs of instance t / n_pairs
+    int64_t c_stride;          // C of instance i at C + i * c_stride
     const double *crit;
     double alpha, z_alpha;
     float band;

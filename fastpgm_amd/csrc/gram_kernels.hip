@@ -32,7 +32,20 @@ template <bool kPerm>
 __global__ __launch_bounds__(kGramWaves * 64) __attribute__((amdgpu_waves_per_eu(2))) void k_gram_l0(GramArgs g)
 {
     const int item = blockIdx.x;
-    const int tp = item / g.k_splits, ks = item - tp * g.k_splits;
+    const int tpi = item / g.k_splits, ks = item - tpi * g.k_splits;
+    int tp = tpi;
+    const uint8_t *cols = g.cols;
+    int64_t stride = g.stride, n_rows = g.n_rows;
+    int32_t *C = g.C;
+    if (g.insts) {  // instance tpi / n_tp
+        const int inst = tpi / g.n_tp;
+        tp = tpi - inst * g.n_tp;
+        const DevInst I = g.insts[inst];
+        cols = I.cols;
+        stride = I.stride;
+        n_rows = I.n_rows;
+        C = g.C + (int64_t)inst * g.c_stride;
+    }
     const int ti = g.tiles[2 * tp], tj = g.tiles[2 * tp + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int h = lane >> 5, l32 = lane & 31;
@@ -43,13 +56,13 @@ __global__ __launch_bounds__(kGramWaves * 64) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
     for (int f = 0; f < kGramA; ++f) {
         const int r = i_base + 32 * f + l32;
-        pa[f] = g.cols + (int64_t)g.row_var[r] * g.stride + h * 16;
+        pa[f] = cols + (int64_t)g.row_var[r] * stride + h * 16;
         onehot_lut<kPerm>(g.row_state[r], la[f], ha[f]);
     }
 #pragma unroll
     for (int f = 0; f < kGramB; ++f) {
         const int r = j_base + 32 * f + l32;
-        pb[f] = g.cols + (int64_t)g.row_var[r] * g.stride + h * 16;
+        pb[f] = cols + (int64_t)g.row_var[r] * stride + h * 16;
         onehot_lut<kPerm>(g.row_state[r], lb[f], hb[f]);
     }
 
@@ -59,7 +72,8 @@ __global__ __launch_bounds__(kGramWaves * 64) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
         for (int e = 0; e < kGramB; ++e) c[f][e] = v16i{0};
     const int64_t r_begin = (int64_t)ks * g.k_per_split;
-    const int64_t r_end = (r_begin + g.k_per_split < g.n_rows) ? r_begin + g.k_per_split : g.n_rows;
+    if (r_begin >= n_rows) return;  // a split past a shorter instance's rows
+    const int64_t r_end = (r_begin + g.k_per_split < n_rows) ? r_begin + g.k_per_split : n_rows;
     // full 32-row steps: no masks; the loads of the next step are issued before the
     // current step's one-hot formation and MFMAs (two register buffers in turn)
     const int64_t full_end = r_begin + (r_end - r_begin) / 32 * 32;
@@ -117,7 +131,7 @@ __global__ __launch_bounds__(kGramWaves * 64) __attribute__((amdgpu_waves_per_eu
         const int row = (q >> 2) * 8 + h * 4 + (q & 3);
 #pragma unroll
         for (int f = 0; f < kGramA; ++f) {
-            int32_t *cr = g.C + (int64_t)(i_base + 32 * f + row) * g.Wp + j_base + l32;
+            int32_t *cr = C + (int64_t)(i_base + 32 * f + row) * g.Wp + j_base + l32;
 #pragma unroll
             for (int e = 0; e < kGramB; ++e)
                 if (c[f][e][q]) atomicAdd(cr + 32 * e, c[f][e][q]);
@@ -151,8 +165,9 @@ __device__ __forceinline__ float term32(int32_t n, int32_t ni, int32_t nj, int32
 // reference's expression (IndependenceTest.cpp:327-347) and k_pvalue decides.
 __global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
 {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= p.n_pairs) return;
+    const int64_t tg = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (tg >= p.n_pairs * p.n_inst) return;
+    const int64_t inst = tg / p.n_pairs, t = tg - inst * p.n_pairs;
     int x, y;
     if (p.px) {
         x = p.px[t];
@@ -169,7 +184,7 @@ __global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
         y = (int)(t - row0(xx) + xx + 1);
     }
     const int cx = p.card[x], cy = p.card[y];
-    const int32_t *base = p.C + (int64_t)p.off[x] * p.Wp + p.off[y];
+    const int32_t *base = p.C + inst * p.c_stride + (int64_t)p.off[x] * p.Wp + p.off[y];
     int32_t nj[kPairMaxCard];
     int32_t nk = 0;
     int aly = 0;
@@ -246,7 +261,7 @@ __global__ __launch_bounds__(256) void k_pairs_l0(PairArgs p)
         R.exact = 1;
         R.need_p = (df != 0) ? 1 : 0;
     }
-    p.res[t] = R;
+    p.res[tg] = R;
 }
 
 __global__ __launch_bounds__(256) void k_pack_l0(const DevResult *res, int64_t n, uint8_t *flags, PairSummary *sum)
@@ -304,8 +319,8 @@ hipError_t launch_gram_l0(const GramArgs &g, int n_items, hipStream_t s)
 
 hipError_t launch_pairs_l0(const PairArgs &p, hipStream_t s)
 {
-    if (p.n_pairs <= 0) return hipSuccess;
-    const int64_t blocks = (p.n_pairs + 255) / 256;
+    if (p.n_pairs <= 0 || p.n_inst <= 0) return hipSuccess;
+    const int64_t blocks = (p.n_pairs * p.n_inst + 255) / 256;
     hipLaunchKernelGGL(k_pairs_l0, dim3((unsigned)blocks), dim3(256), 0, s, p);
     return hipGetLastError();
 }

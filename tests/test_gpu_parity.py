@@ -519,6 +519,26 @@ def test_level0_gram_matches_counting_kernel(n_rows):
     _assert_same(b, o)
 
 
+def test_level0_gram_multi_instance():
+    # level 0 of every instance of a multi-instance context in one Gram launch (a C block
+    # per instance, items over instance x tile pair x row split); instances of different
+    # lengths, one of 33 rows (row splits past its end do nothing); equal to the counting
+    # kernel's level 0 and to each instance's oracle run
+    w, reps = _ensemble(6)
+    reps = reps + [datagen.bootstrap(reps[0], 33, np.random.default_rng(1))]
+    with _lib.Context.of_many(reps) as ctx:
+        a = ctx.pc_skeleton_multi(w.alpha, 1)
+        gs = ctx.gram_stats()
+        b = ctx.pc_skeleton_multi(w.alpha, 1, flags=_lib.FPGM_F_NO_GRAM)
+        gb = ctx.gram_stats()
+    assert gs["launches"] == 1 and gb["launches"] == 0
+    assert np.array_equal(a["adj"], b["adj"]) and np.array_equal(a["edge_freq"], b["edge_freq"])
+    for i, r in enumerate(reps):
+        o = O.pc_skeleton(O.Data.of(r), w.alpha, 1, n_threads=8)
+        assert np.array_equal(a["adj"][i], o["adj"]), i
+        assert a["num_ci_test_per_instance"][i] == o["num_ci_test"]
+
+
 @pytest.mark.parametrize("card_hi", [8, 12])
 def test_level0_gram_card_paths(card_hi):
     # cardinalities <= 8 form the one-hot operands by byte-table lookup, larger ones by
