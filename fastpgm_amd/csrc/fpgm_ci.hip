@@ -157,6 +157,8 @@ struct Slot {
     hipEvent_t ev_gend[kGroups] = {};     // and its end (groups may run concurrently)
     hipEvent_t ev_fork = nullptr, ev_join[kGroups] = {};  // concurrent sink launches
     hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;           // grouped MFMA launches
+    hipGraphExec_t gexec = nullptr;  // the captured sequence (FPGM_GRAPH=1), updated in place when it can be
+    bool graphed = false;            // the pending sequence ran as a graph (no per-launch events)
     int64_t g_tests = 0, g_chunks = 0, g_items = 0;        // ... of the pending sequence
     double g_ops = 0.0;
     bool pending = false;
@@ -214,6 +216,8 @@ struct Slot {
         for (hipEvent_t ev : ev_gend)
             if (ev) (void)hipEventDestroy(ev);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
         if (ev_g0) (void)hipEventDestroy(ev_g0);
         if (ev_g1) (void)hipEventDestroy(ev_g1);
         for (hipEvent_t ev : ev_join)
@@ -284,6 +288,11 @@ struct fpgm_ctx {
     // against 494 ms); FPGM_GROUP_MIN_ROWS sets the bound
     int group = 0;
     int64_t group_min_rows = 0;
+    // FPGM_GRAPH=1: each launch sequence's device work (uploads, compaction, counting
+    // launches, p-values, read-back) captured into a HIP graph and launched as one; off by
+    // default (measured no faster)
+    int graph = 0;
+    int64_t graph_instantiations = 0, graph_updates = 0;
     fpgm_sink_stats sink[kGroups] = {};  // per counting sink of the last call
     double grp_ms = 0.0, grp_ops = 0.0;  // grouped launches of the last call: time, int8 operations
     int64_t grp_tests = 0, grp_launches = 0;
@@ -787,6 +796,25 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         HIP_TRY(hipMemsetAsync(S.d_queue.p, 0, sizeof(uint32_t) * S.d_queue.cap, ctx->stream), "clear work queues");
     }
     HIP_TRY(S.h_res.ensure(nb, ctx->stream), "alloc pinned results");
+    // graph mode: everything from here to the read-back is captured (every buffer is
+    // allocated above; no memset is captured); an error ends the capture
+    const bool graph = ctx->graph != 0;
+    struct CaptureScope {
+        hipStream_t s;
+        bool on = false;
+        ~CaptureScope()
+        {
+            if (!on) return;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } capture{ctx->stream};
+    if (graph) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "begin capture");
+        capture.on = true;
+    }
+    S.graphed = graph;
     if (zlen > 0)
         HIP_TRY(hipMemcpyAsync(S.d_zpool.p, S.h_zpool.p, sizeof(int32_t) * zlen, hipMemcpyHostToDevice, ctx->stream),
                 "upload z pool");
@@ -812,7 +840,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     a.edge_done = ctx->skip_round > 0 ? ctx->d_edge_done.p : nullptr;
     a.round = ctx->skip_round;
     ctx->t_prep += now_s() - tp0;
-    HIP_TRY(hipEventRecord(S.ev0, ctx->stream), "event");
+    if (!graph) HIP_TRY(hipEventRecord(S.ev0, ctx->stream), "event");
     uint32_t *live_count = S.d_queue.p + kGroups * kXcds;
     if (a.edge_done) {
         // a speculative round: drop the items of tests whose edge an earlier round decided
@@ -840,14 +868,14 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         ga.gtest = S.d_gtest.p;
         ga.rows_per_split = g_rps;
         ga.max_card = *std::max_element(ctx->card.begin(), ctx->card.end());
-        HIP_TRY(hipEventRecord(S.ev_g0, ctx->stream), "event");
+        if (!graph) HIP_TRY(hipEventRecord(S.ev_g0, ctx->stream), "event");
         ga.items = S.d_gitems.p;
         ga.n_items = n_gi1;
         HIP_TRY(launch_group_count(ga, 1, ctx->stream), "launch group count");
         ga.items = S.d_gitems.p + n_gi1;
         ga.n_items = n_gi - n_gi1;
         HIP_TRY(launch_group_count(ga, 2, ctx->stream), "launch group count");
-        HIP_TRY(hipEventRecord(S.ev_g1, ctx->stream), "event");
+        if (!graph) HIP_TRY(hipEventRecord(S.ev_g1, ctx->stream), "event");
         ctx->k_launches += (n_gi1 > 0) + (n_gi > n_gi1);
     }
     // one launch per sink group.  The LDS sinks (lane, lane-wide, shared) run one after
@@ -873,7 +901,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             HIP_TRY(hipStreamWaitEvent(gs, S.ev_fork, 0), "stream wait");
             joined[gi] = true;
         }
-        HIP_TRY(hipEventRecord(S.ev_grp[gi], gs), "event");
+        if (!graph) HIP_TRY(hipEventRecord(S.ev_grp[gi], gs), "event");
         a.item_begin = grp_begin[gi];
         a.n_items = grp_len[gi];
         a.rows_per_chunk = rpc[gi];
@@ -890,17 +918,45 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                     "launch finalize");
             ctx->k_launches += 2;
         }
-        HIP_TRY(hipEventRecord(S.ev_gend[gi], gs), "event");
+        if (!graph) HIP_TRY(hipEventRecord(S.ev_gend[gi], gs), "event");
         if (joined[gi]) HIP_TRY(hipEventRecord(S.ev_join[gi], gs), "event");
     }
     for (int gi = 0; gi < kGroups; ++gi)
         if (joined[gi]) HIP_TRY(hipStreamWaitEvent(ctx->stream, S.ev_join[gi], 0), "stream wait");
-    HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
+    if (!graph) HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
     HIP_TRY(launch_pvalue(S.d_res.p, nb, alpha, S.d_tests.p, a.edge_done, a.round, S.d_queue.p, kGroups * (kXcds + 1),
                           ctx->stream),
             "launch pvalue");
     HIP_TRY(hipMemcpyAsync(S.h_res.p, S.d_res.p, sizeof(DevResult) * nb, hipMemcpyDeviceToHost, ctx->stream),
             "download results");
+    if (graph) {
+        // the slot's executable graph is updated in place when the sequence has the same
+        // shape as the last one, else instantiated anew; ev0..ev1 span the whole graph
+        hipGraph_t gr = nullptr;
+        capture.on = false;
+        HIP_TRY(hipStreamEndCapture(ctx->stream, &gr), "end capture");
+        hipError_t e = hipSuccess;
+        if (S.gexec) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult ur = hipGraphExecUpdateSuccess;
+            if (hipGraphExecUpdate(S.gexec, gr, &err_node, &ur) != hipSuccess || ur != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(S.gexec);
+                S.gexec = nullptr;
+            } else {
+                ++ctx->graph_updates;
+            }
+        }
+        if (!S.gexec) {
+            e = hipGraphInstantiate(&S.gexec, gr, nullptr, nullptr, 0);
+            ++ctx->graph_instantiations;
+        }
+        (void)hipGraphDestroy(gr);
+        HIP_TRY(e, "instantiate graph");
+        HIP_TRY(hipEventRecord(S.ev0, ctx->stream), "event");
+        HIP_TRY(hipGraphLaunch(S.gexec, ctx->stream), "launch graph");
+        HIP_TRY(hipEventRecord(S.ev1, ctx->stream), "event");
+    }
     HIP_TRY(hipEventRecord(S.ev_done, ctx->stream), "event");
     S.n = nb;
     // per-sink figures of the sequence (fpgm_last_sink_stats, trace lines)
@@ -941,7 +997,7 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
     ctx->k_ms += ms;
     ctx->k_bytes += S.bytes;
     ctx->k_pbytes += S.pbytes;
-    if (S.g_chunks > 0) {
+    if (S.g_chunks > 0 && !S.graphed) {
         float gms = 0.f;
         HIP_TRY(hipEventElapsedTime(&gms, S.ev_g0, S.ev_g1), "event time");
         ctx->grp_ms += gms;
@@ -959,7 +1015,7 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
             const Slot::GroupInfo &G = S.grp[gi];
             if (G.items == 0) continue;
             float gms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&gms, S.ev_grp[gi], S.ev_gend[gi]), "event time");
+            if (!S.graphed) HIP_TRY(hipEventElapsedTime(&gms, S.ev_grp[gi], S.ev_gend[gi]), "event time");
             fpgm_sink_stats &A = ctx->sink[gi];
             A.launches += 1;
             A.tests += G.tests;
@@ -1327,6 +1383,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     }
     if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP")) ctx->group = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;

@@ -378,6 +378,21 @@ def test_grouped_level1_mfma_path(monkeypatch, case):
     assert out["1"]["tests_per_level"][1] > 0
 
 
+@pytest.mark.parametrize("key,rows,depth", [(0, None, 1000), (2, 20_000, 5)])
+def test_graph_mode_same_results(monkeypatch, key, rows, depth):
+    # FPGM_GRAPH=1: every launch sequence's device work captured into a HIP graph (the
+    # slot's executable updated in place when the shape repeats) and launched as one; many
+    # rounds (small round_tests) replay the graphs with the host's own work between them.
+    # Skeleton, sepsets and counts equal the oracle's
+    w = configs.get(key)
+    s = w.sample(rows)
+    o = O.pc_skeleton(O.Data.of(s), w.alpha, depth, n_threads=8)
+    monkeypatch.setenv("FPGM_GRAPH", "1")
+    with _ctx(s) as ctx:
+        for rt in (0, 64, 0):
+            _assert_same(ctx.pc_skeleton(w.alpha, depth, round_tests=rt), o)
+
+
 def test_skeleton_depth_and_alpha():
     w = configs.get(2)
     s = w.sample(20_000)
