@@ -293,6 +293,9 @@ struct fpgm_ctx {
     // default (measured no faster)
     int graph = 0;
     int64_t graph_instantiations = 0, graph_updates = 0;
+    // host columns uploaded through pinned staging, chunk by chunk, overlapped with their
+    // packing (FPGM_UPLOAD_PIPELINE=0: one pageable copy per instance, then the packing)
+    int upload_pipeline = 1;
     fpgm_sink_stats sink[kGroups] = {};  // per counting sink of the last call
     double grp_ms = 0.0, grp_ops = 0.0;  // grouped launches of the last call: time, int8 operations
     int64_t grp_tests = 0, grp_launches = 0;
@@ -1384,6 +1387,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP")) ctx->group = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_UPLOAD_PIPELINE")) ctx->upload_pipeline = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS"))
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
@@ -1413,9 +1417,101 @@ static bool rows_ok(int64_t n_rows, int64_t row_stride)
     return n_rows >= 0 && row_stride >= n_rows && n_rows <= ((int64_t)1 << 31) - 1;
 }
 
-// Packs the columns for the counting kernels, uploads the instance table and scans
-// every column for values >= card.
-static fpgm_status finish_ctx(fpgm_ctx *ctx)
+// Packing and value scan of columns [v0, v1) of instance I on the context stream.
+static fpgm_status pack_and_scan(fpgm_ctx *ctx, const InstHost &I, uint8_t *pcols, int v0, int v1)
+{
+    const int64_t o = (int64_t)v0 * I.stride;
+    if (pcols)
+        HIP_TRY(launch_pack_cols(I.cols + o, pcols + o, I.stride, v1 - v0, ctx->d_colw + v0, ctx->stream),
+                "launch pack");
+    HIP_TRY(launch_check_values(I.cols + o, I.stride, I.n_rows, v1 - v0, ctx->d_card + v0, ctx->d_flag.p, ctx->stream),
+            "launch check");
+    return FPGM_OK;
+}
+
+// Upload of host columns (instance i at cols[i], row_stride[i] bytes apart) into the
+// context's device columns, pipelined over chunks of consecutive columns in device order
+// (the instances lie back to back): the host copies chunk k+1 into one of two pinned
+// staging buffers -- each column's rows, zeros up to the device stride -- on several
+// threads while chunk k's copy runs on a side stream, and the context stream packs and
+// scans each chunk once its copy is done.  pcols: the packed copy's instance bases.
+static fpgm_status upload_and_scan(fpgm_ctx *ctx, const uint8_t *const *cols, const int64_t *row_stride,
+                                   const std::vector<uint8_t *> &pcols)
+{
+    const int n = ctx->n_vars;
+    const int64_t n_inst = (int64_t)ctx->inst.size();
+    int64_t widest = 0;
+    for (const InstHost &I : ctx->inst) widest = std::max(widest, I.stride);
+    const int64_t cap = std::max<int64_t>((int64_t)32 << 20, widest);
+    HostBuf<uint8_t> stage[2];
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    struct Cleanup {
+        fpgm_ctx *c;
+        HostBuf<uint8_t> *st;
+        hipEvent_t *ev;
+        ~Cleanup()
+        {
+            (void)hipStreamSynchronize(c->side[0]);
+            (void)hipStreamSynchronize(c->stream);
+            for (int b = 0; b < 2; ++b) {
+                st[b].release();
+                if (ev[b]) (void)hipEventDestroy(ev[b]);
+            }
+        }
+    } cleanup{ctx, stage, copied};
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(stage[b].ensure(cap, ctx->stream), "alloc pinned staging");
+        HIP_TRY(hipEventCreateWithFlags(&copied[b], hipEventDisableTiming), "event");
+    }
+    struct Col {
+        int32_t inst, v;
+    };
+    std::vector<Col> chunk;
+    int64_t i = 0, v = 0;
+    for (int64_t k = 0; i < n_inst; ++k) {
+        HostBuf<uint8_t> &S = stage[k & 1];
+        if (k >= 2) HIP_TRY(hipEventSynchronize(copied[k & 1]), "synchronize");  // its previous copy is done
+        // the chunk: whole columns in device order while they fit the staging buffer
+        chunk.clear();
+        int64_t bytes = 0;
+        uint8_t *dst = const_cast<uint8_t *>(ctx->inst[i].cols) + v * ctx->inst[i].stride;
+        while (i < n_inst && bytes + ctx->inst[i].stride <= cap) {
+            chunk.push_back(Col{(int32_t)i, (int32_t)v});
+            bytes += ctx->inst[i].stride;
+            if (++v == n) {
+                v = 0;
+                ++i;
+            }
+        }
+        const int nth = (bytes >= ((int64_t)4 << 20)) ? ctx->host_threads : 1;
+#pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
+        for (int64_t c = 0; c < (int64_t)chunk.size(); ++c) {
+            const InstHost &I = ctx->inst[chunk[c].inst];
+            // the chunk's columns are consecutive in device memory
+            uint8_t *o = S.p + ((I.cols + (int64_t)chunk[c].v * I.stride) - dst);
+            std::memcpy(o, cols[chunk[c].inst] + (int64_t)chunk[c].v * row_stride[chunk[c].inst], (size_t)I.n_rows);
+            std::memset(o + I.n_rows, 0, (size_t)(I.stride - I.n_rows));
+        }
+        HIP_TRY(hipMemcpyAsync(dst, S.p, (size_t)bytes, hipMemcpyHostToDevice, ctx->side[0]), "upload columns");
+        HIP_TRY(hipEventRecord(copied[k & 1], ctx->side[0]), "event");
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, copied[k & 1], 0), "stream wait");
+        // packing and value scan, one launch pair per instance segment of the chunk
+        for (size_t c0 = 0; c0 < chunk.size();) {
+            size_t c1 = c0;
+            while (c1 < chunk.size() && chunk[c1].inst == chunk[c0].inst) ++c1;
+            const int32_t ii = chunk[c0].inst;
+            fpgm_status st = pack_and_scan(ctx, ctx->inst[ii], pcols[ii], chunk[c0].v, chunk[c1 - 1].v + 1);
+            if (st != FPGM_OK) return st;
+            c0 = c1;
+        }
+    }
+    return FPGM_OK;
+}
+
+// Chooses the packed widths, uploads host columns (cols != nullptr; else the instances'
+// columns are already on the device), packs them for the counting kernels, uploads the
+// instance table and scans every column for values >= card.
+static fpgm_status finish_ctx(fpgm_ctx *ctx, const uint8_t *const *cols = nullptr, const int64_t *row_stride = nullptr)
 {
     const int n = ctx->n_vars;
     ctx->colw.assign((size_t)n, 8);
@@ -1435,8 +1531,7 @@ static fpgm_status finish_ctx(fpgm_ctx *ctx)
     HIP_TRY(hipMalloc((void **)&ctx->d_colw, (size_t)std::max(1, n)), "hipMalloc widths");
     HIP_TRY(hipMemcpyAsync(ctx->d_colw, ctx->colw.data(), (size_t)n, hipMemcpyHostToDevice, ctx->stream),
             "upload widths");
-    std::vector<const uint8_t *> pcols(ctx->inst.size());
-    for (size_t i = 0; i < pcols.size(); ++i) pcols[i] = ctx->inst[i].cols;
+    std::vector<uint8_t *> pcols(ctx->inst.size(), nullptr);
     if (any_packed) {
         // one allocation, each instance at the offset of its uint8 columns (same stride)
         int64_t total = 0;
@@ -1446,24 +1541,34 @@ static fpgm_status finish_ctx(fpgm_ctx *ctx)
             total += ctx->inst[i].stride * n;
         }
         HIP_TRY(hipMalloc((void **)&ctx->d_pcols, (size_t)std::max<int64_t>(total, 16)), "hipMalloc packed columns");
-        for (size_t i = 0; i < off.size(); ++i) {
+        for (size_t i = 0; i < off.size(); ++i) pcols[i] = ctx->d_pcols + off[i];
+    }
+    HIP_TRY(ctx->d_flag.ensure(1, true, ctx->stream), "alloc flag");
+    HIP_TRY(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(uint32_t), ctx->stream), "memset flag");
+    if (cols && ctx->upload_pipeline) {
+        fpgm_status st = upload_and_scan(ctx, cols, row_stride, pcols);
+        if (st != FPGM_OK) return st;
+    } else {
+        for (size_t i = 0; cols && i < ctx->inst.size(); ++i) {
             const InstHost &I = ctx->inst[i];
-            HIP_TRY(launch_pack_cols(I.cols, ctx->d_pcols + off[i], I.stride, n, ctx->d_colw, ctx->stream),
-                    "launch pack");
-            pcols[i] = ctx->d_pcols + off[i];
+            HIP_TRY(hipMemsetAsync(const_cast<uint8_t *>(I.cols), 0, (size_t)(I.stride * n), ctx->stream), "clear columns");
+            if (I.n_rows > 0)
+                HIP_TRY(hipMemcpy2DAsync(const_cast<uint8_t *>(I.cols), (size_t)I.stride, cols[i], (size_t)row_stride[i],
+                                         (size_t)I.n_rows, (size_t)n, hipMemcpyHostToDevice, ctx->stream),
+                        "upload columns");
+        }
+        for (size_t i = 0; i < ctx->inst.size(); ++i) {
+            fpgm_status st = pack_and_scan(ctx, ctx->inst[i], pcols[i], 0, n);
+            if (st != FPGM_OK) return st;
         }
     }
     std::vector<DevInst> di(ctx->inst.size());
     for (size_t i = 0; i < di.size(); ++i)
-        di[i] = DevInst{ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows, pcols[i]};
+        di[i] = DevInst{ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows,
+                        pcols[i] ? pcols[i] : ctx->inst[i].cols};
     HIP_TRY(hipMalloc((void **)&ctx->d_inst, sizeof(DevInst) * di.size()), "hipMalloc instances");
     HIP_TRY(hipMemcpyAsync(ctx->d_inst, di.data(), sizeof(DevInst) * di.size(), hipMemcpyHostToDevice, ctx->stream),
             "upload instances");
-    HIP_TRY(ctx->d_flag.ensure(1, true, ctx->stream), "alloc flag");
-    HIP_TRY(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(uint32_t), ctx->stream), "memset flag");
-    for (const InstHost &I : ctx->inst)
-        HIP_TRY(launch_check_values(I.cols, I.stride, I.n_rows, ctx->n_vars, ctx->d_card, ctx->d_flag.p, ctx->stream),
-                "launch check");
     uint32_t bad = 0;
     HIP_TRY(hipMemcpyAsync(&bad, ctx->d_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "flag");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -1496,18 +1601,11 @@ fpgm_status fpgm_ctx_create_multi(const uint8_t *const *cols, const int64_t *n_r
         } else {
             ctx->d_cols = d;
             ctx->owns_cols = true;
-            e = hipMemsetAsync(d, 0, (size_t)off[n_inst], ctx->stream);
-            for (int32_t i = 0; i < n_inst && e == hipSuccess; ++i) {
-                const int64_t stride = (off[i + 1] - off[i]) / n_vars;
-                ctx->inst.push_back(InstHost{d + off[i], stride, n_rows[i]});
-                if (n_rows[i] > 0)
-                    e = hipMemcpy2DAsync(d + off[i], (size_t)stride, cols[i], (size_t)row_stride[i], (size_t)n_rows[i],
-                                         (size_t)n_vars, hipMemcpyHostToDevice, ctx->stream);
-            }
-            if (e != hipSuccess) st = hip_fail(ctx, e, "upload columns");
+            for (int32_t i = 0; i < n_inst; ++i)
+                ctx->inst.push_back(InstHost{d + off[i], (off[i + 1] - off[i]) / n_vars, n_rows[i]});
         }
     }
-    if (st == FPGM_OK) st = finish_ctx(ctx);
+    if (st == FPGM_OK) st = finish_ctx(ctx, cols, row_stride);  // uploads, packs and scans the columns
     if (st != FPGM_OK) {
         fpgm_ctx_destroy(ctx);
         return st;
