@@ -427,12 +427,15 @@ __device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
 // key byte already carries the wave pair's bias (0, or kCells added to the key; see
 // lane_bias), and one v_perm_b32 puts key byte j into byte 1 and lane * 4 (lane4, byte 0)
 // into byte 0.
+// kShr: the offset is shifted right by 1 (the shared sink's [cell][32] column layout: the
+// key byte at byte 1 and (lane & 31) * 8 at byte 0, halved, give key * 128 + (lane & 31) * 4).
+template <int kShr = 0>
 __device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc, uint32_t K, int nvalid)
 {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
         if (j < nvalid) {
-            const uint32_t off = __builtin_amdgcn_perm(K, lane4, 0x0C0C0000u | ((4u + (uint32_t)j) << 8));
+            const uint32_t off = __builtin_amdgcn_perm(K, lane4, 0x0C0C0000u | ((4u + (uint32_t)j) << 8)) >> kShr;
             atomicAdd(reinterpret_cast<uint32_t *>(lb + off), inc);
         }
 }
@@ -465,7 +468,7 @@ __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *
 }
 
 // kOr: the bias is ORed into the Y bytes, else added to each key word (lane_bias)
-template <int D, int NW, bool kOr>
+template <int D, int NW, bool kOr, int kShr = 0>
 __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTest &T, const Shared &sh,
                                                   int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
                                                   uint32_t inc, uint32_t bias)
@@ -484,11 +487,11 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
         if (nv >= kRowsPerThread) {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, 4);
+                lane_put4<kShr>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, 4);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                lane_put4(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, (int)nv - 4 * q);
+                lane_put4<kShr>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, (int)nv - 4 * q);
         }
     };
     Cols<D, NW> b0, b1;
@@ -506,7 +509,7 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
 }
 
 // Runtime |Z|: d > 3, or a column of the test kept at 8 bits.
-template <bool kOr>
+template <bool kOr, int kShr = 0>
 __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevTest &T, const Shared &sh,
                                                     int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
                                                     uint32_t inc, uint32_t bias)
@@ -532,7 +535,7 @@ __device__ __forceinline__ void count_rows_lane_any(const DevInst &I, const DevT
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t k = pk_mad(pk_mad(K[q], cx2, word_of(vx, q)), cy2, word_of(vy, q)) + kbias;
-            lane_put4(lb, lane4, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
+            lane_put4<kShr>(lb, lane4, inc, k, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
         }
     }
 }
@@ -565,6 +568,42 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
         }
     }
     count_rows_lane_any<kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
+}
+
+// The shared sink's column layouts ([cell][64] words up to 128 cells, [cell][32] up to 256)
+// with the per-lane sinks' key arithmetic: keys below 256 formed four rows at a time,
+// a row's LDS offset by one byte permute (and a shift for 32 columns), every wave adding 1.
+template <bool kPacked>
+__device__ __forceinline__ void count_rows_shared_cols(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                       int64_t r_begin, int64_t r_end, int32_t *lds, int ncol)
+{
+    char *lb = reinterpret_cast<char *>(lds);
+    const uint32_t lane = threadIdx.x & 63;
+    constexpr int NW = kPacked ? 2 : 8;
+    const bool fixed = T.d <= 3 && (!kPacked || (uniform(T.flags) & kTFNarrow));
+    if (ncol == 64) {
+        const uint32_t lane4 = lane * 4u;
+        if (fixed) {
+            switch (T.d) {
+            case 0: count_rows_lane_d<0, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
+            case 1: count_rows_lane_d<1, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
+            case 2: count_rows_lane_d<2, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
+            default: count_rows_lane_d<3, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
+            }
+        }
+        count_rows_lane_any<true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u);
+    } else {
+        const uint32_t lane8 = (lane & 31u) * 8u;
+        if (fixed) {
+            switch (T.d) {
+            case 0: count_rows_lane_d<0, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
+            case 1: count_rows_lane_d<1, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
+            case 2: count_rows_lane_d<2, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
+            default: count_rows_lane_d<3, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
+            }
+        }
+        count_rows_lane_any<true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u);
+    }
 }
 
 // |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
@@ -1062,9 +1101,10 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             }
             for (int i = tid; i < span; i += kWG) lds[i] = 0;
             __syncthreads();
-            count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end,
-                                             col ? lds + (lane & (ncol - 1)) : lds + (copies > 1 ? my * stride : 0),
-                                             col ? (ncol == 64 ? 6u : 5u) : 0u);
+            if (col)
+                count_rows_shared_cols<kPacked>(I, T, sh, r_begin, r_end, lds, ncol);
+            else
+                count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end, lds + (copies > 1 ? my * stride : 0), 0u);
             __syncthreads();
             if (col) {
                 int32_t sum = 0;
