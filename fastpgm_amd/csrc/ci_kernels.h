@@ -124,7 +124,7 @@ constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 48 KiB (<= 96 c
 constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
 constexpr int kSinkLaneW128 = 5;  // kSinkLaneW's instance for tables <= 128 cells (a launch mode, not a group)
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
-constexpr int kColCells = kLdsCells / 32;  // shared sink's column layout up to 256 cells ([cell][32]; [cell][64] up to 128)
+constexpr int kColCells = kLdsCells / 32;  // shared sink column layouts: [cell][64] to 128 cells, [cell][32] to 256, [cell][16] to 512
 constexpr int64_t kColMinRows = 65536;     // ... for work items of at least this many rows
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 

@@ -570,9 +570,10 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
     count_rows_lane_any<kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
 }
 
-// The shared sink's column layouts ([cell][64] words up to 128 cells, [cell][32] up to 256)
-// with the per-lane sinks' key arithmetic: keys below 256 formed four rows at a time,
-// a row's LDS offset by one byte permute (and a shift for 32 columns), every wave adding 1.
+// The shared sink's column layouts: [cell][64] words up to 128 cells and [cell][32] up to
+// 256 with the per-lane sinks' key arithmetic (keys below 256 formed four rows at a time,
+// a row's LDS offset by one byte permute, and a shift for 32 columns, every wave adding
+// 1); [cell][16] up to 512 cells with the generic row loop, key << 4 from the lane's column.
 template <bool kPacked>
 __device__ __forceinline__ void count_rows_shared_cols(const DevInst &I, const DevTest &T, const Shared &sh,
                                                        int64_t r_begin, int64_t r_end, int32_t *lds, int ncol)
@@ -592,7 +593,7 @@ __device__ __forceinline__ void count_rows_shared_cols(const DevInst &I, const D
             }
         }
         count_rows_lane_any<true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u);
-    } else {
+    } else if (ncol == 32) {
         const uint32_t lane8 = (lane & 31u) * 8u;
         if (fixed) {
             switch (T.d) {
@@ -603,6 +604,8 @@ __device__ __forceinline__ void count_rows_shared_cols(const DevInst &I, const D
             }
         }
         count_rows_lane_any<true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u);
+    } else {
+        count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end, lds + (lane & 15u), 4u);
     }
 }
 
@@ -1074,12 +1077,13 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             __syncthreads();
             tab = small;
         } else if (kMode == kSinkShared) {
-            // Tables of at most kColCells cells: column layout, one table of [cell][64]
-            // words (up to 128 cells) or [cell][32] for the workgroup, lane l counting in
-            // column l (l & 31), key << 6 (5) from the lane's column, so the lanes of a wave
-            // (of a half-wave) always hit distinct banks, however skewed the data (waves meet
-            // only through the atomics, never within one instruction); then thread t sums
-            // cell t, reading its columns from (t + j) on: again one bank per lane.  Larger
+            // Tables of at most 2 * kColCells cells: column layout, one table of [cell][64]
+            // words (up to 128 cells), [cell][32] (up to 256) or [cell][16] for the
+            // workgroup, lane l counting in column l (l & 31, l & 15), key << 6 (5, 4) from
+            // the lane's column, so the lanes of a wave (of a half-wave, of a quarter) always
+            // hit distinct banks, however skewed the data (waves meet only through the
+            // atomics, never within one instruction); then thread t sums cell t (and
+            // t + 256), reading its columns from (t + j) on.  Larger
             // tables: k histogram copies shared by the waves (8 at most, 4 for items below
             // kColMinRows rows, whose clearing and reduction the counting does not repay;
             // fewer when they do not fit), lane l counting into copy l % k, so skewed data
@@ -1089,8 +1093,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             const int c32 = (int)cells;
             // (the column layout clears and reduces twice the words of the copies: it is
             // used for items of enough rows to repay that)
-            const bool col = c32 <= kColCells && r_end - r_begin >= kColMinRows;
-            const int ncol = (c32 <= kColCells / 2) ? 64 : 32;  // columns: every lane its own when they fit
+            const bool col = c32 <= 2 * kColCells && r_end - r_begin >= kColMinRows;
+            // columns: every lane its own when they fit
+            const int ncol = (c32 <= kColCells / 2) ? 64 : (c32 <= kColCells) ? 32 : 16;
             int copies = 1, my = 0, stride = c32, span = c32 * ncol;
             if (!col) {
                 stride = ((c32 + 63) & ~63) + 4;
@@ -1107,12 +1112,18 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
                 count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end, lds + (copies > 1 ? my * stride : 0), 0u);
             __syncthreads();
             if (col) {
-                int32_t sum = 0;
-                if (tid < c32)
+                int32_t sum[2] = {0, 0};
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int c = tid + k * kWG;
+                    if (c < c32)
 #pragma unroll 4
-                    for (int j = 0; j < ncol; ++j) sum += lds[tid * ncol + ((tid + j) & (ncol - 1))];
+                        for (int j = 0; j < ncol; ++j) sum[k] += lds[c * ncol + ((c + j) & (ncol - 1))];
+                }
                 __syncthreads();
-                if (tid < c32) lds[tid] = sum;
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    if (tid + k * kWG < c32) lds[tid + k * kWG] = sum[k];
                 __syncthreads();
             } else if (copies > 1) {
                 for (int i = tid; i < c32; i += kWG) {

@@ -574,19 +574,26 @@ def test_level0_gram_card_paths(card_hi):
     _assert_same(b, o)
 
 
-def test_mid_tables_many_steps_per_chunk():
-    # 300 003 rows and thousands of tests of 65..256 cells: every chunk spans tens of
+@pytest.mark.parametrize("lo,hi,skew", [(65, 256, False), (257, 512, False), (257, 512, True)],
+                         ids=["65-256", "257-512", "257-512-skewed"])
+def test_mid_tables_many_steps_per_chunk(lo, hi, skew):
+    # 300 003 rows and thousands of tests of lo..hi cells: every chunk spans tens of
     # 4096-row steps, through the 16-bit per-lane counters (65..96 cells, and 97..128 when
-    # the launch has no shared-sink table) and the shared sink's column layout (97..256)
+    # the launch has no shared-sink table) and the shared sink's column layouts ([cell][32]
+    # for 97..256 cells, [cell][16] for 257..512); skewed: nine rows in ten take value 0 in
+    # every column, so most lanes of an instruction count into one cell
     w = configs.get(3)
     net = w.network()
     s = datagen.forward_sample(net, 300_003, w.seed)
     card = s.card
     rng = np.random.default_rng(11)
+    if skew:
+        zero = rng.random(s.n_rows) < 0.9
+        s.cols[:, :s.n_rows][:, zero] = 0
     tests = []
     while len(tests) < 3000:
         vs = rng.choice(s.n_vars, size=3, replace=False)
-        if 65 <= int(np.prod(card[vs])) <= 256:
+        if lo <= int(np.prod(card[vs])) <= hi:
             x, y = sorted(int(v) for v in vs[:2])
             tests.append((x, y, (int(vs[2]),)))
     D = O.Data.of(s)
@@ -594,6 +601,8 @@ def test_mid_tables_many_steps_per_chunk():
     o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
     with _ctx(s) as ctx:
         r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+        for (x, y, z) in tests[:4]:
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
     assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
     np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
 
