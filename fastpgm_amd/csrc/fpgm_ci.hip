@@ -120,6 +120,10 @@ struct HostBuf {
 constexpr int kGroups = 5;
 constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
 constexpr int64_t kPackMinRows = 262144;  // mean rows per instance from which columns are packed
+// instances of at least this many rows count 65..128-cell tables in the shared sink's column
+// layout instead of the per-lane 16-bit counters (1000 variables x 10^6 rows: stage 175.7 ->
+// 168.4 ms, the two sinks' time 84.1 -> 77.3 ms; 300 variables x 5 * 10^5 rows: unchanged)
+constexpr int64_t kWideMaxRows = 262144;
 constexpr int kGroupPending = kGroups;     // a 97..128-cell table whose group is decided per launch
 constexpr int kGroupMfma = kGroups + 1;    // a level-1 test counted by the grouped MFMA kernel
 
@@ -278,7 +282,8 @@ struct fpgm_ctx {
     std::vector<fpgm_level_tune> tune;
     const fpgm_level_tune *tune_try = nullptr;
     int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
-    uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices
+    int64_t wide_max_rows = kWideMaxRows;  // FPGM_WIDE_MAX_ROWS
+    uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices; =nowide: bit 0; =nolane: bits 0, 2 (no per-lane sink)
     // grouped level-1 tests on the MFMA units (k_group_count) for instances of at least
     // group_min_rows rows: off unless FPGM_GROUP=1.  Measured slower than the LDS sinks on
     // level 1 at 1000 variables x 10^6 rows (1856 against 122 ms: in the best-grouped
@@ -392,18 +397,22 @@ double packed_row_bytes(const fpgm_ctx *ctx, const HostTest &h, const int32_t *z
 // launch it joins: 0 per-lane (<= 64 cells), 1 lane-wide (<= 96), kGroupPending (97..128:
 // lane-wide or shared, decided per launch), 2 shared (<= kLdsCells); -1 for larger tables.
 // Sets the table's kTF* bits; *fits: table and marginals fit the workgroup's LDS scratch.
-int lds_group(const fpgm_ctx *ctx, DevTest &T, int64_t cells, int64_t marg, bool *fits)
+int lds_group(const fpgm_ctx *ctx, DevTest &T, int64_t cells, int64_t marg, int64_t rows, bool *fits)
 {
-    if (cells <= kLaneCells) {
+    if (cells <= kLaneCells && !(ctx->sink_mask & 4)) {
         T.flags |= kTFLane;
         *fits = cells + marg <= kSmallCells;
         return 0;  // kSinkLane
     }
-    if (cells <= kLaneWideCells && !(ctx->sink_mask & 1)) {
+    // 65..128-cell tables of long instances count in the shared sink's [cell][64] column
+    // layout (32 KiB, 4 workgroups per CU), short ones per lane (their items are too short
+    // for the column layout to repay its clearing and reduction)
+    const bool wide = !(ctx->sink_mask & 1) && rows < ctx->wide_max_rows;
+    if (cells <= kLaneWideCells && wide) {
         *fits = cells + marg <= kSmallCells;
         return 1;  // kSinkLaneW
     }
-    if (cells <= kLaneWide128Cells && !(ctx->sink_mask & 1)) {
+    if (cells <= kLaneWide128Cells && wide) {
         *fits = cells + marg <= kSmallCells;
         return kGroupPending;
     }
@@ -490,7 +499,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         const int64_t marg = dimz * (T.cx + T.cy + 1);
         T.flags = tflags | (narrow ? kTFNarrow : 0);
         bool fits;
-        int gi = lds_group(ctx, T, cells, marg, &fits);
+        int gi = lds_group(ctx, T, cells, marg, ctx->inst[h.inst].n_rows, &fits);
         if (grp_on && h.d == 1) {
             // a level-1 test whose (u, z) rows and other columns fit the grouped kernel
             const int32_t u = h.side ? h.y : h.x, o = h.side ? h.x : h.y;
@@ -1389,8 +1398,12 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_UPLOAD_PIPELINE")) ctx->upload_pipeline = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
-    if (const char *e = std::getenv("FPGM_SINKS"))
+    if (const char *e = std::getenv("FPGM_WIDE_MAX_ROWS")) ctx->wide_max_rows = std::max<int64_t>(0, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_SINKS")) {
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
+        if (std::strcmp(e, "nowide") == 0) ctx->sink_mask |= 1;
+        if (std::strcmp(e, "nolane") == 0) ctx->sink_mask |= 5;
+    }
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipStream_t &sd : ctx->side) HIP_TRY(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ctx->ev0), "hipEventCreate");
@@ -2961,7 +2974,8 @@ extern "C" fpgm_status fpgm_ci_batch_noalloc(fpgm_ctx *ctx, const fpgm_test *tes
         }
         const int64_t cells = dimz * T.cx * T.cy;
         bool fits = false;
-        const int gi = cells <= kLdsCells ? lds_group(ctx, T, cells, dimz * (T.cx + T.cy + 1), &fits) : -1;
+        const int gi =
+            cells <= kLdsCells ? lds_group(ctx, T, cells, dimz * (T.cx + T.cy + 1), ctx->inst[0].n_rows, &fits) : -1;
         if (gi < 0) return fail(ctx, FPGM_ETOOBIG, "allocation-free calls take tables of at most 8192 cells");
         T.dimz = (int32_t)dimz;
         T.n_slices = 1;

@@ -5,7 +5,8 @@ oracle (oracle/liboracle.so, the CPU restatement) minutes to tens of minutes, to
 a test, so their results are computed once here and stored with the SHA-256 of the data
 they came from; the GPU tests regenerate the data, check the SHA-256 and compare.
 
-    make -C oracle && python tests/golden/make_stage_fixtures.py [--threads T] [3] [4]
+    make -C oracle && python tests/golden/make_stage_fixtures.py [--threads T] [--procs P]
+        [--resume DIR] [3] [4]
 
 Outputs (data only):
   stage_dag-1000v-1500e-1m.npz   data_sha256; alpha, depth; tests_per_level,
@@ -76,7 +77,21 @@ def stage_dag(key, threads):
     print(f"wrote {out} ({os.path.getsize(out)} bytes)", flush=True)
 
 
-def stage_ensemble(threads):
+def _replicate(args):
+    i, r, alpha, threads, resume = args
+    path = os.path.join(resume, f"rep{i:04d}.npz") if resume else None
+    if path and os.path.exists(path):
+        with np.load(path) as f:
+            return i, f["adj"], int(f["tests"])
+    o = O.pc_skeleton(O.Data.of(r), alpha, ENSEMBLE_DEPTH, n_threads=threads)
+    adj = o["adj"].astype(np.uint8)
+    if path:
+        np.savez(path + ".tmp.npz", adj=adj, tests=np.int64(o["num_ci_test"]))
+        os.replace(path + ".tmp.npz", path)
+    return i, adj, int(o["num_ci_test"])
+
+
+def stage_ensemble(threads, procs, resume):
     w = configs.get(4)
     base = w.sample()
     reps = list(w.replicates(base))
@@ -84,29 +99,43 @@ def stage_ensemble(threads):
     n = w.n_vars
     freq = np.zeros((n, n), dtype=np.int32)
     tests = np.zeros(len(reps), dtype=np.int64)
-    bits = []
+    bits = [None] * len(reps)
+    if resume:
+        os.makedirs(resume, exist_ok=True)
     t0 = time.time()
-    for i, r in enumerate(reps):
-        o = O.pc_skeleton(O.Data.of(r), w.alpha, ENSEMBLE_DEPTH, n_threads=threads)
-        freq += o["adj"]
-        tests[i] = o["num_ci_test"]
-        bits.append(pair_bits(o["adj"]))
-        if i % 32 == 31:
-            print(f"{w.name}: {i + 1} replicates, {int(tests[:i + 1].sum())} tests, {time.time() - t0:.0f} s",
-                  flush=True)
+    jobs = [(i, r, w.alpha, threads, resume) for i, r in enumerate(reps)]
+    done = 0
+    if procs > 1:
+        import multiprocessing as mp
+        pool = mp.get_context("fork").Pool(procs)
+        it = pool.imap_unordered(_replicate, jobs)
+    else:
+        pool, it = None, map(_replicate, jobs)
+    for i, adj, nt in it:
+        freq += adj
+        tests[i] = nt
+        bits[i] = pair_bits(adj)
+        done += 1
+        if done % 32 == 0:
+            print(f"{w.name}: {done} replicates, {time.time() - t0:.0f} s", flush=True)
+    if pool:
+        pool.close()
+        pool.join()
     out = os.path.join(HERE, f"stage_{w.name}-d{ENSEMBLE_DEPTH}.npz")
     np.savez_compressed(out, data_sha256=np.array(sha), alpha=np.float64(w.alpha), depth=np.int32(ENSEMBLE_DEPTH),
                         edge_freq=freq, num_ci_test=tests, adj_bits=np.stack(bits))
-    print(f"wrote {out} ({os.path.getsize(out)} bytes)", flush=True)
+    print(f"wrote {out} ({os.path.getsize(out)} bytes), {int(tests.sum())} tests", flush=True)
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 8)
+    ap.add_argument("--procs", type=int, default=1, help="replicates run in this many processes (ensemble)")
+    ap.add_argument("--resume", default="", help="directory keeping each replicate's result (ensemble)")
     ap.add_argument("which", nargs="*", type=int, default=[3, 4])
     a = ap.parse_args()
     for k in a.which:
         if k == 4:
-            stage_ensemble(a.threads)
+            stage_ensemble(a.threads, a.procs, a.resume)
         else:
             stage_dag(k, a.threads)

@@ -158,10 +158,11 @@ _LANE_WIDE_TESTS = [(0, 3, ()), (0, 4, ()), (0, 5, ()), (1, 6, ()), (1, 7, ()), 
                     (0, 4, (1,)), (1, 4, (0,)), (0, 5, (1,)), (1, 3, (2,)), (1, 5, (2,)), (2, 5, (1,))]
 
 
-@pytest.mark.parametrize("n_rows", [4099, 20_011, 262_147])
+@pytest.mark.parametrize("n_rows", [4099, 20_011, 262_143, 300_007])
 def test_lane_wide_sink_large_y(n_rows):
     # every test 65..96 cells (no 97..128-cell table in the launch, so the 48 KiB instance
-    # counts them), at least 4096 rows so that waves 2 and 3 count rows; through the
+    # counts them below 262 144 rows; from 262 144 rows the shared sink's [cell][64] column
+    # layout does), at least 4096 rows so that waves 2 and 3 count rows; through the
     # counts, the batch and a two-instance level 0 on the counting kernel
     s = _lane_wide_sample(n_rows, 17 + n_rows)
     D = O.Data.of(s)
@@ -574,17 +575,20 @@ def test_level0_gram_card_paths(card_hi):
     _assert_same(b, o)
 
 
-@pytest.mark.parametrize("lo,hi,skew", [(65, 256, False), (257, 512, False), (257, 512, True)],
-                         ids=["65-256", "257-512", "257-512-skewed"])
-def test_mid_tables_many_steps_per_chunk(lo, hi, skew):
-    # 300 003 rows and thousands of tests of lo..hi cells: every chunk spans tens of
-    # 4096-row steps, through the 16-bit per-lane counters (65..96 cells, and 97..128 when
-    # the launch has no shared-sink table) and the shared sink's column layouts ([cell][32]
-    # for 97..256 cells, [cell][16] for 257..512); skewed: nine rows in ten take value 0 in
-    # every column, so most lanes of an instruction count into one cell
+@pytest.mark.parametrize("lo,hi,skew,n_rows",
+                         [(65, 128, False, 200_003), (65, 256, False, 300_003), (257, 512, False, 300_003),
+                          (257, 512, True, 300_003)],
+                         ids=["65-128-per-lane", "65-256", "257-512", "257-512-skewed"])
+def test_mid_tables_many_steps_per_chunk(lo, hi, skew, n_rows):
+    # hundreds of thousands of rows and thousands of tests of lo..hi cells: every chunk spans
+    # tens of 4096-row steps.  Below 262 144 rows, 65..128-cell tables count in the 16-bit
+    # per-lane counters (65..96 cells, and 97..128 when the launch has no shared-sink table);
+    # from 262 144 rows in the shared sink's column layouts ([cell][64] up to 128 cells,
+    # [cell][32] for 129..256, [cell][16] for 257..512); skewed: nine rows in ten take
+    # value 0 in every column, so most lanes of an instruction count into one cell
     w = configs.get(3)
     net = w.network()
-    s = datagen.forward_sample(net, 300_003, w.seed)
+    s = datagen.forward_sample(net, n_rows, w.seed)
     card = s.card
     rng = np.random.default_rng(11)
     if skew:
