@@ -570,43 +570,20 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
     count_rows_lane_any<kOr>(I, T, sh, r_begin, r_end, lb, lane4, inc, bias);
 }
 
-// The shared sink's column layouts: [cell][64] words up to 128 cells and [cell][32] up to
-// 256 with the per-lane sinks' key arithmetic (keys below 256 formed four rows at a time,
-// a row's LDS offset by one byte permute, and a shift for 32 columns, every wave adding
-// 1); [cell][16] up to 512 cells with the generic row loop, key << 4 from the lane's column.
+// The column layouts of the per-lane sink's launch: [cell][64] int32 words up to 128 cells,
+// [cell][32] up to 256, with the per-lane key arithmetic (keys below 256 formed four rows
+// at a time, a row's LDS offset by one byte permute, and a shift for 32 columns), every
+// wave adding 1.  |Z| = 1 with every column in the launch's form (the host routes only
+// such tests here): with |Z| = 0, 2 and 3 as well the kernel exceeds 128 VGPRs.
 template <bool kPacked>
-__device__ __forceinline__ void count_rows_shared_cols(const DevInst &I, const DevTest &T, const Shared &sh,
-                                                       int64_t r_begin, int64_t r_end, int32_t *lds, int ncol)
+__device__ __forceinline__ void count_rows_cols(const DevInst &I, const DevTest &T, const Shared &sh,
+                                                int64_t r_begin, int64_t r_end, int32_t *lds, int ncol)
 {
     char *lb = reinterpret_cast<char *>(lds);
     const uint32_t lane = threadIdx.x & 63;
     constexpr int NW = kPacked ? 2 : 8;
-    const bool fixed = T.d <= 3 && (!kPacked || (uniform(T.flags) & kTFNarrow));
-    if (ncol == 64) {
-        const uint32_t lane4 = lane * 4u;
-        if (fixed) {
-            switch (T.d) {
-            case 0: count_rows_lane_d<0, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
-            case 1: count_rows_lane_d<1, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
-            case 2: count_rows_lane_d<2, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
-            default: count_rows_lane_d<3, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u); return;
-            }
-        }
-        count_rows_lane_any<true, 0>(I, T, sh, r_begin, r_end, lb, lane4, 1u, 0u);
-    } else if (ncol == 32) {
-        const uint32_t lane8 = (lane & 31u) * 8u;
-        if (fixed) {
-            switch (T.d) {
-            case 0: count_rows_lane_d<0, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
-            case 1: count_rows_lane_d<1, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
-            case 2: count_rows_lane_d<2, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
-            default: count_rows_lane_d<3, NW, true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u); return;
-            }
-        }
-        count_rows_lane_any<true, 1>(I, T, sh, r_begin, r_end, lb, lane8, 1u, 0u);
-    } else {
-        count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end, lds + (lane & 15u), 4u);
-    }
+    if (ncol == 64) count_rows_lane_d<1, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane * 4u, 1u, 0u);
+    else count_rows_lane_d<1, NW, true, 1>(I, T, sh, r_begin, r_end, lb, (lane & 31u) * 8u, 1u, 0u);
 }
 
 // |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
@@ -1043,7 +1020,27 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         const int64_t cells = (int64_t)T.dimz * T.cx * T.cy;
         int32_t *g = (T.ws_off >= 0) ? a.ws + T.ws_off : nullptr;
         const int32_t *tab = nullptr;  // the chunk's table in LDS, when there is one
-        if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128) {
+        if (kMode == kSinkLane && (uniform(T.flags) & kTFCol)) {
+            // column layout: one [cell][ncol] int32 table for the workgroup, lane l counting
+            // in column l (l & 31), so the lanes of a wave (of a half-wave) always hit
+            // distinct banks however skewed the data (waves meet only through the atomics,
+            // never within one instruction); then thread t sums cell t, reading its columns
+            // from (t + j) on
+            const int c32 = (int)cells;
+            const int ncol = (c32 <= kColCells / 2) ? 64 : 32;
+            for (int i = tid; i < c32 * ncol; i += kWG) lds[i] = 0;
+            __syncthreads();
+            count_rows_cols<kPacked>(I, T, sh, r_begin, r_end, lds, ncol);
+            __syncthreads();
+            int32_t sum = 0;
+            if (tid < c32)
+#pragma unroll 4
+                for (int j = 0; j < ncol; ++j) sum += lds[tid * ncol + ((tid + j) & (ncol - 1))];
+            __syncthreads();
+            if (tid < c32) lds[tid] = sum;
+            __syncthreads();
+            tab = lds;
+        } else if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128) {
             // per-lane 16-bit counters: [pair][cell][lane], even wave low half, odd wave high
             constexpr int kCells = (kMode == kSinkLaneW)      ? kLaneWideCells
                                    : (kMode == kSinkLaneW128) ? kLaneWide128Cells
@@ -1106,10 +1103,13 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             }
             for (int i = tid; i < span; i += kWG) lds[i] = 0;
             __syncthreads();
-            if (col)
-                count_rows_shared_cols<kPacked>(I, T, sh, r_begin, r_end, lds, ncol);
-            else
-                count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end, lds + (copies > 1 ? my * stride : 0), 0u);
+            {
+                // one row loop for every layout (a second inlined copy costs the kernel its
+                // 128-VGPR budget)
+                const uint32_t ksh = col ? (ncol == 64 ? 6u : ncol == 32 ? 5u : 4u) : 0u;
+                int32_t *hp = col ? lds + (lane & (ncol - 1)) : lds + (copies > 1 ? my * stride : 0);
+                count_rows<kSinkShared, kPacked>(I, T, sh, r_begin, r_end, hp, ksh);
+            }
             __syncthreads();
             if (col) {
                 int32_t sum[2] = {0, 0};

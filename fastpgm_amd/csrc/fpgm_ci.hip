@@ -283,7 +283,7 @@ struct fpgm_ctx {
     const fpgm_level_tune *tune_try = nullptr;
     int64_t max_slices = kMaxSlices;  // z slices before a table goes to the device-memory sink
     int64_t wide_max_rows = kWideMaxRows;  // FPGM_WIDE_MAX_ROWS
-    uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices; =nowide: bit 0; =nolane: bits 0, 2 (no per-lane sink)
+    uint32_t sink_mask = 0;  // FPGM_SINKS=shared: bit 0 no wide per-lane sink, bit 1 no z slices; =nowide: bit 0; =nolane: bits 0, 2 (no per-lane sink); =nocol: bit 3 (no column layout in the per-lane launch)
     // grouped level-1 tests on the MFMA units (k_group_count) for instances of at least
     // group_min_rows rows: off unless FPGM_GROUP=1.  Measured slower than the LDS sinks on
     // level 1 at 1000 variables x 10^6 rows (1856 against 122 ms: in the best-grouped
@@ -404,9 +404,18 @@ int lds_group(const fpgm_ctx *ctx, DevTest &T, int64_t cells, int64_t marg, int6
         *fits = cells + marg <= kSmallCells;
         return 0;  // kSinkLane
     }
-    // 65..128-cell tables of long instances count in the shared sink's [cell][64] column
-    // layout (32 KiB, 4 workgroups per CU), short ones per lane (their items are too short
-    // for the column layout to repay its clearing and reduction)
+    // 65..256-cell tables of long instances (|Z| = 1, columns of the launch's form) count
+    // in a column layout -- [cell][64] int32 words up to 128 cells, [cell][32] up to 256:
+    // 32 KiB, lane l in column l (l & 31) -- within the per-lane sink's launch, whose
+    // kernel holds its key arithmetic at 128 VGPRs; short ones per lane (their items are
+    // too short for the column layout to repay its clearing and reduction)
+    const bool packed = ctx->d_pcols != nullptr;
+    if (cells <= kColCells && rows >= ctx->wide_max_rows && T.d == 1 && (!packed || (T.flags & kTFNarrow)) &&
+        !(ctx->sink_mask & 8)) {
+        T.flags |= kTFCol | kTFLdsTable;
+        *fits = cells + marg <= kLdsCells;
+        return 0;  // kSinkLane, column layout
+    }
     const bool wide = !(ctx->sink_mask & 1) && rows < ctx->wide_max_rows;
     if (cells <= kLaneWideCells && wide) {
         *fits = cells + marg <= kSmallCells;
@@ -1403,6 +1412,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
         if (std::strcmp(e, "nowide") == 0) ctx->sink_mask |= 1;
         if (std::strcmp(e, "nolane") == 0) ctx->sink_mask |= 5;
+        if (std::strcmp(e, "nocol") == 0) ctx->sink_mask |= 8;
     }
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipStream_t &sd : ctx->side) HIP_TRY(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking), "hipStreamCreate");
@@ -2974,13 +2984,13 @@ extern "C" fpgm_status fpgm_ci_batch_noalloc(fpgm_ctx *ctx, const fpgm_test *tes
         }
         const int64_t cells = dimz * T.cx * T.cy;
         bool fits = false;
+        T.flags |= narrow ? kTFNarrow : 0;  // before the routing, which reads it
         const int gi =
             cells <= kLdsCells ? lds_group(ctx, T, cells, dimz * (T.cx + T.cy + 1), ctx->inst[0].n_rows, &fits) : -1;
         if (gi < 0) return fail(ctx, FPGM_ETOOBIG, "allocation-free calls take tables of at most 8192 cells");
         T.dimz = (int32_t)dimz;
         T.n_slices = 1;
         T.slice_cells = (int32_t)cells;
-        T.flags |= narrow ? kTFNarrow : 0;
         T.ws_off = -1;
         na_tests[i] = T;
         na_gidx[i] = gi;

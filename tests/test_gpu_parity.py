@@ -575,34 +575,37 @@ def test_level0_gram_card_paths(card_hi):
     _assert_same(b, o)
 
 
-@pytest.mark.parametrize("lo,hi,skew,n_rows",
-                         [(65, 128, False, 200_003), (65, 256, False, 300_003), (257, 512, False, 300_003),
-                          (257, 512, True, 300_003)],
-                         ids=["65-128-per-lane", "65-256", "257-512", "257-512-skewed"])
-def test_mid_tables_many_steps_per_chunk(lo, hi, skew, n_rows):
+@pytest.mark.parametrize("lo,hi,skew,n_rows,d",
+                         [(65, 128, False, 200_003, 1), (33, 256, False, 300_003, 1), (65, 256, True, 300_003, 1),
+                          (65, 256, False, 300_003, 2), (257, 512, False, 300_003, 1), (257, 512, True, 300_003, 1)],
+                         ids=["65-128-per-lane", "33-256-col", "65-256-col-skewed", "65-256-d2", "257-512",
+                              "257-512-skewed"])
+def test_mid_tables_many_steps_per_chunk(lo, hi, skew, n_rows, d):
     # hundreds of thousands of rows and thousands of tests of lo..hi cells: every chunk spans
     # tens of 4096-row steps.  Below 262 144 rows, 65..128-cell tables count in the 16-bit
-    # per-lane counters (65..96 cells, and 97..128 when the launch has no shared-sink table);
-    # from 262 144 rows in the shared sink's column layouts ([cell][64] up to 128 cells,
-    # [cell][32] for 129..256, [cell][16] for 257..512); skewed: nine rows in ten take
-    # value 0 in every column, so most lanes of an instruction count into one cell
+    # per-lane counters (65..96 cells, and 97..128 when the launch has no shared-sink table).
+    # From 262 144 rows, |Z| = 1 tables of 65..256 cells count in the per-lane launch's
+    # column layouts ([cell][64] up to 128 cells, [cell][32] up to 256), |Z| = 2 ones in the
+    # shared sink's (same layouts, its own row loop), 257..512 cells in the shared sink's
+    # [cell][16]; skewed: nine rows in ten take value 0 in every column, so most lanes of an
+    # instruction count into one cell
     w = configs.get(3)
     net = w.network()
     s = datagen.forward_sample(net, n_rows, w.seed)
     card = s.card
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(11 + d)
     if skew:
         zero = rng.random(s.n_rows) < 0.9
         s.cols[:, :s.n_rows][:, zero] = 0
     tests = []
     while len(tests) < 3000:
-        vs = rng.choice(s.n_vars, size=3, replace=False)
+        vs = rng.choice(s.n_vars, size=d + 2, replace=False)
         if lo <= int(np.prod(card[vs])) <= hi:
             x, y = sorted(int(v) for v in vs[:2])
-            tests.append((x, y, (int(vs[2]),)))
+            tests.append((x, y, tuple(sorted(int(v) for v in vs[2:]))))
     D = O.Data.of(s)
-    arr = np.array([[x, y, 1, z[0]] for (x, y, z) in tests], dtype=np.int32)
-    o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
+    arr = np.array([[x, y, d] + list(z) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, d, 0.05, n_threads=8)
     with _ctx(s) as ctx:
         r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
         for (x, y, z) in tests[:4]:
