@@ -276,6 +276,8 @@ struct fpgm_ctx {
         int pipeline = 1;                    // build round r+1 while the device counts round r
         int64_t gram_items_per_cu = 8;       // level-0 Gram work items (tile pairs x row splits) per CU
         int concurrent = 1;                  // slice and device-memory sinks on streams of their own
+        int lds_own_chunks = 0;              // FPGM_LDS_OWN_CHUNKS: the shared sink's group chunked by its own test count
+        int64_t side_cu_mult = 1;            // FPGM_SIDE_CU_MULT: LDS groups below this many items per CU on a side stream
     } sched;
     // per-level scheduling and launch shapes chosen by fpgm_autotune (index = level); a
     // candidate being timed by the tuner overrides them
@@ -571,7 +573,8 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         if (rows_per_chunk_forced > 0) {
             r = (rows_per_chunk_forced + kRowsPerStep - 1) / kRowsPerStep * kRowsPerStep;
         } else {
-            const int64_t u = (kGroupSink[g] == kSinkSlice || kGroupSink[g] == kSinkGlobal)
+            const int64_t u = (kGroupSink[g] == kSinkSlice || kGroupSink[g] == kSinkGlobal ||
+                               (ctx->sched.lds_own_chunks && kGroupSink[g] == kSinkShared))
                                   ? std::max<int64_t>(units[g], 1)
                                   : m;
             const int64_t cpt = std::min(steps, std::max<int64_t>(1, (target_items + u - 1) / u));
@@ -916,7 +919,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         if (grp_len[gi] == 0) continue;
         hipStream_t gs = ctx->stream;
         int side = kGroupSink[gi] == kSinkSlice ? 0 : kGroupSink[gi] == kSinkGlobal ? 1 : -1;
-        if (side < 0 && grp_len[gi] < ctx->n_cu) side = 2 + gi;  // lane, lane-wide, shared: 2, 3, 4
+        if (side < 0 && grp_len[gi] < ctx->n_cu * ctx->sched.side_cu_mult) side = 2 + gi;  // lane, lane-wide, shared: 2, 3, 4
         if (concurrent && side >= 0) {
             gs = ctx->side[side];
             HIP_TRY(hipStreamWaitEvent(gs, S.ev_fork, 0), "stream wait");
@@ -1407,6 +1410,8 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_UPLOAD_PIPELINE")) ctx->upload_pipeline = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_LDS_OWN_CHUNKS")) ctx->sched.lds_own_chunks = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_SIDE_CU_MULT")) ctx->sched.side_cu_mult = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_WIDE_MAX_ROWS")) ctx->wide_max_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS")) {
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;
