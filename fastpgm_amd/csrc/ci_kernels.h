@@ -53,7 +53,7 @@ constexpr int32_t kTFKeep = 8;       // leave the counted table in the workspace
 constexpr int32_t kTFLane = 16;      // per-lane 16-bit LDS counters (cells <= kLaneCells)
 constexpr int32_t kTFWide = 32;      // statistic by the grid-wide finalize kernels, not the counting kernel
 constexpr int32_t kTFNarrow = 64;    // every column of the test packed at 4 or 2 bits per value
-constexpr int32_t kTFCol = 128;      // per-lane sink launch, table in the column layout (<= kColCells cells, |Z| = 1)
+constexpr int32_t kTFCol = 128;      // per-lane sink launch, table in a column layout (<= 2 kColCells cells, |Z| = 1)
 
 // Grid-wide finalize of large device-memory tables: one workgroup per range of z slices
 // writes a partial, then one thread per test sums its partials in order.

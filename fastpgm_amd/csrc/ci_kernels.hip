@@ -440,6 +440,22 @@ __device__ __forceinline__ void lane_put4(char *lb, uint32_t lane4, uint32_t inc
         }
 }
 
+// The [cell][16] column layout of 257..512-cell tables with |Z| = 1: K holds the keys of
+// the 4 rows over (X, Y) only, x * c(Y) + y < 256, and Z is the page: a row's byte offset
+// is (z * c(X) c(Y) + x * c(Y) + y) * 64 + (lane & 15) * 4, the permuted low key
+// (lane16 = (lane & 15) * 16 in byte 0) shifted right by 2, plus z * page_bytes.
+__device__ __forceinline__ void lane_put4_paged(char *lb, uint32_t lane16, uint32_t K, uint32_t Zw,
+                                                uint32_t page_bytes, int nvalid)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < nvalid) {
+            const uint32_t lo = __builtin_amdgcn_perm(K, lane16, 0x0C0C0000u | ((4u + (uint32_t)j) << 8)) >> 2;
+            const uint32_t off = __builtin_amdgcn_ubfe(Zw, 8u * (uint32_t)j, 8u) * page_bytes + lo;
+            atomicAdd(reinterpret_cast<uint32_t *>(lb + off), 1u);
+        }
+}
+
 // The wave pair's bias enters every key byte in one of two ways (the key of a valid row
 // is below kCells, the biased key below 2 * kCells <= 256, so no byte carries into the
 // next; rows past the end only carry into later rows, which are past the end as well):
@@ -467,12 +483,15 @@ __device__ __forceinline__ uint32_t lane_key4(const Cols<D> &c, const uint32_t *
     return pk_mad(K, cy2, word_of(c.v[X + 1], q));
 }
 
-// kOr: the bias is ORed into the Y bytes, else added to each key word (lane_bias)
-template <int D, int NW, bool kOr, int kShr = 0>
+// kOr: the bias is ORed into the Y bytes, else added to each key word (lane_bias).
+// kPage (D = 1): the [cell][16] layout with Z as the page (lane_put4_paged; lane4 is
+// lane16 there, inc 1 and bias 0).
+template <int D, int NW, bool kOr, int kShr = 0, bool kPage = false>
 __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTest &T, const Shared &sh,
                                                   int64_t r_begin, int64_t r_end, char *lb, uint32_t lane4,
-                                                  uint32_t inc, uint32_t bias)
+                                                  uint32_t inc, uint32_t bias, uint32_t page_bytes = 0u)
 {
+    static_assert(!kPage || D == 1, "paged keys take |Z| = 1");
     const uint32_t ybias = kOr ? bias : 0u, kbias = kOr ? 0u : bias;
     const ColSet<D, NW> cs(I, T, sh);
     uint32_t zc2[D > 0 ? D : 1];
@@ -484,7 +503,13 @@ __device__ __forceinline__ void count_rows_lane_d(const DevInst &I, const DevTes
         Cols<D> cur;
         cs.unpack(raw, cur, ybias);
         const int64_t nv = r_end - r;
-        if (nv >= kRowsPerThread) {
+        if constexpr (kPage) {
+            constexpr int X = 1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                lane_put4_paged(lb, lane4, pk_mad(word_of(cur.v[X], q), cy2, word_of(cur.v[X + 1], q)),
+                                word_of(cur.v[0], q), page_bytes, nv >= kRowsPerThread ? 4 : (int)nv - 4 * q);
+        } else if (nv >= kRowsPerThread) {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 lane_put4<kShr>(lb, lane4, inc, lane_key4<D>(cur, zc2, cx2, cy2, q) + kbias, 4);
@@ -571,9 +596,10 @@ __device__ __forceinline__ void count_rows_lane(const DevInst &I, const DevTest 
 }
 
 // The column layouts of the per-lane sink's launch: [cell][64] int32 words up to 128 cells,
-// [cell][32] up to 256, with the per-lane key arithmetic (keys below 256 formed four rows
-// at a time, a row's LDS offset by one byte permute, and a shift for 32 columns), every
-// wave adding 1.  |Z| = 1 with every column in the launch's form (the host routes only
+// [cell][32] up to 256, [cell][16] up to 512, with the per-lane key arithmetic (keys below
+// 256 formed four rows at a time, a row's LDS offset by one byte permute, and a shift for
+// 32 columns; for 16 columns the key over (X, Y) and Z as the page), every wave adding 1.
+// |Z| = 1 with every column in the launch's form and c(X) c(Y) <= 256 (the host routes only
 // such tests here): with |Z| = 0, 2 and 3 as well the kernel exceeds 128 VGPRs.
 template <bool kPacked>
 __device__ __forceinline__ void count_rows_cols(const DevInst &I, const DevTest &T, const Shared &sh,
@@ -583,7 +609,10 @@ __device__ __forceinline__ void count_rows_cols(const DevInst &I, const DevTest 
     const uint32_t lane = threadIdx.x & 63;
     constexpr int NW = kPacked ? 2 : 8;
     if (ncol == 64) count_rows_lane_d<1, NW, true, 0>(I, T, sh, r_begin, r_end, lb, lane * 4u, 1u, 0u);
-    else count_rows_lane_d<1, NW, true, 1>(I, T, sh, r_begin, r_end, lb, (lane & 31u) * 8u, 1u, 0u);
+    else if (ncol == 32) count_rows_lane_d<1, NW, true, 1>(I, T, sh, r_begin, r_end, lb, (lane & 31u) * 8u, 1u, 0u);
+    else
+        count_rows_lane_d<1, NW, true, 0, true>(I, T, sh, r_begin, r_end, lb, (lane & 15u) * 16u, 1u, 0u,
+                                                (uint32_t)(T.cx * T.cy) * 64u);
 }
 
 // |Z| <= 3 with compile-time loops of the context's column form (the caller checks that
@@ -1027,17 +1056,23 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             // never within one instruction); then thread t sums cell t, reading its columns
             // from (t + j) on
             const int c32 = (int)cells;
-            const int ncol = (c32 <= kColCells / 2) ? 64 : 32;
+            const int ncol = (c32 <= kColCells / 2) ? 64 : (c32 <= kColCells) ? 32 : 16;
             for (int i = tid; i < c32 * ncol; i += kWG) lds[i] = 0;
             __syncthreads();
             count_rows_cols<kPacked>(I, T, sh, r_begin, r_end, lds, ncol);
             __syncthreads();
-            int32_t sum = 0;
-            if (tid < c32)
+            int32_t sum[2] = {0, 0};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int c = tid + k * kWG;
+                if (c < c32)
 #pragma unroll 4
-                for (int j = 0; j < ncol; ++j) sum += lds[tid * ncol + ((tid + j) & (ncol - 1))];
+                    for (int j = 0; j < ncol; ++j) sum[k] += lds[c * ncol + ((c + j) & (ncol - 1))];
+            }
             __syncthreads();
-            if (tid < c32) lds[tid] = sum;
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (tid + k * kWG < c32) lds[tid + k * kWG] = sum[k];
             __syncthreads();
             tab = lds;
         } else if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128) {

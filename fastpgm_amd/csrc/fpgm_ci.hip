@@ -406,14 +406,15 @@ int lds_group(const fpgm_ctx *ctx, DevTest &T, int64_t cells, int64_t marg, int6
         *fits = cells + marg <= kSmallCells;
         return 0;  // kSinkLane
     }
-    // 65..256-cell tables of long instances (|Z| = 1, columns of the launch's form) count
-    // in a column layout -- [cell][64] int32 words up to 128 cells, [cell][32] up to 256:
-    // 32 KiB, lane l in column l (l & 31) -- within the per-lane sink's launch, whose
-    // kernel holds its key arithmetic at 128 VGPRs; short ones per lane (their items are
-    // too short for the column layout to repay its clearing and reduction)
+    // 65..512-cell tables of long instances (|Z| = 1, columns of the launch's form) count
+    // in a column layout -- [cell][64] int32 words up to 128 cells, [cell][32] up to 256,
+    // [cell][16] up to 512 (then c(X) c(Y) <= 256, Z the page): 32 KiB, lane l in column
+    // l (l & 31, l & 15) -- within the per-lane sink's launch, whose kernel holds its key
+    // arithmetic at 128 VGPRs; short ones per lane (their items are too short for the
+    // column layout to repay its clearing and reduction)
     const bool packed = ctx->d_pcols != nullptr;
-    if (cells <= kColCells && rows >= ctx->wide_max_rows && T.d == 1 && (!packed || (T.flags & kTFNarrow)) &&
-        !(ctx->sink_mask & 8)) {
+    if (cells <= 2 * kColCells && (cells <= kColCells || (int64_t)T.cx * T.cy <= 256) &&
+        rows >= ctx->wide_max_rows && T.d == 1 && (!packed || (T.flags & kTFNarrow)) && !(ctx->sink_mask & 8)) {
         T.flags |= kTFCol | kTFLdsTable;
         *fits = cells + marg <= kLdsCells;
         return 0;  // kSinkLane, column layout

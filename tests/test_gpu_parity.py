@@ -577,17 +577,18 @@ def test_level0_gram_card_paths(card_hi):
 
 @pytest.mark.parametrize("lo,hi,skew,n_rows,d",
                          [(65, 128, False, 200_003, 1), (33, 256, False, 300_003, 1), (65, 256, True, 300_003, 1),
-                          (65, 256, False, 300_003, 2), (257, 512, False, 300_003, 1), (257, 512, True, 300_003, 1)],
-                         ids=["65-128-per-lane", "33-256-col", "65-256-col-skewed", "65-256-d2", "257-512",
-                              "257-512-skewed"])
+                          (65, 256, False, 300_003, 2), (257, 512, False, 300_003, 1), (257, 512, True, 300_003, 1),
+                          (257, 512, False, 300_003, 2)],
+                         ids=["65-128-per-lane", "33-256-col", "65-256-col-skewed", "65-256-d2", "257-512-paged",
+                              "257-512-paged-skewed", "257-512-d2"])
 def test_mid_tables_many_steps_per_chunk(lo, hi, skew, n_rows, d):
     # hundreds of thousands of rows and thousands of tests of lo..hi cells: every chunk spans
     # tens of 4096-row steps.  Below 262 144 rows, 65..128-cell tables count in the 16-bit
     # per-lane counters (65..96 cells, and 97..128 when the launch has no shared-sink table).
-    # From 262 144 rows, |Z| = 1 tables of 65..256 cells count in the per-lane launch's
-    # column layouts ([cell][64] up to 128 cells, [cell][32] up to 256), |Z| = 2 ones in the
-    # shared sink's (same layouts, its own row loop), 257..512 cells in the shared sink's
-    # [cell][16]; skewed: nine rows in ten take value 0 in every column, so most lanes of an
+    # From 262 144 rows, |Z| = 1 tables of 65..512 cells count in the per-lane launch's
+    # column layouts ([cell][64] up to 128 cells, [cell][32] up to 256, [cell][16] with Z as
+    # the page up to 512), |Z| = 2 ones in the shared sink's (same layouts, its own row
+    # loop); skewed: nine rows in ten take value 0 in every column, so most lanes of an
     # instruction count into one cell
     w = configs.get(3)
     net = w.network()
