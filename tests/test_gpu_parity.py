@@ -559,8 +559,11 @@ def test_level0_gram_multi_instance():
 def test_level0_gram_card_paths(card_hi):
     # cardinalities <= 8 form the one-hot operands by byte-table lookup, larger ones by
     # byte comparison; both level-0 results equal the counting kernel's and the oracle's.
-    # 5003 rows: a partial last 32-row step in every row split
-    s = datagen.random_sample(77 + card_hi, 40, 5003, 2, card_hi)
+    # 5003 rows: a partial last 32-row step in every row split; 75 variables (one-hot width
+    # not a multiple of the 128-row tile), variable 7 constant (cardinality 1)
+    s = datagen.random_sample(77 + card_hi, 75, 5003, 2, card_hi)
+    s.card[7] = 1
+    s.cols[7, :5003] = 0
     # a few dependent columns so that level 0 keeps some edges
     s.cols[1, :5003] = s.cols[0, :5003] % s.card[1]
     s.cols[3, :5003] = (s.cols[2, :5003] + s.cols[5, :5003]) % s.card[3]
