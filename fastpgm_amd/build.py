@@ -19,8 +19,8 @@ HOST_LIB = os.path.join(HERE, "libfpgm_host.so")
 HOST_CLI = os.path.join(HERE, "fpgm_pcstable")
 HOST_SOURCES = ["dataset.cpp", "network.cpp", "pc_stable.cpp", "bnsl_comparison.cpp", "param_learning.cpp", "host_c_api.cpp"]
 
-SOURCES = ["ci_kernels.hip", "gram_kernels.hip", "mle_kernels.hip", "fpgm_ci.hip"]
-HEADERS = ["ci_kernels.h", "gram_kernels.h", "mle_kernels.h", "chisq.h", "onehot.h"]
+SOURCES = ["ci_kernels.hip", "gram_kernels.hip", "mle_kernels.hip", "round_kernels.hip", "fpgm_ci.hip"]
+HEADERS = ["ci_kernels.h", "gram_kernels.h", "mle_kernels.h", "round_kernels.h", "chisq.h", "onehot.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

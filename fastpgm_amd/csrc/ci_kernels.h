@@ -129,6 +129,57 @@ constexpr int kColCells = kLdsCells / 32;  // shared sink column layouts: [cell]
 constexpr int64_t kColMinRows = 65536;     // ... for work items of at least this many rows
 constexpr int kMaxSlices = 8;              // more slices than this: device-memory sink (measured)
 
+// Sink routing of one table of at most kLdsCells cells, shared by the host's launch
+// builder and the device's round expansion.  Returns the group of the launch it joins:
+// 0 per-lane (<= 64 cells, or a column layout up to 512), 1 lane-wide (<= 96),
+// kRoutePending (97..128: lane-wide or shared, decided per launch), 2 shared
+// (<= kLdsCells); -1 for larger tables.  Sets the table's kTF* bits in flags (kTFNarrow
+// must be set already); *fits: table and marginals fit the workgroup's LDS scratch.
+struct RouteParams {
+    uint32_t sink_mask;     // bit 0: no lane-wide sink, bit 2: no per-lane sink, bit 3: no column layout
+    int32_t packed;         // the counting kernels read packed columns
+    int64_t wide_max_rows;  // instances of at least this many rows: column layout, no lane-wide sink
+};
+constexpr int kRoutePending = 5;
+
+__host__ __device__ inline int route_lds(const RouteParams &p, int32_t &flags, int64_t cells, int64_t marg,
+                                         int64_t rows, int32_t cx, int32_t cy, int32_t d, bool *fits)
+{
+    if (cells <= kLaneCells && !(p.sink_mask & 4)) {
+        flags |= kTFLane;
+        *fits = cells + marg <= kSmallCells;
+        return 0;
+    }
+    // 65..512-cell tables of long instances (|Z| = 1, columns of the launch's form) count
+    // in a column layout -- [cell][64] int32 words up to 128 cells, [cell][32] up to 256,
+    // [cell][16] up to 512 (then c(X) c(Y) <= 256, Z the page): 32 KiB, lane l in column
+    // l (l & 31, l & 15) -- within the per-lane sink's launch, whose kernel holds its key
+    // arithmetic at 128 VGPRs; short ones per lane (their items are too short for the
+    // column layout to repay its clearing and reduction)
+    if (cells <= 2 * (kLdsCells / 32) && (cells <= kLdsCells / 32 || (int64_t)cx * cy <= 256) &&
+        rows >= p.wide_max_rows && d == 1 && (!p.packed || (flags & kTFNarrow)) && !(p.sink_mask & 8)) {
+        flags |= kTFCol | kTFLdsTable;
+        *fits = cells + marg <= kLdsCells;
+        return 0;
+    }
+    const bool wide = !(p.sink_mask & 1) && rows < p.wide_max_rows;
+    if (cells <= kLaneWideCells && wide) {
+        *fits = cells + marg <= kSmallCells;
+        return 1;
+    }
+    if (cells <= kLaneWide128Cells && wide) {
+        *fits = cells + marg <= kSmallCells;
+        return kRoutePending;
+    }
+    if (cells <= kLdsCells) {
+        flags |= kTFLdsTable;
+        *fits = cells + marg <= kLdsCells;
+        return 2;
+    }
+    *fits = false;
+    return -1;
+}
+
 // Compaction of a round's work items before its counting launches: items of tests whose
 // edge was done in an earlier round are dropped (their result is marked skipped), the rest
 // keep their order within each 256-item tile; count[g] receives group g's live items.

@@ -74,3 +74,21 @@ def test_dag_1000v_1m_rows_full_stage():
         ctx.autotune(float(f["alpha"]), int(f["depth"]))
         g = ctx.pc_skeleton(float(f["alpha"]), int(f["depth"]))
         _check_stage_fixture(g, f, s.n_vars)
+
+
+def test_bootstrap_512_replicates_full_ensemble():
+    # all 512 bootstrap replicates (2 000 - 50 000 rows) of the 100-variable network at the
+    # bench depth (levels 0-2), level-synchronously in one context, against 512 independent
+    # oracle runs: the edge-frequency matrix, every replicate's skeleton and test count
+    w = configs.get(4)
+    f = stage_fixture(f"{w.name}-d3")
+    base = w.sample()
+    reps = list(w.replicates(base))
+    assert len(reps) == 512 == len(f["num_ci_test"])
+    assert ensemble_sha256(reps) == str(f["data_sha256"]), "regenerated replicates differ from the fixture's"
+    with _lib.Context.of_many(reps) as ctx:
+        g = ctx.pc_skeleton_multi(float(f["alpha"]), int(f["depth"]))
+    assert np.array_equal(g["num_ci_test_per_instance"], f["num_ci_test"])
+    assert np.array_equal(np.stack([_pair_bits(a) for a in g["adj"]]), f["adj_bits"])
+    assert np.array_equal(g["edge_freq"], f["edge_freq"])
+    assert g["num_ci_test"] == int(f["num_ci_test"].sum())

@@ -379,6 +379,50 @@ def test_grouped_level1_mfma_path(monkeypatch, case):
     assert out["1"]["tests_per_level"][1] > 0
 
 
+@pytest.mark.parametrize("case", ["alarm-5k", "dag-300v-20k", "dag-1000v-8k", "cards-6-12"])
+def test_device_rounds_same_results(monkeypatch, case):
+    # FPGM_DEVROUNDS=1: the rounds of every level whose tables fit LDS are planned per edge
+    # on the host and expanded, routed and reduced on the device; =0: built test by test on
+    # the host.  Both equal the oracle's stage: default rounds, small rounds (cut into row
+    # chunks with workspace tables), and FP64 decisions.  dag-1000v-8k's deeper levels have
+    # tables beyond LDS, so its stage switches to host rounds from level 3 on
+    if case == "alarm-5k":
+        w = configs.get(0)
+        s, alpha, depth = w.sample(), w.alpha, w.depth
+    elif case == "dag-300v-20k":
+        w = configs.get(2)
+        s, alpha, depth = w.sample(20_000), w.alpha, w.depth
+    elif case == "dag-1000v-8k":
+        w = configs.get(3)
+        s, alpha, depth = w.sample(8_000), w.alpha, 5
+    else:
+        net = datagen.make_network(77, 40, 90, 4, ("uniform", 6, 12), 0.4)
+        s, alpha, depth = datagen.forward_sample(net, 50_003, 77), 0.05, 3
+    o = O.pc_skeleton(O.Data.of(s), alpha, depth, n_threads=8)
+    for on in ("1", "0"):
+        monkeypatch.setenv("FPGM_DEVROUNDS", on)
+        with _ctx(s) as ctx:
+            for rt, fl in ((0, 0), (64, 0), (0, _lib.FPGM_F_EXACT)):
+                g = ctx.pc_skeleton(alpha, depth, round_tests=rt, flags=fl)
+                _assert_same(g, o)
+
+
+@pytest.mark.parametrize("on", ["1", "0"])
+def test_device_rounds_ensemble(monkeypatch, on):
+    # the bootstrap ensemble's level-synchronous stage (levels 0-2) with device and host
+    # rounds, default and small rounds: every replicate's skeleton and test count equal
+    # its own oracle run
+    w, reps = _ensemble(12, max_rows=12_000)
+    monkeypatch.setenv("FPGM_DEVROUNDS", on)
+    with _lib.Context.of_many(reps) as ctx:
+        runs = [ctx.pc_skeleton_multi(w.alpha, 3), ctx.pc_skeleton_multi(w.alpha, 3, round_tests=128)]
+    for i, r in enumerate(reps):
+        o = O.pc_skeleton(O.Data.of(r), w.alpha, 3, n_threads=8)
+        for g in runs:
+            assert np.array_equal(g["adj"][i], o["adj"]), i
+            assert g["num_ci_test_per_instance"][i] == o["num_ci_test"], i
+
+
 @pytest.mark.parametrize("key,rows,depth", [(0, None, 1000), (2, 20_000, 5)])
 def test_graph_mode_same_results(monkeypatch, key, rows, depth):
     # FPGM_GRAPH=1: every launch sequence's device work captured into a HIP graph (the
