@@ -2588,8 +2588,14 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
             while (b > 0 && b < ntv && R.tv[b].edge == R.tv[b - 1].edge) ++b;
             cb[c] = b;
         }
+        // the slot's descriptors and sink groups of a single-sequence round: the skipped
+        // tests come off the per-sink figures batch_wait added
+        const Slot &slr = ctx->slot[R.slot];
+        const bool have_groups = R.async && slr.mode_of.size() == R.tv.size();
         struct Part {
             double skipped_bytes = 0.0, skipped_pbytes = 0.0, max_err = 0.0;
+            double sk_rows[kGroups] = {}, sk_b[kGroups] = {}, sk_pb[kGroups] = {};
+            int64_t sk_t[kGroups] = {}, sk_items[kGroups] = {};
             int64_t host_skipped = 0, dev_skipped = 0, executed = 0, in_band = 0;
             bool any_done = false, bad = false;
             std::vector<std::pair<int32_t, size_t>> sep;  // (edge, test) of each edge's first independent test
@@ -2608,8 +2614,17 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
                     const double rows = (double)ctx->inst[active[g].first].n_rows;
                     Q.skipped_bytes += (double)(k1 - k) * (d + 2) * rows;
                     for (size_t j = k; j < k1; ++j) {
-                        Q.skipped_pbytes += packed_row_bytes(ctx, R.tv[j], R.zpool.data()) * rows;
+                        const double pb = packed_row_bytes(ctx, R.tv[j], R.zpool.data()) * rows;
+                        Q.skipped_pbytes += pb;
                         Q.dev_skipped += (r[j].df == kSkippedDf);
+                        const int gi = have_groups ? slr.mode_of[j] : -1;
+                        if (gi >= 0 && gi < kGroups) {
+                            Q.sk_t[gi] += 1;
+                            Q.sk_items[gi] += (int64_t)slr.h_tests.p[j].n_chunks * slr.h_tests.p[j].n_slices;
+                            Q.sk_rows[gi] += rows;
+                            Q.sk_b[gi] += (double)(d + 2) * rows;
+                            Q.sk_pb[gi] += pb;
+                        }
                     }
                     Q.host_skipped += (int64_t)(k1 - k);
                     k = k1;
@@ -2647,6 +2662,14 @@ fpgm_status run_level(fpgm_ctx *ctx, double alpha, int d, const fpgm_pc_opts &o,
             inb += Q.in_band;
             S.max_rel_err_g2_fp32 = std::max(S.max_rel_err_g2_fp32, Q.max_err);
             any_done = any_done || Q.any_done;
+            for (int gi = 0; gi < kGroups; ++gi) {
+                fpgm_sink_stats &A = ctx->sink[gi];
+                A.tests -= Q.sk_t[gi];
+                A.items -= Q.sk_items[gi];
+                A.row_tests -= Q.sk_rows[gi];
+                A.alg_bytes -= Q.sk_b[gi];
+                A.packed_bytes -= Q.sk_pb[gi];
+            }
             for (const auto &gk : Q.sep) {
                 PcInst &I = P[active[gk.first].first];
                 EdgeWalk &w = I.walks[active[gk.first].second];

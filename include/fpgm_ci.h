@@ -292,7 +292,9 @@ fpgm_status fpgm_last_gram_stats(const fpgm_ctx *ctx, double *gram_ms, double *i
  * (<= 8192 cells), 3 z-sliced LDS histogram, 4 device-memory table.  ms: summed HIP-event
  * durations of its launches; alg_bytes: (|Z| + 2) * rows per test; packed_bytes: the same
  * tests' packed column bytes; row_tests: rows summed over its tests (each row of each test
- * is one histogram increment). */
+ * is one histogram increment).  tests, items, bytes and row_tests count the tests the
+ * device counted: those of a speculative round whose edge an earlier round decided are
+ * dropped before counting and left out. */
 typedef struct {
     int32_t sink;
     int32_t pad;
