@@ -44,14 +44,17 @@ from fastpgm_amd import _lib, configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_OPS = 5.0e15   # dense int8 MFMA, MI355X
-SINKS = ["lane", "lane-wide", "shared", "slice", "device"]
+SINKS = ["lane", "lane-wide", "shared", "slice", "device", "bits"]
 # k_count_stat's template argument (the sink kind) -> the sink the library reports it under
-SINK_OF_KIND = {"2": "lane", "3": "lane-wide", "5": "lane-wide", "0": "shared", "4": "slice", "1": "device"}
+SINK_OF_KIND = {"2": "lane", "3": "lane-wide", "5": "lane-wide", "0": "shared", "4": "slice", "1": "device",
+                "6": "bits"}
 SINK_BOUND = {"lane": ("lds_atomics", "LDS atomic adds into lane-private 16-bit counters, one per row and test"),
               "lane-wide": ("lds_atomics", "LDS atomic adds into lane-private 16-bit counters, one per row and test"),
               "shared": ("lds_atomics", "LDS atomic adds into shared histogram copies, one per row and test"),
               "slice": ("lds_atomics", "LDS atomic adds into a z slice, one per row and test"),
-              "device": ("dev_atomics", "device-memory atomic adds, one per row and test")}
+              "device": ("dev_atomics", "device-memory atomic adds, one per row and test"),
+              "bits": ("valu", "VALU: per 32 rows an AND per (x, y) pair and an AND plus a population count per cell; "
+                               "rate in row-tests/s, no probe of its own")}
 TUNE_DIR = os.path.join(ROOT, "profiles")
 
 

@@ -62,7 +62,7 @@ class SinkStats(C.Structure):
                 ("row_tests", C.c_double)]
 
 
-SINK_NAMES = ["lane", "lane-wide", "shared", "slice", "device"]
+SINK_NAMES = ["lane", "lane-wide", "shared", "slice", "device", "bits"]
 
 
 class PcOpts(C.Structure):

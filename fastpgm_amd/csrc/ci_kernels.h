@@ -29,6 +29,11 @@ struct DevInst {
     int64_t stride;
     int64_t n_rows;
     const uint8_t *pcols;
+    // bit planes (large instances): plane (v, i) -- bit r of word r / 32 is bit i of row
+    // r's value -- at planes + (poff[v] + i) * pwords, nbits_of(card) planes per variable;
+    // rows past n_rows read as value 0; nullptr when the instance has none
+    const uint32_t *planes;
+    int64_t pwords;
 };
 
 // One test as the device sees it (48 bytes).
@@ -54,6 +59,12 @@ constexpr int32_t kTFLane = 16;      // per-lane 16-bit LDS counters (cells <= k
 constexpr int32_t kTFWide = 32;      // statistic by the grid-wide finalize kernels, not the counting kernel
 constexpr int32_t kTFNarrow = 64;    // every column of the test packed at 4 or 2 bits per value
 constexpr int32_t kTFCol = 128;      // per-lane sink launch, table in a column layout (<= 2 kColCells cells, |Z| = 1)
+constexpr int32_t kTFBits = 256;     // per-lane sink launch, counted from bit planes (|Z| = 1, small tables)
+constexpr int kBitsMaxCard = 16;     // variables with planes: cardinality at most this
+// planes of a variable of c states: bits of its largest value
+__host__ __device__ constexpr int nbits_of(int c) { return c <= 1 ? 0 : c <= 2 ? 1 : c <= 4 ? 2 : c <= 8 ? 3 : 4; }
+constexpr int kBitsWords = 4;        // plane words per thread and step (128 rows)
+constexpr int64_t kPlaneAlign = 1024;  // plane length in words: a multiple of one workgroup step (kWG * kBitsWords)
 
 // Grid-wide finalize of large device-memory tables: one workgroup per range of z slices
 // writes a partial, then one thread per test sums its partials in order.
@@ -90,6 +101,7 @@ struct LaunchArgs {
     const int32_t *zpool;
     const int32_t *card;
     const uint8_t *colw;     // bits per value of each packed column: 8, 4 or 2
+    const int32_t *poff;     // first bit plane of each variable (-1: none)
     const uint64_t *items;   // work items: test << 32 | (slice * n_chunks + chunk)
     int64_t item_begin;      // this launch's items: [item_begin, item_begin + n_items)
     int64_t n_items;         // (an upper bound when n_items_dev is set)
@@ -124,6 +136,7 @@ constexpr int kSinkLane = 2;    // per-lane 16-bit LDS counters (tiny tables)
 constexpr int kSinkLaneW = 3;   // per-lane 16-bit LDS counters, 48 KiB (<= 96 cells)
 constexpr int kSinkSlice = 4;   // large tables: one z slice (<= kSliceCells) per work item in LDS
 constexpr int kSinkLaneW128 = 5;  // kSinkLaneW's instance for tables <= 128 cells (a launch mode, not a group)
+constexpr int kSinkBits = 6;    // per-lane 16-bit LDS counters filled from bit planes (small |Z| = 1 tables)
 constexpr int kSliceCells = 2 * kLdsCells;  // 64 KiB LDS slice table
 constexpr int kColCells = kLdsCells / 32;  // shared sink column layouts: [cell][64] to 128 cells, [cell][32] to 256, [cell][16] to 512
 constexpr int64_t kColMinRows = 65536;     // ... for work items of at least this many rows
@@ -133,18 +146,39 @@ constexpr int kMaxSlices = 8;              // more slices than this: device-memo
 // builder and the device's round expansion.  Returns the group of the launch it joins:
 // 0 per-lane (<= 64 cells, or a column layout up to 512), 1 lane-wide (<= 96),
 // kRoutePending (97..128: lane-wide or shared, decided per launch), 2 shared
-// (<= kLdsCells); -1 for larger tables.  Sets the table's kTF* bits in flags (kTFNarrow
+// (<= kLdsCells), kRouteBits (bit planes); -1 for larger tables.  Sets the table's kTF* bits in flags (kTFNarrow
 // must be set already); *fits: table and marginals fit the workgroup's LDS scratch.
 struct RouteParams {
     uint32_t sink_mask;     // bit 0: no lane-wide sink, bit 2: no per-lane sink, bit 3: no column layout
     int32_t packed;         // the counting kernels read packed columns
     int64_t wide_max_rows;  // instances of at least this many rows: column layout, no lane-wide sink
+    int32_t bits_cells;     // |Z| = 1 tables of at most this many cells count from bit planes (0: none)
 };
-constexpr int kRoutePending = 5;
+constexpr int kRouteBits = 5;
+constexpr int kRoutePending = 6;
+
+// Cardinalities of a bit-plane table: sorted, the two smallest (P, Q) are one of the
+// counting kernel's instances -- (2, 2), (2, 3), (2, 4), (3, 3) -- and every variable has
+// planes.
+__host__ __device__ inline bool bits_shape(int64_t c0, int64_t c1, int64_t c2)
+{
+    if (c0 > c1) { const int64_t t = c0; c0 = c1; c1 = t; }
+    if (c1 > c2) { const int64_t t = c1; c1 = c2; c2 = t; }
+    if (c0 > c1) { const int64_t t = c0; c0 = c1; c1 = t; }
+    return c2 <= kBitsMaxCard && ((c0 == 2 && c1 >= 2 && c1 <= 4) || (c0 == 3 && c1 == 3));
+}
 
 __host__ __device__ inline int route_lds(const RouteParams &p, int32_t &flags, int64_t cells, int64_t marg,
                                          int64_t rows, int32_t cx, int32_t cy, int32_t d, bool *fits)
 {
+    // small |Z| = 1 tables of instances with bit planes: one AND and one population count
+    // per 32 rows and cell, no per-row LDS operation
+    if (p.bits_cells > 0 && d == 1 && cells <= p.bits_cells && cells <= kLaneCells &&
+        bits_shape(cx, cy, cells / ((int64_t)cx * cy)) && !(p.sink_mask & 4)) {
+        flags |= kTFBits | kTFLane;
+        *fits = cells + marg <= kSmallCells;
+        return kRouteBits;
+    }
     if (cells <= kLaneCells && !(p.sink_mask & 4)) {
         flags |= kTFLane;
         *fits = cells + marg <= kSmallCells;
@@ -190,9 +224,9 @@ struct CompactArgs {
     const int32_t *edge_done;
     int32_t round;
     DevResult *res;
-    int64_t begin[5];  // per sink group: first item (in and out use the same group regions)
-    int64_t len[5];
-    uint32_t *count;   // [5], zero before the launch
+    int64_t begin[6];  // per sink group: first item (in and out use the same group regions)
+    int64_t len[6];
+    uint32_t *count;   // [6], zero before the launch
 };
 
 // Grouped level-1 counting on the MFMA units.  The tests (x, y | z) of a round that share
@@ -249,5 +283,9 @@ hipError_t launch_check_values(const uint8_t *cols, int64_t stride, int64_t n_ro
 hipError_t launch_group_count(const GroupArgs &g, int n_a, hipStream_t s);
 // Zeroes n 32-bit words at p (a fill kernel: no memset node in code a caller may capture).
 hipError_t launch_zero(uint32_t *p, int64_t n, hipStream_t s);
+// Bit planes of one instance (see DevInst): planes of variable v at poff[v] (none when
+// poff[v] < 0), pwords words each.
+hipError_t launch_build_planes(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars, const int32_t *card,
+                               const int32_t *poff, uint32_t *planes, int64_t pwords, hipStream_t s);
 
 }  // namespace fpgm

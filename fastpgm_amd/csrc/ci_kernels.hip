@@ -666,6 +666,147 @@ __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, c
     else count_rows_any<kSink>(I, T, sh, r_begin, r_end, h, k);
 }
 
+// ---------------------------------------------------------------- bit-plane counting
+
+// One-hot masks of the P values of a variable over words [w0, w0 + kBitsWords) from its
+// nbits(P) planes: value p's mask is the AND of every plane or its complement, by p's bits.
+template <int P>
+__device__ __forceinline__ void onehot_words(const uint32_t *S, int64_t pw, int64_t w0, uint4 (&o)[P])
+{
+    constexpr int NB = nbits_of(P);
+    uint4 b[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) b[i] = *reinterpret_cast<const uint4 *>(S + i * pw + w0);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        uint4 m = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const bool one = (p >> i) & 1;
+            m.x &= one ? b[i].x : ~b[i].x;
+            m.y &= one ? b[i].y : ~b[i].y;
+            m.z &= one ? b[i].z : ~b[i].z;
+            m.w &= one ? b[i].w : ~b[i].w;
+        }
+        o[p] = m;
+    }
+}
+
+// A |Z| = 1 table from bit planes: with the three variables sorted by cardinality, A (P
+// states) and B (Q states, both compile-time) and C (R states, NBR planes), cell (p, q, r)
+// counts the rows where A = p, B = q and C = r: per 32-row word one AND per (p, q), the
+// value masks of C, and one AND and one population count per cell -- no per-row LDS
+// operation.  Each thread takes kBitsWords consecutive words per step and adds each
+// cell's count over them to its lane-private 16-bit counter of the n[z][x][y] cell
+// p mA + q mB + r mC (the per-lane sink's [pair][cell][lane] layout: even wave low half,
+// odd wave high).  Rows at or past n_rows read as value 0 in every plane, so the words
+// there are masked.
+template <int P, int Q, int NBR>
+__device__ __forceinline__ void count_bits_pq(const uint32_t *A, const uint32_t *B, const uint32_t *C, int64_t pw,
+                                              int R, int mA, int mB, int mC, int64_t w_b, int64_t w_e,
+                                              int64_t n_rows, uint32_t *pl)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint32_t *base = pl + (wave >> 1) * (kLaneCells * 64) + lane;
+    const uint32_t sh = (wave & 1) ? 16u : 0u;
+    for (int64_t w0 = w_b + (int64_t)tid * kBitsWords; w0 < w_e; w0 += (int64_t)kWG * kBitsWords) {
+        uint4 a[P], b[Q];
+        onehot_words<P>(A, pw, w0, a);
+        onehot_words<Q>(B, pw, w0, b);
+        if ((w0 + kBitsWords) * 32 > n_rows) {  // the instance's last rows: words past them clear
+            uint32_t v[kBitsWords];
+#pragma unroll
+            for (int j = 0; j < kBitsWords; ++j) {
+                const int64_t left = n_rows - (w0 + j) * 32;
+                v[j] = left >= 32 ? 0xffffffffu : left <= 0 ? 0u : ((1u << left) - 1u);
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                a[p].x &= v[0];
+                a[p].y &= v[1];
+                a[p].z &= v[2];
+                a[p].w &= v[3];
+            }
+        }
+        uint4 ab[P * Q];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                ab[p * Q + q] = make_uint4(a[p].x & b[q].x, a[p].y & b[q].y, a[p].z & b[q].z, a[p].w & b[q].w);
+        uint4 cb[NBR];
+#pragma unroll
+        for (int i = 0; i < NBR; ++i) cb[i] = *reinterpret_cast<const uint4 *>(C + i * pw + w0);
+#pragma unroll
+        for (int r = 0; r < (1 << NBR); ++r) {
+            if (r >= R) break;
+            uint4 c = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+            for (int i = 0; i < NBR; ++i) {
+                const bool one = (r >> i) & 1;
+                c.x &= one ? cb[i].x : ~cb[i].x;
+                c.y &= one ? cb[i].y : ~cb[i].y;
+                c.z &= one ? cb[i].z : ~cb[i].z;
+                c.w &= one ? cb[i].w : ~cb[i].w;
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const uint4 &u = ab[p * Q + q];
+                    const uint32_t n = __popc(u.x & c.x) + __popc(u.y & c.y) + __popc(u.z & c.z) + __popc(u.w & c.w);
+                    atomicAdd(base + (p * mA + q * mB + r * mC) * 64, n << sh);
+                }
+        }
+    }
+}
+
+// Rows [r_begin, r_end) of a bit-plane test (kTFBits: |Z| = 1, cardinalities accepted by
+// bits_shape) into the per-lane counters; r_begin is a multiple of 4096, so whole words
+// are counted.
+__device__ __forceinline__ void count_rows_bits(const LaunchArgs &a, const DevInst &I, const DevTest &T,
+                                                const Shared &sh, int64_t r_begin, int64_t r_end, uint32_t *pl)
+{
+    // the roles, sorted by cardinality: x (multiplier c(Y) in the cell index), y (1), z
+    // (c(X) c(Y))
+    int v0 = T.x, c0 = T.cx, m0 = T.cy;
+    int v1 = T.y, c1 = T.cy, m1 = 1;
+    int v2 = sh.zcol[0], c2 = sh.zcard[0], m2 = T.cx * T.cy;
+    auto swap3 = [](int &va, int &ca, int &ma, int &vb, int &cb, int &mb) {
+        if (ca > cb) {
+            int t = va; va = vb; vb = t;
+            t = ca; ca = cb; cb = t;
+            t = ma; ma = mb; mb = t;
+        }
+    };
+    swap3(v0, c0, m0, v1, c1, m1);
+    swap3(v1, c1, m1, v2, c2, m2);
+    swap3(v0, c0, m0, v1, c1, m1);
+    const int64_t pw = I.pwords;
+    const uint32_t *A = I.planes + (int64_t)a.poff[v0] * pw;
+    const uint32_t *B = I.planes + (int64_t)a.poff[v1] * pw;
+    const uint32_t *C = I.planes + (int64_t)a.poff[v2] * pw;
+    const int64_t w_b = r_begin >> 5, w_e = (r_end + 31) >> 5, nr = I.n_rows;
+#define FPGM_BITS_CASE(P, Q, NBR)                                                                     \
+    case (P * 16 + Q) * 8 + NBR:                                                                      \
+        count_bits_pq<P, Q, NBR>(A, B, C, pw, c2, m0, m1, m2, w_b, w_e, nr, pl);                      \
+        break;
+    switch (uniform((c0 * 16 + c1) * 8 + nbits_of(c2))) {
+        FPGM_BITS_CASE(2, 2, 1)
+        FPGM_BITS_CASE(2, 2, 2)
+        FPGM_BITS_CASE(2, 2, 3)
+        FPGM_BITS_CASE(2, 2, 4)
+        FPGM_BITS_CASE(2, 3, 2)
+        FPGM_BITS_CASE(2, 3, 3)
+        FPGM_BITS_CASE(2, 4, 2)
+        FPGM_BITS_CASE(2, 4, 3)
+        FPGM_BITS_CASE(3, 3, 2)
+        FPGM_BITS_CASE(3, 3, 3)
+    default: break;  // routing (bits_shape) sends no other shape here
+    }
+#undef FPGM_BITS_CASE
+}
+
 // Critical value of the chi-square test at the call's alpha: tabulated (exact FP64
 // inversion on the host) for df <= kCritTab, Wilson-Hilferty beyond, whose relative
 // error there is below 1e-6 (far inside the recheck band).
@@ -959,7 +1100,7 @@ struct LdsWords {
 // held to 128 VGPRs; the 64 KiB sinks fit 2 workgroups per CU whatever their registers.
 template <int kMode, bool kPacked>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
-    (kMode == kSinkLane || kMode == kSinkShared) ? 4 : 1))) void k_count_stat(LaunchArgs a)
+    (kMode == kSinkLane || kMode == kSinkShared || kMode == kSinkBits) ? 4 : 1))) void k_count_stat(LaunchArgs a)
 {
     __shared__ int32_t lds[LdsWords<kMode>::value];
     __shared__ int32_t small[kSmallCells];
@@ -1075,7 +1216,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
                 if (tid + k * kWG < c32) lds[tid + k * kWG] = sum[k];
             __syncthreads();
             tab = lds;
-        } else if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128) {
+        } else if (kMode == kSinkLane || kMode == kSinkLaneW || kMode == kSinkLaneW128 || kMode == kSinkBits) {
             // per-lane 16-bit counters: [pair][cell][lane], even wave low half, odd wave high
             constexpr int kCells = (kMode == kSinkLaneW)      ? kLaneWideCells
                                    : (kMode == kSinkLaneW128) ? kLaneWide128Cells
@@ -1089,7 +1230,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
                 pl[kCells * 64 + i] = 0;
             }
             __syncthreads();
-            count_rows_lane<kCells, kPacked>(I, T, sh, r_begin, r_end, pl);
+            if (kMode == kSinkBits) count_rows_bits(a, I, T, sh, r_begin, r_end, pl);
+            else count_rows_lane<kCells, kPacked>(I, T, sh, r_begin, r_end, pl);
             __syncthreads();
             // sum the 256 private counters of every cell: 8 threads per cell, each over
             // 8 lanes of both pairs (both halves), then a 3-step shuffle reduction
@@ -1669,6 +1811,39 @@ __global__ __launch_bounds__(kWG) void k_zero(uint32_t *p, int64_t n)
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) p[i] = 0u;
 }
 
+// Bit planes of one instance: one thread per (variable, 32-row word) reads the word's 32
+// values and writes one word per plane of the variable (plane i: bit i of every value);
+// words past the rows are written clear.  32 w < n_rows implies 32 w + 32 <= stride
+// (strides are multiples of 256).
+__global__ __launch_bounds__(kWG) void k_build_planes(const uint8_t *cols, int64_t stride, int64_t n_rows,
+                                                      const int32_t *card, const int32_t *poff, uint32_t *planes,
+                                                      int64_t pwords)
+{
+    const int v = blockIdx.y;
+    const int32_t base = poff[v];
+    if (base < 0) return;
+    const int nb = nbits_of(card[v]);
+    for (int64_t w = (int64_t)blockIdx.x * kWG + threadIdx.x; w < pwords; w += (int64_t)gridDim.x * kWG) {
+        const int64_t r0 = w * 32;
+        uint32_t val[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        uint32_t valid = 0u;
+        if (r0 < n_rows) {
+            const uint4 q0 = ld16(cols + (int64_t)v * stride + r0);
+            const uint4 q1 = ld16(cols + (int64_t)v * stride + r0 + 16);
+            val[0] = q0.x; val[1] = q0.y; val[2] = q0.z; val[3] = q0.w;
+            val[4] = q1.x; val[5] = q1.y; val[6] = q1.z; val[7] = q1.w;
+            const int64_t nv = n_rows - r0;
+            valid = nv >= 32 ? 0xffffffffu : ((1u << nv) - 1u);
+        }
+        for (int i = 0; i < nb; ++i) {
+            uint32_t m = 0u;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) m |= ((val[j >> 2] >> ((j & 3) * 8 + i)) & 1u) << j;
+            planes[(int64_t)(base + i) * pwords + w] = m & valid;
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_group_count(const GroupArgs &g, int n_a, hipStream_t s)
@@ -1683,6 +1858,16 @@ hipError_t launch_group_count(const GroupArgs &g, int n_a, hipStream_t s)
         if (perm) hipLaunchKernelGGL((k_group_count<true, 2>), dim3((unsigned)grid), dim3(kWG), 0, s, g);
         else hipLaunchKernelGGL((k_group_count<false, 2>), dim3((unsigned)grid), dim3(kWG), 0, s, g);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_build_planes(const uint8_t *cols, int64_t stride, int64_t n_rows, int n_vars, const int32_t *card,
+                               const int32_t *poff, uint32_t *planes, int64_t pwords, hipStream_t s)
+{
+    if (n_vars <= 0 || pwords <= 0) return hipSuccess;
+    const int64_t bx = std::min<int64_t>((pwords + kWG - 1) / kWG, 256);
+    hipLaunchKernelGGL(k_build_planes, dim3((unsigned)bx, (unsigned)n_vars), dim3(kWG), 0, s, cols, stride, n_rows, card,
+                       poff, planes, pwords);
     return hipGetLastError();
 }
 
@@ -1705,6 +1890,7 @@ void launch_count_stat_form(const LaunchArgs &a, int mode, int grid, hipStream_t
         break;
     case kSinkSlice: hipLaunchKernelGGL((k_count_stat<kSinkSlice, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
     case kSinkShared: hipLaunchKernelGGL((k_count_stat<kSinkShared, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
+    case kSinkBits: hipLaunchKernelGGL((k_count_stat<kSinkBits, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
     default: hipLaunchKernelGGL((k_count_stat<kSinkGlobal, kPacked>), dim3(grid), dim3(kWG), 0, s, a); break;
     }
 }
@@ -1720,7 +1906,7 @@ hipError_t launch_count_stat(const LaunchArgs &a, int mode, bool packed, int gri
 hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s)
 {
     if (max_len <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((max_len + kWG - 1) / kWG), 5), dim3(kWG), 0, s, c);
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((max_len + kWG - 1) / kWG), 6), dim3(kWG), 0, s, c);
     return hipGetLastError();
 }
 

@@ -118,14 +118,17 @@ struct HostBuf {
 };
 
 // Counting launches are grouped by sink kind, in this order.
-constexpr int kGroups = 5;
-constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal};
+constexpr int kGroups = 6;
+constexpr int kGroupSink[kGroups] = {kSinkLane, kSinkLaneW, kSinkShared, kSinkSlice, kSinkGlobal, kSinkBits};
+constexpr int kGroupBits = 5;   // the bit-plane launch (grid shape of the per-lane launch)
+constexpr int kTuneGroups = 5;  // groups with a grid shape of their own (fpgm_level_tune::wg_per_cu)
 constexpr int64_t kPackMinRows = 262144;  // mean rows per instance from which columns are packed
 // instances of at least this many rows count 65..128-cell tables in the shared sink's column
 // layout instead of the per-lane 16-bit counters (1000 variables x 10^6 rows: stage 175.7 ->
 // 168.4 ms, the two sinks' time 84.1 -> 77.3 ms; 300 variables x 5 * 10^5 rows: unchanged)
 constexpr int64_t kWideMaxRows = 262144;
 constexpr int kGroupPending = kGroups;     // a 97..128-cell table whose group is decided per launch
+static_assert(kGroupPending == kRoutePending && kGroupBits == kRouteBits, "routing groups");
 constexpr int kGroupMfma = kGroups + 1;    // a level-1 test counted by the grouped MFMA kernel
 
 // Buffers of one in-flight batch.  Two slots let the host build one batch while the
@@ -266,7 +269,7 @@ struct InstHost {
 struct fpgm_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side[5] = {};  // streams of a round's slice, device-memory and small LDS-sink launches
+    hipStream_t side[6] = {};  // streams of a round's slice, device-memory and small LDS-sink launches
     const uint8_t *d_cols = nullptr;  // all instances' columns (owned unless borrowed)
     bool owns_cols = false;
     std::vector<InstHost> inst;       // dataset instances (1, or the bootstrap ensemble)
@@ -281,6 +284,14 @@ struct fpgm_ctx {
     std::vector<uint8_t> colw;
     uint8_t *d_colw = nullptr;
     uint8_t *d_pcols = nullptr;  // owned; nullptr when every column has 8 bits (the uint8 columns are read)
+    // bit planes of every instance (DevInst::planes), built with the packed copy for
+    // variables of at most kBitsMaxCard states: |Z| = 1 tables of at most bits_cells cells
+    // count from them.  FPGM_BITS=0 builds none; FPGM_BITS_CELLS sets the bound
+    int bits = 1;
+    int32_t bits_cells = 48;
+    uint32_t *d_planes = nullptr;
+    int32_t *d_poff = nullptr;
+    int64_t planes_bytes = 0;
     std::string err;
     std::mutex mu;
     int n_cu = 256;
@@ -431,18 +442,24 @@ double packed_row_bytes(const fpgm_ctx *ctx, const HostTest &h, const int32_t *z
 // launch it joins: 0 per-lane (<= 64 cells), 1 lane-wide (<= 96), kGroupPending (97..128:
 // lane-wide or shared, decided per launch), 2 shared (<= kLdsCells); -1 for larger tables.
 // Sets the table's kTF* bits; *fits: table and marginals fit the workgroup's LDS scratch.
+// grid workgroups per CU of a group's launch (the bit-plane launch takes the per-lane one's)
+int64_t wpc_of(const fpgm_ctx *ctx, int g)
+{
+    return ctx->sched.wpc[g == kGroupBits ? 0 : g];
+}
+
 RouteParams route_params(const fpgm_ctx *ctx)
 {
     RouteParams p{};
     p.sink_mask = ctx->sink_mask;
     p.packed = ctx->d_pcols != nullptr ? 1 : 0;
     p.wide_max_rows = ctx->wide_max_rows;
+    p.bits_cells = ctx->d_planes ? ctx->bits_cells : 0;
     return p;
 }
 
 int lds_group(const fpgm_ctx *ctx, DevTest &T, int64_t cells, int64_t marg, int64_t rows, bool *fits)
 {
-    static_assert(kRoutePending == kGroupPending, "routing groups");
     static_assert(kColCells == kLdsCells / 32, "column-layout cells");
     return route_lds(route_params(ctx), T.flags, cells, marg, rows, T.cx, T.cy, T.d, fits);
 }
@@ -863,6 +880,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     a.zpool = S.d_zpool.p;
     a.card = ctx->d_card;
     a.colw = ctx->d_colw;
+    a.poff = ctx->d_poff;
     a.items = S.d_items.p;
     a.ws = ctx->d_ws.p;
     a.done = ctx->d_done.p;
@@ -925,12 +943,13 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     for (int gi = 0; gi < kGroups; ++gi) n_grp += (grp_len[gi] > 0);
     const bool concurrent = ctx->sched.concurrent && n_grp > 1;
     if (concurrent) HIP_TRY(hipEventRecord(S.ev_fork, ctx->stream), "event");
-    bool joined[kGroups] = {false, false, false, false, false};
+    bool joined[kGroups] = {false, false, false, false, false, false};
     for (int gi = 0; gi < kGroups; ++gi) {
         if (grp_len[gi] == 0) continue;
         hipStream_t gs = ctx->stream;
         int side = kGroupSink[gi] == kSinkSlice ? 0 : kGroupSink[gi] == kSinkGlobal ? 1 : -1;
-        if (side < 0 && grp_len[gi] < ctx->n_cu * ctx->sched.side_cu_mult) side = 2 + gi;  // lane, lane-wide, shared: 2, 3, 4
+        if (side < 0 && grp_len[gi] < ctx->n_cu * ctx->sched.side_cu_mult)
+            side = gi == kGroupBits ? 5 : 2 + gi;  // lane, lane-wide, shared: 2, 3, 4; bit planes: 5
         if (concurrent && side >= 0) {
             gs = ctx->side[side];
             HIP_TRY(hipStreamWaitEvent(gs, S.ev_fork, 0), "stream wait");
@@ -943,7 +962,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         a.queue = S.d_queue.p + gi * kXcds;
         a.n_items_dev = a.edge_done ? live_count + gi : nullptr;
         const int sink = (gi == 1 && lane_w128) ? kSinkLaneW128 : kGroupSink[gi];
-        const int64_t wg_per_cu = ctx->sched.wpc[gi];
+        const int64_t wg_per_cu = wpc_of(ctx, gi);
         int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wg_per_cu);
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
         HIP_TRY(launch_count_stat(a, sink, ctx->d_pcols != nullptr, (int)grid, gs), "launch count_stat");
@@ -1033,14 +1052,14 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
         S.bytes = 0.0;
         S.pbytes = 0.0;
         for (int gi = 0; gi < kGroups; ++gi) S.grp[gi] = Slot::GroupInfo{};
-        for (int gi = 0; gi < kRouteGroups; ++gi) {
-            Slot::GroupInfo &G = S.grp[gi];
-            G.tests = (int64_t)R.tests[gi];
-            G.items = (int64_t)R.items[gi];
+        for (int dg = 0; dg < kRouteGroups; ++dg) {
+            Slot::GroupInfo &G = S.grp[dg == round_group(kRouteBits) ? kGroupBits : dg];
+            G.tests = (int64_t)R.tests[dg];
+            G.items = (int64_t)R.items[dg];
             G.dsum = G.tests * S.dev_d;
-            G.rows = (double)R.rows[gi];
-            G.bytes = (double)R.alg[gi];
-            G.pbytes = (double)R.pbits[gi] / 8.0;
+            G.rows = (double)R.rows[dg];
+            G.bytes = (double)R.alg[dg];
+            G.pbytes = (double)R.pbits[dg] / 8.0;
             S.bytes += G.bytes;
             S.pbytes += G.pbytes;
         }
@@ -1063,7 +1082,7 @@ fpgm_status batch_wait(fpgm_ctx *ctx, Slot &S)
                     gms > 0 ? S.g_ops / (gms * 1e9) : 0.0);
     }
     {
-        static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device"};
+        static const char *names[kGroups] = {"lane", "lane-wide", "shared", "slice", "device", "bits"};
         for (int gi = 0; gi < kGroups; ++gi) {
             const Slot::GroupInfo &G = S.grp[gi];
             if (G.items == 0) continue;
@@ -1095,7 +1114,7 @@ struct DevLevel {
     int64_t need_max = 0;           // largest table + marginals of the level (int32 entries, a multiple of 64)
     bool w128_shared = false;       // 97..128-cell tables go to the shared group
     bool lane_w128 = false;         // the lane-wide group runs the 64 KiB instance
-    bool launch[kRouteGroups] = {true, true, true};  // sink groups the level's tables can reach
+    bool launch[kRouteGroups] = {true, true, true, true};  // sink groups the level's tables can reach
     int64_t walk_tests = 0;         // tests of every edge's whole walk (the level's most), saturated
     std::vector<int32_t> nbr;       // every running instance's frozen adjacency, concatenated
     std::vector<int64_t> nbr_base;  // instance -> its first entry in nbr
@@ -1167,7 +1186,8 @@ fpgm_status dev_round_submit(fpgm_ctx *ctx, Slot &S, const DevLevel &L, const st
     e.edge_done = ctx->d_edge_done.p;
     e.round = round;
     e.w128_shared = L.w128_shared ? 1 : 0;
-    e.launch_mask = (L.launch[0] ? 1u : 0u) | (L.launch[1] ? 2u : 0u) | (L.launch[2] ? 4u : 0u);
+    e.launch_mask = 0u;
+    for (int dg = 0; dg < kRouteGroups; ++dg) e.launch_mask |= L.launch[dg] ? 1u << dg : 0u;
     e.rp = route_params(ctx);
     e.rows_per_chunk = rpc;
     e.tests = S.d_tests.p;
@@ -1187,6 +1207,7 @@ fpgm_status dev_round_submit(fpgm_ctx *ctx, Slot &S, const DevLevel &L, const st
     a.zpool = S.d_zpool.p;
     a.card = ctx->d_card;
     a.colw = ctx->d_colw;
+    a.poff = ctx->d_poff;
     a.items = S.d_items.p;
     a.ws = ctx->d_ws.p;
     a.done = ctx->d_done.p;
@@ -1201,15 +1222,16 @@ fpgm_status dev_round_submit(fpgm_ctx *ctx, Slot &S, const DevLevel &L, const st
     a.rows_per_chunk = rpc;
     // every group the level's tables can reach is launched (a launch whose count reads
     // zero ends at once)
-    for (int gi = 0; gi < kRouteGroups; ++gi) {
-        if (!L.launch[gi]) continue;
+    for (int dg = 0; dg < kRouteGroups; ++dg) {
+        if (!L.launch[dg]) continue;
+        const int gi = dg == round_group(kRouteBits) ? kGroupBits : dg;  // the host's group
         HIP_TRY(hipEventRecord(S.ev_grp[gi], ctx->stream), "event");
-        a.item_begin = gi * cap;
+        a.item_begin = dg * cap;
         a.n_items = cap;
         a.queue = S.d_queue.p + gi * kXcds;
-        a.n_items_dev = group_count + gi;
+        a.n_items_dev = group_count + dg;
         const int sink = (gi == 1 && L.lane_w128) ? kSinkLaneW128 : kGroupSink[gi];
-        int64_t grid = std::min<int64_t>(cap, (int64_t)ctx->n_cu * ctx->sched.wpc[gi]);
+        int64_t grid = std::min<int64_t>(cap, (int64_t)ctx->n_cu * wpc_of(ctx, gi));
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
         HIP_TRY(launch_count_stat(a, sink, ctx->d_pcols != nullptr, (int)grid, ctx->stream), "launch count_stat");
         HIP_TRY(hipEventRecord(S.ev_gend[gi], ctx->stream), "event");
@@ -1588,14 +1610,16 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_ITEMS_PER_CU"))
         ctx->sched.items_per_cu = std::max<int64_t>(1, std::atoll(e));
     if (const char *e = std::getenv("FPGM_WPC")) {  // five comma-separated values, in sink-group order
-        int64_t v[kGroups];
-        if (std::sscanf(e, "%ld,%ld,%ld,%ld,%ld", &v[0], &v[1], &v[2], &v[3], &v[4]) == kGroups)
-            for (int g = 0; g < kGroups; ++g) ctx->sched.wpc[g] = std::min<int64_t>(64, std::max<int64_t>(1, v[g]));
+        int64_t v[kTuneGroups];
+        if (std::sscanf(e, "%ld,%ld,%ld,%ld,%ld", &v[0], &v[1], &v[2], &v[3], &v[4]) == kTuneGroups)
+            for (int g = 0; g < kTuneGroups; ++g) ctx->sched.wpc[g] = std::min<int64_t>(64, std::max<int64_t>(1, v[g]));
     }
     if (const char *e = std::getenv("FPGM_PACK")) ctx->pack = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP")) ctx->group = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_DEVROUNDS")) ctx->devrounds = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_BITS")) ctx->bits = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_BITS_CELLS")) ctx->bits_cells = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("FPGM_UPLOAD_PIPELINE")) ctx->upload_pipeline = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_LDS_OWN_CHUNKS")) ctx->sched.lds_own_chunks = std::atoi(e) != 0 ? 1 : 0;
@@ -1778,10 +1802,45 @@ static fpgm_status finish_ctx(fpgm_ctx *ctx, const uint8_t *const *cols = nullpt
             if (st != FPGM_OK) return st;
         }
     }
+    // bit planes with the packed copy (large instances): every variable of at most
+    // kBitsMaxCard states, nbits_of(card) planes each, each instance's planes kPlaneAlign-
+    // word aligned and clear past its rows.  Without the memory the tables count as they
+    // would without planes.
+    std::vector<int32_t> poff((size_t)n, -1);
+    int32_t n_planes = 0;
+    for (int v = 0; v < n; ++v)
+        if (ctx->card[v] <= kBitsMaxCard) {
+            poff[(size_t)v] = n_planes;
+            n_planes += nbits_of(ctx->card[v]);
+        }
+    std::vector<int64_t> pwords(ctx->inst.size(), 0), pbase(ctx->inst.size(), 0);
+    if (pack && ctx->bits && ctx->bits_cells > 0 && n_planes > 0) {
+        int64_t total = 0;
+        for (size_t i = 0; i < ctx->inst.size(); ++i) {
+            pwords[i] = ((ctx->inst[i].n_rows + 31) / 32 + kPlaneAlign - 1) / kPlaneAlign * kPlaneAlign;
+            pbase[i] = total;
+            total += pwords[i] * n_planes;
+        }
+        if (hipMalloc((void **)&ctx->d_planes, sizeof(uint32_t) * (size_t)total) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->d_planes = nullptr;
+        } else {
+            ctx->planes_bytes = (int64_t)sizeof(uint32_t) * total;
+            HIP_TRY(hipMalloc((void **)&ctx->d_poff, sizeof(int32_t) * (size_t)n), "hipMalloc plane offsets");
+            HIP_TRY(hipMemcpyAsync(ctx->d_poff, poff.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
+                                   ctx->stream),
+                    "upload plane offsets");
+            for (size_t i = 0; i < ctx->inst.size(); ++i)
+                HIP_TRY(launch_build_planes(ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows, n, ctx->d_card,
+                                            ctx->d_poff, ctx->d_planes + pbase[i], pwords[i], ctx->stream),
+                        "launch build planes");
+        }
+    }
     std::vector<DevInst> di(ctx->inst.size());
     for (size_t i = 0; i < di.size(); ++i)
         di[i] = DevInst{ctx->inst[i].cols, ctx->inst[i].stride, ctx->inst[i].n_rows,
-                        pcols[i] ? pcols[i] : ctx->inst[i].cols};
+                        pcols[i] ? pcols[i] : ctx->inst[i].cols, ctx->d_planes ? ctx->d_planes + pbase[i] : nullptr,
+                        pwords[i]};
     HIP_TRY(hipMalloc((void **)&ctx->d_inst, sizeof(DevInst) * di.size()), "hipMalloc instances");
     HIP_TRY(hipMemcpyAsync(ctx->d_inst, di.data(), sizeof(DevInst) * di.size(), hipMemcpyHostToDevice, ctx->stream),
             "upload instances");
@@ -1870,6 +1929,8 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     if (ctx->d_card) (void)hipFree(ctx->d_card);
     if (ctx->d_colw) (void)hipFree(ctx->d_colw);
     if (ctx->d_pcols) (void)hipFree(ctx->d_pcols);
+    if (ctx->d_planes) (void)hipFree(ctx->d_planes);
+    if (ctx->d_poff) (void)hipFree(ctx->d_poff);
     if (ctx->d_crit) (void)hipFree(ctx->d_crit);
     for (Slot &sl : ctx->slot) sl.release();
     ctx->d_ws.release();
@@ -2211,6 +2272,7 @@ bool dev_level_plan(const fpgm_ctx *ctx, int d, const std::vector<PcInst> &P,
         // outside the column layout
         L.launch[1] = cells_max > kLaneCells && rows_min < ctx->wide_max_rows;
         L.launch[2] = cells_max > kLaneWide128Cells || (cells_max > kLaneCells && rows_max >= ctx->wide_max_rows);
+        L.launch[3] = d == 1 && route_params(ctx).bits_cells > 0;
     }
     L.nbr_base.assign(P.size(), -1);
     for (size_t i = 0; i < P.size(); ++i) {
@@ -2224,7 +2286,7 @@ bool dev_level_plan(const fpgm_ctx *ctx, int d, const std::vector<PcInst> &P,
 void apply_tune(fpgm_ctx *ctx, const fpgm_level_tune &t)
 {
     if (t.items_per_cu > 0) ctx->sched.items_per_cu = t.items_per_cu;
-    for (int g = 0; g < kGroups; ++g)
+    for (int g = 0; g < kTuneGroups; ++g)
         if (t.wg_per_cu[g] > 0) ctx->sched.wpc[g] = t.wg_per_cu[g];
     if (t.max_slices > 0) ctx->max_slices = t.max_slices;
     if (t.round_cap > 0) ctx->sched.cap = t.round_cap;
@@ -2974,7 +3036,7 @@ fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm
         t.pipeline = ctx->sched.pipeline;
         t.concurrent = ctx->sched.concurrent;
         t.items_per_cu = lvl == 0 ? ctx->sched.gram_items_per_cu : ctx->sched.items_per_cu;
-        for (int g = 0; g < kGroups; ++g) t.wg_per_cu[g] = ctx->sched.wpc[g];
+        for (int g = 0; g < kTuneGroups; ++g) t.wg_per_cu[g] = ctx->sched.wpc[g];
         t.max_slices = ctx->max_slices;
         t.round_cap = ctx->sched.cap;
         return t;
@@ -3079,7 +3141,7 @@ fpgm_status fpgm_autotune(fpgm_ctx *ctx, double alpha, int32_t depth, const fpgm
                 cands.push_back(c);
             }
         if ((st = try_all()) != FPGM_OK) return st;
-        for (int g = 0; g < kGroups; ++g) {
+        for (int g = 0; g < kTuneGroups; ++g) {
             for (int64_t f : {-1, 1}) {
                 fpgm_level_tune c = cur;
                 c.wg_per_cu[g] = f < 0 ? std::max<int64_t>(1, cur.wg_per_cu[g] / 2) : std::min<int64_t>(16, cur.wg_per_cu[g] * 2);
@@ -3124,7 +3186,7 @@ fpgm_status fpgm_tune_set(fpgm_ctx *ctx, const fpgm_level_tune *levels, int32_t 
         if (r.level >= 64 || r.pipeline > 1 || r.concurrent > 1 || r.items_per_cu < 0 || r.items_per_cu > 4096 || r.max_slices < 0 ||
             r.round_cap < 0)
             return fail(ctx, FPGM_EINVAL, "bad tuning record");
-        for (int g = 0; g < kGroups; ++g)
+        for (int g = 0; g < kTuneGroups; ++g)
             if (r.wg_per_cu[g] < 0 || r.wg_per_cu[g] > 64) return fail(ctx, FPGM_EINVAL, "bad tuning record");
         if ((int)t.size() <= r.level) {
             fpgm_level_tune none{};
@@ -3355,6 +3417,7 @@ fpgm_status na_sequence(fpgm_ctx *ctx, uint8_t *base, int64_t nb, const int32_t 
     a.zpool = d_z;
     a.card = ctx->d_card;
     a.colw = ctx->d_colw;
+    a.poff = ctx->d_poff;
     a.items = d_items;
     a.ws = d_ws;
     a.done = d_done;
@@ -3374,7 +3437,7 @@ fpgm_status na_sequence(fpgm_ctx *ctx, uint8_t *base, int64_t nb, const int32_t 
         a.queue = d_queue + gi * kXcds;
         a.n_items_dev = nullptr;
         const int sink = (gi == 1 && lane_w128) ? kSinkLaneW128 : kGroupSink[gi];
-        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * ctx->sched.wpc[gi]);
+        int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wpc_of(ctx, gi));
         grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
         HIP_TRY(launch_count_stat(a, sink, ctx->d_pcols != nullptr, (int)grid, s), "launch count_stat");
     }
@@ -3451,7 +3514,7 @@ extern "C" fpgm_status fpgm_ci_batch_noalloc(fpgm_ctx *ctx, const fpgm_test *tes
         bool any_shared = false;
         for (int64_t k = i0; k < n; ++k) any_shared = any_shared || na_gidx[k] == 2;
         int64_t ws_used = 0, items = 0, nb = 0;
-        int64_t grp_len[kGroups] = {0, 0, 0, 0, 0};
+        int64_t grp_len[kGroups] = {0, 0, 0, 0, 0, 0};
         for (; i0 + nb < n; ++nb) {
             const int64_t k = i0 + nb;
             DevTest T = na_tests[k];

@@ -14,7 +14,9 @@
 namespace fpgm {
 
 constexpr int kMaxDevD = 16;      // largest conditioning set of a device-generated round
-constexpr int kRouteGroups = 3;   // sink groups of a device-generated round: per-lane, lane-wide, shared
+constexpr int kRouteGroups = 4;   // sink groups of a device-generated round: per-lane, lane-wide, shared, bit planes
+// a round's group index of a route_lds group (kRouteBits -> 3)
+__host__ __device__ constexpr int round_group(int g) { return g == kRouteBits ? 3 : g; }
 constexpr int kExpandPer = 8;     // consecutive tests per thread of the expansion
 constexpr int32_t kSegRun = 0x7fffffff;   // seg_out: the segment ran, no test independent
 constexpr int32_t kSegSkipped = -2;       // seg_out: the segment's edge was done before the round

@@ -88,11 +88,11 @@ __global__ __launch_bounds__(kRT) void k_expand(ExpandArgs e)
     const int64_t tbeg = ((int64_t)blockIdx.x * kRT + tid) * kExpandPer;
     int grp[kExpandPer];
     int32_t nch[kExpandPer];
-    uint32_t cnt[kRouteGroups] = {0u, 0u, 0u};
-    uint32_t ntest[kRouteGroups] = {0u, 0u, 0u};
-    unsigned long long rows_acc[kRouteGroups] = {0ull, 0ull, 0ull};
-    unsigned long long alg[kRouteGroups] = {0ull, 0ull, 0ull};
-    unsigned long long pbits[kRouteGroups] = {0ull, 0ull, 0ull};
+    uint32_t cnt[kRouteGroups] = {0u, 0u, 0u, 0u};
+    uint32_t ntest[kRouteGroups] = {0u, 0u, 0u, 0u};
+    unsigned long long rows_acc[kRouteGroups] = {0ull, 0ull, 0ull, 0ull};
+    unsigned long long alg[kRouteGroups] = {0ull, 0ull, 0ull, 0ull};
+    unsigned long long pbits[kRouteGroups] = {0ull, 0ull, 0ull, 0ull};
     uint32_t err = 0u;
     int32_t c[D > 0 ? D : 1];
     int64_t s = -1;
@@ -163,6 +163,7 @@ __global__ __launch_bounds__(kRT) void k_expand(ExpandArgs e)
                 g = 1;
             }
         }
+        if (g >= 0) g = round_group(g);
         if (g < 0 || !((e.launch_mask >> g) & 1u)) {  // the host's bounds for the level exclude both
             err |= g < 0 ? 1u : 16u;
             e.res[t] = skipped_res();

@@ -379,6 +379,42 @@ def test_grouped_level1_mfma_path(monkeypatch, case):
     assert out["1"]["tests_per_level"][1] > 0
 
 
+@pytest.mark.parametrize("n_rows", [4099, 70_001, 300_007])
+def test_bit_plane_tables(monkeypatch, n_rows):
+    # |Z| = 1 tables of at most 48 cells whose two smallest cardinalities are (2, 2), (2, 3),
+    # (2, 4) or (3, 3) count from one-hot bit planes when the columns are packed (planes are
+    # built with the packed copy): counts, batch results and a small stage equal the oracle's
+    # with the planes on and off.  300 007 rows in a small batch are cut into row chunks
+    # (chunks start at multiples of 4096 rows, the last word of the instance is partial);
+    # variable 0 is skewed toward one value
+    s = datagen.random_sample(91 + n_rows, 12, n_rows, 2, 2)
+    s.card[:] = [2, 2, 3, 3, 4, 4, 5, 6, 8, 12, 2, 3]
+    rng = np.random.default_rng(n_rows)
+    for v in range(12):
+        s.cols[v, :n_rows] = rng.integers(0, s.card[v], n_rows)
+    s.cols[0, :n_rows] = np.where(rng.random(n_rows) < 0.9, 1, s.cols[0, :n_rows])
+    s.cols[10, :n_rows] = np.where(rng.random(n_rows) < 0.5, s.cols[0, :n_rows], s.cols[10, :n_rows])
+    D = O.Data.of(s)
+    tests = [(0, 1, (2,)), (0, 2, (1,)), (2, 3, (0,)), (0, 4, (5,)), (4, 10, (2,)), (1, 2, (6,)), (0, 1, (8,)),
+             (2, 3, (4,)), (0, 6, (10,)), (1, 7, (11,)), (3, 10, (7,)), (0, 9, (1,)), (4, 5, (0,)), (0, 10, (1,))]
+    for (x, y, z) in tests:
+        assert np.prod([s.card[v] for v in (x, y) + z]) <= 48
+    arr = np.array([[x, y, 1, z[0]] for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
+    ostage = O.pc_skeleton(D, 0.05, 2, n_threads=8)
+    for on in ("1", "0"):
+        monkeypatch.setenv("FPGM_BITS", on)
+        with _ctx(s) as ctx:
+            for (x, y, z) in tests:
+                assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z)), (on, x, y, z)
+            for flags in (_lib.FPGM_F_EXACT, 0):
+                r = ctx.ci_batch(tests, alpha=0.05, flags=flags)
+                assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+            r = ctx.ci_batch(tests, alpha=0.05, flags=_lib.FPGM_F_EXACT)
+            np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+            _assert_same(ctx.pc_skeleton(0.05, 2), ostage)
+
+
 @pytest.mark.parametrize("case", ["alarm-5k", "dag-300v-20k", "dag-1000v-8k", "cards-6-12"])
 def test_device_rounds_same_results(monkeypatch, case):
     # FPGM_DEVROUNDS=1: the rounds of every level whose tables fit LDS are planned per edge
