@@ -157,15 +157,16 @@ struct RouteParams {
 constexpr int kRouteBits = 5;
 constexpr int kRoutePending = 6;
 
-// Cardinalities of a bit-plane table: sorted, the two smallest (P, Q) are one of the
-// counting kernel's instances -- (2, 2), (2, 3), (2, 4), (3, 3) -- and every variable has
-// planes.
+// Cardinalities of a bit-plane table of at most kLaneCells cells: sorted, the two smallest
+// (P, Q) are one of the counting kernel's instances -- (2, 2..5), (3, 3..4), (4, 4) -- and
+// every variable has planes.
 __host__ __device__ inline bool bits_shape(int64_t c0, int64_t c1, int64_t c2)
 {
     if (c0 > c1) { const int64_t t = c0; c0 = c1; c1 = t; }
     if (c1 > c2) { const int64_t t = c1; c1 = c2; c2 = t; }
     if (c0 > c1) { const int64_t t = c0; c0 = c1; c1 = t; }
-    return c2 <= kBitsMaxCard && ((c0 == 2 && c1 >= 2 && c1 <= 4) || (c0 == 3 && c1 == 3));
+    return c2 <= kBitsMaxCard && c0 * c1 * c2 <= kLaneCells &&
+           ((c0 == 2 && c1 >= 2 && c1 <= 5) || (c0 == 3 && c1 >= 3 && c1 <= 4) || (c0 == 4 && c1 == 4));
 }
 
 __host__ __device__ inline int route_lds(const RouteParams &p, int32_t &flags, int64_t cells, int64_t marg,

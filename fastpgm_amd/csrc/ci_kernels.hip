@@ -739,7 +739,7 @@ __device__ __forceinline__ void count_bits_pq(const uint32_t *A, const uint32_t 
         for (int i = 0; i < NBR; ++i) cb[i] = *reinterpret_cast<const uint4 *>(C + i * pw + w0);
 #pragma unroll
         for (int r = 0; r < (1 << NBR); ++r) {
-            if (r >= R) break;
+            if (r >= R) continue;  // uniform
             uint4 c = make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
             for (int i = 0; i < NBR; ++i) {
@@ -792,16 +792,22 @@ __device__ __forceinline__ void count_rows_bits(const LaunchArgs &a, const DevIn
         count_bits_pq<P, Q, NBR>(A, B, C, pw, c2, m0, m1, m2, w_b, w_e, nr, pl);                      \
         break;
     switch (uniform((c0 * 16 + c1) * 8 + nbits_of(c2))) {
+        // every (P, Q, planes of C) with P Q R <= kLaneCells and R >= Q
         FPGM_BITS_CASE(2, 2, 1)
         FPGM_BITS_CASE(2, 2, 2)
         FPGM_BITS_CASE(2, 2, 3)
         FPGM_BITS_CASE(2, 2, 4)
         FPGM_BITS_CASE(2, 3, 2)
         FPGM_BITS_CASE(2, 3, 3)
+        FPGM_BITS_CASE(2, 3, 4)
         FPGM_BITS_CASE(2, 4, 2)
         FPGM_BITS_CASE(2, 4, 3)
+        FPGM_BITS_CASE(2, 5, 3)
         FPGM_BITS_CASE(3, 3, 2)
         FPGM_BITS_CASE(3, 3, 3)
+        FPGM_BITS_CASE(3, 4, 2)
+        FPGM_BITS_CASE(3, 4, 3)
+        FPGM_BITS_CASE(4, 4, 2)
     default: break;  // routing (bits_shape) sends no other shape here
     }
 #undef FPGM_BITS_CASE

@@ -1619,7 +1619,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_DEVROUNDS")) ctx->devrounds = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_BITS")) ctx->bits = std::atoi(e) != 0 ? 1 : 0;
-    if (const char *e = std::getenv("FPGM_BITS_CELLS")) ctx->bits_cells = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("FPGM_BITS_CELLS")) ctx->bits_cells = std::min(kLaneCells, std::max(0, std::atoi(e)));
     if (const char *e = std::getenv("FPGM_UPLOAD_PIPELINE")) ctx->upload_pipeline = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_LDS_OWN_CHUNKS")) ctx->sched.lds_own_chunks = std::atoi(e) != 0 ? 1 : 0;

@@ -381,14 +381,15 @@ def test_grouped_level1_mfma_path(monkeypatch, case):
 
 @pytest.mark.parametrize("n_rows", [4099, 70_001, 300_007])
 def test_bit_plane_tables(monkeypatch, n_rows):
-    # |Z| = 1 tables of at most 48 cells whose two smallest cardinalities are (2, 2), (2, 3),
-    # (2, 4) or (3, 3) count from one-hot bit planes when the columns are packed (planes are
-    # built with the packed copy): counts, batch results and a small stage equal the oracle's
-    # with the planes on and off.  300 007 rows in a small batch are cut into row chunks
-    # (chunks start at multiples of 4096 rows, the last word of the instance is partial);
+    # |Z| = 1 tables of at most 64 cells (FPGM_BITS_CELLS=64) whose two smallest
+    # cardinalities are (2, 2..5), (3, 3..4) or (4, 4) count from bit planes when the columns
+    # are packed (planes are built with the packed copy): counts, batch results and a small
+    # stage equal the oracle's with the planes on and off.  Every instance of the kernel is
+    # reached (C with 1 to 4 planes); 300 007 rows in a small batch are cut into row chunks
+    # (chunks start at multiples of 4096 rows, the instance's last word is partial);
     # variable 0 is skewed toward one value
     s = datagen.random_sample(91 + n_rows, 12, n_rows, 2, 2)
-    s.card[:] = [2, 2, 3, 3, 4, 4, 5, 6, 8, 12, 2, 3]
+    s.card[:] = [2, 2, 3, 3, 4, 4, 5, 6, 8, 10, 2, 4]
     rng = np.random.default_rng(n_rows)
     for v in range(12):
         s.cols[v, :n_rows] = rng.integers(0, s.card[v], n_rows)
@@ -396,12 +397,14 @@ def test_bit_plane_tables(monkeypatch, n_rows):
     s.cols[10, :n_rows] = np.where(rng.random(n_rows) < 0.5, s.cols[0, :n_rows], s.cols[10, :n_rows])
     D = O.Data.of(s)
     tests = [(0, 1, (2,)), (0, 2, (1,)), (2, 3, (0,)), (0, 4, (5,)), (4, 10, (2,)), (1, 2, (6,)), (0, 1, (8,)),
-             (2, 3, (4,)), (0, 6, (10,)), (1, 7, (11,)), (3, 10, (7,)), (0, 9, (1,)), (4, 5, (0,)), (0, 10, (1,))]
+             (2, 3, (4,)), (0, 6, (10,)), (1, 7, (11,)), (3, 10, (7,)), (0, 9, (1,)), (4, 5, (0,)), (0, 10, (1,)),
+             (2, 4, (5,)), (2, 4, (6,)), (4, 5, (11,)), (0, 6, (7,)), (0, 2, (9,)), (0, 1, (9,)), (3, 5, (4,))]
     for (x, y, z) in tests:
-        assert np.prod([s.card[v] for v in (x, y) + z]) <= 48
+        assert np.prod([s.card[v] for v in (x, y) + z]) <= 64
     arr = np.array([[x, y, 1, z[0]] for (x, y, z) in tests], dtype=np.int32)
     o = O.ci_batch(D, arr, 1, 0.05, n_threads=8)
     ostage = O.pc_skeleton(D, 0.05, 2, n_threads=8)
+    monkeypatch.setenv("FPGM_BITS_CELLS", "64")
     for on in ("1", "0"):
         monkeypatch.setenv("FPGM_BITS", on)
         with _ctx(s) as ctx:
