@@ -692,19 +692,48 @@ __device__ __forceinline__ void onehot_words(const uint32_t *S, int64_t pw, int6
     }
 }
 
-// A |Z| = 1 table from bit planes: with the three variables sorted by cardinality, A (P
-// states) and B (Q states, both compile-time) and C (R states, NBR planes), cell (p, q, r)
-// counts the rows where A = p, B = q and C = r: per 32-row word one AND per (p, q), the
-// value masks of C, and one AND and one population count per cell -- no per-row LDS
-// operation.  Each thread takes kBitsWords consecutive words per step and adds each
-// cell's count over them to its lane-private 16-bit counter of the n[z][x][y] cell
-// p mA + q mB + r mC (the per-lane sink's [pair][cell][lane] layout: even wave low half,
-// odd wave high).  Rows at or past n_rows read as value 0 in every plane, so the words
-// there are masked.
+// The roles of a bit-plane test's variables, sorted by cardinality: A (P states), B (Q),
+// C (R), and each one's multiplier in the n[z][x][y] cell index (x: c(Y), y: 1, z:
+// c(X) c(Y)).
+struct BitsRoles {
+    int vA, vB, vC;
+    int P, Q, R;
+    int mA, mB, mC;
+};
+
+__device__ __forceinline__ BitsRoles bits_roles(const DevTest &T, const Shared &sh)
+{
+    BitsRoles o;
+    int v0 = T.x, c0 = T.cx, m0 = T.cy;
+    int v1 = T.y, c1 = T.cy, m1 = 1;
+    int v2 = sh.zcol[0], c2 = sh.zcard[0], m2 = T.cx * T.cy;
+    auto swap3 = [](int &va, int &ca, int &ma, int &vb, int &cb, int &mb) {
+        if (ca > cb) {
+            int t = va; va = vb; vb = t;
+            t = ca; ca = cb; cb = t;
+            t = ma; ma = mb; mb = t;
+        }
+    };
+    swap3(v0, c0, m0, v1, c1, m1);
+    swap3(v1, c1, m1, v2, c2, m2);
+    swap3(v0, c0, m0, v1, c1, m1);
+    o.vA = v0; o.vB = v1; o.vC = v2;
+    o.P = c0; o.Q = c1; o.R = c2;
+    o.mA = m0; o.mB = m1; o.mC = m2;
+    return o;
+}
+
+// A |Z| = 1 table from bit planes: cell (p, q, r) of the roles' table counts the rows
+// where A = p, B = q and C = r.  Per 32-row word: one AND per (p, q), the value masks of C,
+// and one AND and one population count per cell -- no per-row LDS operation.  Each thread
+// takes kBitsWords consecutive words per step and adds each cell's count over them to its
+// lane-private 16-bit counter of cell (r P + p) Q + q (the per-lane sink's [pair][cell][lane]
+// layout, even wave low half, odd wave high; the offsets are compile-time, the reduction
+// maps the cells to n[z][x][y]).  Rows at or past n_rows read as value 0 in every plane, so
+// the words there are masked.
 template <int P, int Q, int NBR>
 __device__ __forceinline__ void count_bits_pq(const uint32_t *A, const uint32_t *B, const uint32_t *C, int64_t pw,
-                                              int R, int mA, int mB, int mC, int64_t w_b, int64_t w_e,
-                                              int64_t n_rows, uint32_t *pl)
+                                              int R, int64_t w_b, int64_t w_e, int64_t n_rows, uint32_t *pl)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     uint32_t *base = pl + (wave >> 1) * (kLaneCells * 64) + lane;
@@ -750,13 +779,11 @@ __device__ __forceinline__ void count_bits_pq(const uint32_t *A, const uint32_t 
                 c.w &= one ? cb[i].w : ~cb[i].w;
             }
 #pragma unroll
-            for (int p = 0; p < P; ++p)
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    const uint4 &u = ab[p * Q + q];
-                    const uint32_t n = __popc(u.x & c.x) + __popc(u.y & c.y) + __popc(u.z & c.z) + __popc(u.w & c.w);
-                    atomicAdd(base + (p * mA + q * mB + r * mC) * 64, n << sh);
-                }
+            for (int pq = 0; pq < P * Q; ++pq) {
+                const uint4 &u = ab[pq];
+                const uint32_t n = __popc(u.x & c.x) + __popc(u.y & c.y) + __popc(u.z & c.z) + __popc(u.w & c.w);
+                atomicAdd(base + (r * P * Q + pq) * 64, n << sh);
+            }
         }
     }
 }
@@ -767,31 +794,17 @@ __device__ __forceinline__ void count_bits_pq(const uint32_t *A, const uint32_t 
 __device__ __forceinline__ void count_rows_bits(const LaunchArgs &a, const DevInst &I, const DevTest &T,
                                                 const Shared &sh, int64_t r_begin, int64_t r_end, uint32_t *pl)
 {
-    // the roles, sorted by cardinality: x (multiplier c(Y) in the cell index), y (1), z
-    // (c(X) c(Y))
-    int v0 = T.x, c0 = T.cx, m0 = T.cy;
-    int v1 = T.y, c1 = T.cy, m1 = 1;
-    int v2 = sh.zcol[0], c2 = sh.zcard[0], m2 = T.cx * T.cy;
-    auto swap3 = [](int &va, int &ca, int &ma, int &vb, int &cb, int &mb) {
-        if (ca > cb) {
-            int t = va; va = vb; vb = t;
-            t = ca; ca = cb; cb = t;
-            t = ma; ma = mb; mb = t;
-        }
-    };
-    swap3(v0, c0, m0, v1, c1, m1);
-    swap3(v1, c1, m1, v2, c2, m2);
-    swap3(v0, c0, m0, v1, c1, m1);
+    const BitsRoles ro = bits_roles(T, sh);
     const int64_t pw = I.pwords;
-    const uint32_t *A = I.planes + (int64_t)a.poff[v0] * pw;
-    const uint32_t *B = I.planes + (int64_t)a.poff[v1] * pw;
-    const uint32_t *C = I.planes + (int64_t)a.poff[v2] * pw;
+    const uint32_t *A = I.planes + (int64_t)a.poff[ro.vA] * pw;
+    const uint32_t *B = I.planes + (int64_t)a.poff[ro.vB] * pw;
+    const uint32_t *C = I.planes + (int64_t)a.poff[ro.vC] * pw;
     const int64_t w_b = r_begin >> 5, w_e = (r_end + 31) >> 5, nr = I.n_rows;
 #define FPGM_BITS_CASE(P, Q, NBR)                                                                     \
     case (P * 16 + Q) * 8 + NBR:                                                                      \
-        count_bits_pq<P, Q, NBR>(A, B, C, pw, c2, m0, m1, m2, w_b, w_e, nr, pl);                      \
+        count_bits_pq<P, Q, NBR>(A, B, C, pw, ro.R, w_b, w_e, nr, pl);                                \
         break;
-    switch (uniform((c0 * 16 + c1) * 8 + nbits_of(c2))) {
+    switch (uniform((ro.P * 16 + ro.Q) * 8 + nbits_of(ro.R))) {
         // every (P, Q, planes of C) with P Q R <= kLaneCells and R >= Q
         FPGM_BITS_CASE(2, 2, 1)
         FPGM_BITS_CASE(2, 2, 2)
@@ -1252,7 +1265,13 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
                         sum += (v & 0xffffu) + (v >> 16);
                     }
                 for (int o = 1; o < 8; o <<= 1) sum += __shfl_xor(sum, o, 64);
-                if (part == 0) small[k] = (int32_t)sum;
+                int dst = k;
+                if (kMode == kSinkBits) {  // counter (r P + p) Q + q -> cell p mA + q mB + r mC
+                    const BitsRoles ro = bits_roles(T, sh);
+                    const int pq = k % (ro.P * ro.Q), r = k / (ro.P * ro.Q);
+                    dst = (pq / ro.Q) * ro.mA + (pq % ro.Q) * ro.mB + r * ro.mC;
+                }
+                if (part == 0) small[dst] = (int32_t)sum;
             }
             __syncthreads();
             tab = small;
