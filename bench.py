@@ -511,12 +511,13 @@ def summarize_trace(trace_dir, out_path):
         "  python bench.py --summarize-trace gpurun_out/prof_kt",
         f"The run executes the CI stage {stages} times (timed step + read-back run); per-stage figures",
         "below are the trace totals divided by the stage count.",
-        "Sinks (every counting kernel reads the packed columns: 2, 4 or 8 bits per value by cardinality):",
+        "Sinks (the counting kernels read the packed columns, 2, 4 or 8 bits per value, or the bit planes):",
         "  lane       k_count_stat<2>  per-lane 16-bit LDS counters, tables <= 64 cells",
         "  lane-wide  k_count_stat<3>, <5>  the same with 48 / 64 KiB of counters, <= 96 / 128 cells",
         "  shared     k_count_stat<0>  shared LDS histogram, tables <= 8192 cells",
         "  slice      k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
         "  device     k_count_stat<1> + k_fin_wide_*  device-memory table",
+        "  bits       k_count_stat<6>  |Z| = 1 tables <= 48 cells from bit planes (AND + population count)",
         "  k_gram_l0  level 0 as the int8 one-hot Gram matrix (v_mfma_i32_32x32x32_i8)",
         "",
         "family        calls/stage   ms/stage   avg us/call   bench (HIP events): ms/stage  launches/stage",
@@ -571,10 +572,11 @@ def summarize_traffic(bench_json, out_path):
     for k, L in rl["sinks"].items():
         b = L["bound"]
         hb = L.get("hbm_bytes")
+        bound = (f"{b['achieved']:.3g} / {b['peak']:.3g} {b['unit']} ({b['frac']:.3f})" if b.get("peak")
+                 else f"{b['achieved']:.3g} {b['unit']} (VALU issue: see the SQ-counter profile)")
         lines.append(f"{k:10s} {L['ms']:9.2f} {L['launches']:8.1f}  "
                      f"{(f'{hb:10.4g}' if hb else '         -')}  {(L.get('hbm_gbs') or 0):9.0f}  "
-                     f"{(L.get('hbm_frac') or 0):11.3f}  {L.get('effective_gbs', 0):14.0f}   "
-                     f"{b['achieved']:.3g} / {b['peak']:.3g} {b['unit']} ({b['frac']:.3f})")
+                     f"{(L.get('hbm_frac') or 0):11.3f}  {L.get('effective_gbs', 0):14.0f}   {bound}")
     st = rl["stage"]
     lines += ["",
               f"Counting kernels per stage: {st['counting_ms']:.1f} ms elapsed (events), {st['launches']:.0f} launches,",
@@ -589,7 +591,7 @@ def summarize_traffic(bench_json, out_path):
     print("\n".join(lines))
 
 
-def summarize_sq(sq_dir, out_path):
+def summarize_sq(sq_dir, out_path, trace_dir=None):
     """SQ instruction-counter profile text from DIR/*counter_collection.csv.  No GPU use."""
     import collections
     import csv
@@ -609,7 +611,9 @@ def summarize_sq(sq_dir, out_path):
              "      SQ_INSTS_VMEM_RD SQ_LDS_BANK_CONFLICT SQ_INSTS_SALU -f csv -d gpurun_out/pmc_sq -o sq -- \\",
              "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --no-traffic --no-g2 \\",
              "      --tune-file profiles/tune_dag-1000v-1500e-1m.json",
-             "  then: python bench.py --summarize-sq gpurun_out/pmc_sq",
+             "  (its JSON line saved as gpurun_out/pmc_sq/run.json), then, with the kernel-trace run of the",
+             "  same command in gpurun_out/prof_kt:",
+             "  python bench.py --summarize-sq gpurun_out/pmc_sq --sq-times gpurun_out/prof_kt",
              "kernel                 disp " + " ".join(f"{n[3:]:>17s}" for n in names)]
     for k in sorted(agg, key=lambda k: -agg[k]["SQ_BUSY_CYCLES"]):
         d = agg[k]
@@ -620,6 +624,29 @@ def summarize_sq(sq_dir, out_path):
         if d["SQ_INSTS_LDS"] > 0:
             lines.append(f"{k:22s} {d['SQ_INSTS_VALU'] / d['SQ_INSTS_LDS']:18.2f}   "
                          f"{d['SQ_LDS_BANK_CONFLICT'] / d['SQ_INSTS_LDS']:10.3f}")
+    # issue rates of the counting kernels against the CU's peaks: one wave64 VALU instruction
+    # per clock (4 SIMD16s, 4 clocks each) and one wave64 LDS instruction per 4 clocks (16
+    # lanes per clock), 256 CUs at 2.4 GHz, over the kernels' durations in a kernel-trace
+    # run of the same command (counter collection itself slows the kernels)
+    run_path = os.path.join(sq_dir, "run.json")
+    stats = glob.glob(os.path.join(trace_dir or "", "**", "*kernel_stats.csv"), recursive=True) if trace_dir else []
+    if os.path.exists(run_path) and stats and os.path.exists(os.path.join(trace_dir, "run.json")):
+        sq_stages = json.load(open(run_path)).get("stages_in_run", 1)
+        kt_stages = json.load(open(os.path.join(trace_dir, "run.json"))).get("stages_in_run", 1)
+        dur = collections.defaultdict(float)
+        for r in csv.DictReader(open(stats[0])):
+            dur[_kernel_short(r["Name"])] += float(r["TotalDurationNs"]) * 1e-9 / kt_stages
+        cu_clk = 256 * 2.4e9
+        lines += ["", f"Issue rates per CU and clock (counters per stage over the kernel-trace durations per stage, "
+                      f"{trace_dir}):",
+                  "kernel                 s/stage    VALU instr/clk (peak 1)   LDS instr x 4 clk/clk (peak 1)"]
+        for k in sorted(agg, key=lambda k: -agg[k]["SQ_BUSY_CYCLES"]):
+            if not k.startswith("k_count_stat") or dur.get(k, 0.0) <= 0.0:
+                continue
+            d = agg[k]
+            sec = dur[k]
+            lines.append(f"{k:22s} {sec:8.4f}   {d['SQ_INSTS_VALU'] / sq_stages / (sec * cu_clk):22.3f}   "
+                         f"{4 * d['SQ_INSTS_LDS'] / sq_stages / (sec * cu_clk):28.3f}")
     with open(out_path, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -666,6 +693,9 @@ def main():
                     help="only write the HBM-traffic profile from a bench run's JSON line; no GPU use")
     ap.add_argument("--summarize-sq", default="", metavar="DIR",
                     help="only write the SQ counter profile from DIR (rocprofv3 --pmc SQ_* output); no GPU use")
+    ap.add_argument("--sq-times", default="", metavar="DIR",
+                    help="with --summarize-sq: kernel-trace run of the same command whose durations the issue "
+                         "rates divide by")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -683,7 +713,8 @@ def main():
                           os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_hbm_traffic.txt"))
         return
     if args.summarize_sq:
-        summarize_sq(args.summarize_sq, os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_sq_counters.txt"))
+        summarize_sq(args.summarize_sq, os.path.join(ROOT, "profiles", "k_count_stat_dag-1000v-1500e-1m_sq_counters.txt"),
+                     args.sq_times or None)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
