@@ -299,6 +299,19 @@ struct ColSet {
     }
 };
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_mad(uint32_t a, uint32_t b, uint32_t c)
+{
+    const u16x2 r = __builtin_bit_cast(u16x2, a) * __builtin_bit_cast(u16x2, b) + __builtin_bit_cast(u16x2, c);
+    return __builtin_bit_cast(uint32_t, r);
+}
+
+__device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
+{
+    return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+}
+
 // Rows [r_begin, r_end) of test T into the sink, |Z| = D known at compile time.
 // Loads for the next step are issued before the current step is counted.
 template <int D, int kSink, int NW>
@@ -317,6 +330,41 @@ __device__ __forceinline__ void count_rows_d(const DevInst &I, const DevTest &T,
     auto step = [&](const Cols<D, NW> &raw, int64_t r) {
         Cols<D> cur;
         cs.unpack(raw, cur);
+        if constexpr (kSink == kSinkShared) {
+            // every key is below kLdsCells, and so is its word offset key << ksh in the
+            // column layouts: its byte offset stays below 2^15.  The byte offsets of two
+            // rows are formed at once in the 16-bit halves of a word (v_pk_mad_u16): rows
+            // 4q and 4q + 2 from the even bytes of column word q, rows 4q + 1 and 4q + 3
+            // from the odd ones; partial keys only grow toward the final key, so no half
+            // carries into the other
+            const uint32_t m2 = (4u << ksh) * 0x10001u;
+            uint32_t zc2[D > 0 ? D : 1];
+#pragma unroll
+            for (int j = 0; j < D; ++j) zc2[j] = zc[j] * 0x10001u;
+            const uint32_t cx2 = cx * 0x10001u, cy2 = cy * 0x10001u;
+            const int64_t nv = r_end - r;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t k[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    auto col = [&](int j) { return (word_of(cur.v[j], q) >> (8 * e)) & 0x00ff00ffu; };
+                    uint32_t kk = (D > 0) ? col(0) : col(X);
+#pragma unroll
+                    for (int j = 1; j < D; ++j) kk = pk_mad(kk, zc2[j], col(j));
+                    if (D > 0) kk = pk_mad(kk, cx2, col(X));
+                    kk = pk_mad(kk, cy2, col(X + 1));
+                    k[e] = pk_mad(kk, m2, 0u);
+                }
+                // rows 4q + 0, 4q + 1, 4q + 2, 4q + 3
+                const uint32_t off[4] = {k[0] & 0xffffu, k[1] & 0xffffu, k[0] >> 16, k[1] >> 16};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (nv >= kRowsPerThread || 4 * q + i < nv)
+                        atomicAdd(reinterpret_cast<int32_t *>(reinterpret_cast<char *>(h) + off[i]), 1);
+            }
+            return;
+        }
         uint32_t key[kRowsPerThread];
 #pragma unroll
         for (int b = 0; b < kRowsPerThread; ++b) {
@@ -409,19 +457,6 @@ __device__ __forceinline__ void count_rows_any(const DevInst &I, const DevTest &
 // offset comes from that word with one byte permute (lane_put4).  Rows past the end
 // of the data only ever carry into later rows of the same word, which are past the
 // end as well and are not counted.
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ uint32_t pk_mad(uint32_t a, uint32_t b, uint32_t c)
-{
-    const u16x2 r = __builtin_bit_cast(u16x2, a) * __builtin_bit_cast(u16x2, b) + __builtin_bit_cast(u16x2, c);
-    return __builtin_bit_cast(uint32_t, r);
-}
-
-__device__ __forceinline__ uint32_t word_of(const uint4 &v, int q)
-{
-    return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
-}
-
 // Counts the (up to) 4 rows whose keys are the bytes of K; rows >= nvalid are skipped.
 // A row's byte offset in the counters is pair * kCells * 256 + key * 256 + lane * 4: every
 // key byte already carries the wave pair's bias (0, or kCells added to the key; see
