@@ -353,7 +353,7 @@ def test_skeleton_concurrent_sink_launches_same_results(monkeypatch, key, alpha,
 @pytest.mark.parametrize("case", ["dag-1000v-30k", "cards-6-12", "cards-2-8-small"])
 def test_grouped_level1_mfma_path(monkeypatch, case):
     # level 1's tests that share the walking endpoint u and the conditioning node z are
-    # counted as one-hot int8 products on the MFMA units (FPGM_GROUP, on by default); the
+    # counted as one-hot int8 products on the MFMA units (FPGM_GROUP=1; off by default); the
     # stage equals the oracle's with the path on and off.  The cases: the 1000-variable
     # network (chunks of 128 columns cut from large groups, row splits), cardinalities 6-12
     # (byte-comparison one-hot form, 36..64 A rows: two fragments), and a small sample whose
