@@ -1382,6 +1382,15 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
             __syncthreads();
             count_rows<kSinkSlice, kPacked>(I, T, sh, r_begin, r_end, lds);
             __syncthreads();
+            if (T.n_slices == 1 && T.n_chunks == 1 && !(T.flags & kTFKeep)) {
+                // the whole table counted in LDS by this item: the statistic straight from
+                // it, marginals in the workspace, which then reads zero again
+                if (tid == kQ) pk_next = resolve();
+                const int64_t marg = (int64_t)T.dimz * (T.cx + T.cy + 1);
+                finalize(a, T, t, lds, g + cells, sh, s_crit, CritLds<kMode>::value);
+                for (int64_t i = tid; i < marg; i += kWG) g[cells + i] = 0;
+                continue;
+            }
             for (int i = tid; i < len; i += kWG) {
                 const int32_t v = lds[i];
                 if (v) atomicAdd(&g[lo + i], v);

@@ -308,6 +308,7 @@ struct fpgm_ctx {
         int concurrent = 1;                  // slice and device-memory sinks on streams of their own
         int lds_own_chunks = 0;              // FPGM_LDS_OWN_CHUNKS: the shared sink's group chunked by its own test count
         int64_t side_cu_mult = 1;            // FPGM_SIDE_CU_MULT: LDS groups below this many items per CU on a side stream
+        int64_t slice_min_rows = 0;          // FPGM_SLICE_MIN_ROWS: rows per z-slice work item at least this many
     } sched;
     // per-level scheduling and launch shapes chosen by fpgm_autotune (index = level); a
     // candidate being timed by the tuner overrides them
@@ -607,6 +608,8 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
                                   : m;
             const int64_t cpt = std::min(steps, std::max<int64_t>(1, (target_items + u - 1) / u));
             r = (steps + cpt - 1) / cpt * kRowsPerStep;
+            if (kGroupSink[g] == kSinkSlice && ctx->sched.slice_min_rows > 0)
+                r = std::max(r, (ctx->sched.slice_min_rows + kRowsPerStep - 1) / kRowsPerStep * kRowsPerStep);
         }
         rpc[g] = std::min<int64_t>(r, kLaneMaxRowsPerChunk);
     }
@@ -1624,6 +1627,7 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_GROUP_MIN_ROWS")) ctx->group_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_LDS_OWN_CHUNKS")) ctx->sched.lds_own_chunks = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_SIDE_CU_MULT")) ctx->sched.side_cu_mult = std::max<int64_t>(1, std::atoll(e));
+    if (const char *e = std::getenv("FPGM_SLICE_MIN_ROWS")) ctx->sched.slice_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_WIDE_MAX_ROWS")) ctx->wide_max_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_SINKS")) {
         if (std::strcmp(e, "shared") == 0) ctx->sink_mask |= 3;

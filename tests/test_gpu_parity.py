@@ -751,6 +751,41 @@ def test_sliced_and_device_memory_sinks():
     np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
 
 
+@pytest.mark.parametrize("min_rows", ["0", "1048576"], ids=["chunked", "whole-rows"])
+def test_slice_sink_whole_row_items(monkeypatch, min_rows):
+    # z-slice tables (8193 .. 131072 cells) with the rows cut into chunks, whose slices are
+    # merged in the workspace, and with every work item over all rows: a one-slice table
+    # then gets its statistic straight from LDS, a several-slice one merges only slices
+    monkeypatch.setenv("FPGM_SLICE_MIN_ROWS", min_rows)
+    s = datagen.random_sample(31, 10, 300_001, 5, 8)
+    D = O.Data.of(s)
+    tests = []
+    for x in range(10):
+        for y in range(x + 1, 10):
+            rest = [v for v in range(10) if v not in (x, y)]
+            for d in (2, 3, 4):
+                z = tuple(rest[(x + y + k) % len(rest)] for k in range(d))
+                if len(set(z)) < d:
+                    continue
+                z = tuple(sorted(z))
+                cells = int(s.card[x]) * int(s.card[y]) * int(np.prod([s.card[v] for v in z]))
+                if 8192 < cells <= 131072:
+                    tests.append((x, y, z))
+    tests = tests[::3]
+    n_cells = [int(s.card[x]) * int(s.card[y]) * int(np.prod([s.card[v] for v in z])) for (x, y, z) in tests]
+    assert any(c <= 16384 for c in n_cells) and any(c > 16384 for c in n_cells)
+    arr = np.array([[x, y, len(z)] + list(z) + [0] * (4 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 4, 0.05, n_threads=8)
+    with _ctx(s) as ctx:
+        for flags in (0, _lib.FPGM_F_EXACT):
+            r = ctx.ci_batch(tests, alpha=0.05, flags=flags)
+            assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+            if flags:
+                np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+        for (x, y, z) in tests[:2]:
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
+
+
 @pytest.mark.parametrize("key,alpha,depth,rows", [(2, 0.01, 5, 20_000), (3, 0.05, 4, 8_000)])
 def test_autotune_changes_time_only(key, alpha, depth, rows):
     # the tuner replays every level under candidate launch shapes and round schedules;
