@@ -60,6 +60,7 @@ constexpr int32_t kTFWide = 32;      // statistic by the grid-wide finalize kern
 constexpr int32_t kTFNarrow = 64;    // every column of the test packed at 4 or 2 bits per value
 constexpr int32_t kTFCol = 128;      // per-lane sink launch, table in a column layout (<= 2 kColCells cells, |Z| = 1)
 constexpr int32_t kTFBits = 256;     // per-lane sink launch, counted from bit planes (|Z| = 1, small tables)
+constexpr int32_t kTFPart = 512;     // z-slice launch over rows partitioned by slice first (k_part_keys)
 constexpr int kBitsMaxCard = 16;     // variables with planes: cardinality at most this
 // planes of a variable of c states: bits of its largest value
 __host__ __device__ constexpr int nbits_of(int c) { return c <= 1 ? 0 : c <= 2 ? 1 : c <= 4 ? 2 : c <= 8 ? 3 : 4; }
@@ -84,6 +85,35 @@ struct WideTest {
     int32_t part0, nparts;  // its partials
     int32_t pad;
 };
+
+// Partitioned large tables (kTFPart): a first launch (k_part_keys, one work item per
+// kPartRows rows of a test) computes every row's key, counts its rows per z slice in LDS
+// and stores each row's slice-relative key (uint16) grouped by slice within the item's
+// rows; the z-slice launch then counts slice s of the test from those keys alone, every
+// row once, and reduces the slice's strata to a partial statistic (its slice holds them
+// whole), which the last slice combines.  Its partition buffer (int32 words from the
+// test's offset): marginal scratch [dimz (cx + cy + 1)], the slices' partials, the
+// (offset, count) of every (row chunk, slice) segment, then the keys, row chunk c at key
+// c * kPartRows.
+constexpr int kPartRows = 8192;       // rows per partitioning work item (2 workgroup steps)
+constexpr int kPartStage = 128;       // row chunks whose segments a slice item stages at a time
+constexpr int kPartItemChunks = 8;    // row chunks per partitioning work item
+constexpr int kPartMaxSlices = 4096;  // z slices of a partitioned table at most
+struct PartLayout {
+    int64_t parts, seg, keys, words;
+};
+__host__ __device__ inline int64_t part_round64(int64_t w) { return (w + 63) / 64 * 64; }
+__host__ __device__ inline PartLayout part_layout(int64_t dimz, int32_t cx, int32_t cy, int64_t n_slices,
+                                                  int64_t n_rows)
+{
+    const int64_t chunks = (n_rows + kPartRows - 1) / kPartRows;
+    PartLayout L;
+    L.parts = part_round64(dimz * (cx + cy + 1));
+    L.seg = L.parts + part_round64(n_slices * (int64_t)(sizeof(WidePart) / sizeof(int32_t)));
+    L.keys = L.seg + part_round64(2 * chunks * n_slices);
+    L.words = L.keys + part_round64((chunks * kPartRows + 1) / 2);
+    return L;
+}
 
 // Result of one test on the device (32 bytes).
 struct DevResult {
@@ -123,6 +153,15 @@ struct LaunchArgs {
     // work queue of this launch: kXcds counters, zero at launch; workgroups of XCD x take
     // items of slab x in turn, then of the other slabs
     uint32_t *queue;
+    int32_t *part;           // partition buffer of kTFPart tests (their ws_off indexes it)
+};
+
+// The partitioning launch: work items test << 32 | first row chunk (kPartItemChunks chunks
+// of kPartRows rows each).
+struct PartArgs {
+    LaunchArgs a;
+    const uint64_t *items;
+    int64_t n_items;
 };
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
@@ -267,6 +306,7 @@ struct GroupArgs {
 // packed: some column of the context is packed (the kernels' packed-column form)
 hipError_t launch_count_stat(const LaunchArgs &a, int mode, bool packed, int grid, hipStream_t s);
 hipError_t launch_compact(const CompactArgs &c, int64_t max_len, hipStream_t s);
+hipError_t launch_part_keys(const PartArgs &p, bool packed, int grid, hipStream_t s);
 hipError_t launch_finalize_wide(const LaunchArgs &a, const WideItem *items, int64_t n_items, WidePart *parts,
                                 const WideTest *wt, int64_t n_wide, hipStream_t s);
 // p-values of the rechecked tests; with edge_done, an independent one marks its edge (round).

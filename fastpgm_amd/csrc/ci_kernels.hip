@@ -701,6 +701,144 @@ __device__ __forceinline__ void count_rows(const DevInst &I, const DevTest &T, c
     else count_rows_any<kSink>(I, T, sh, r_begin, r_end, h, k);
 }
 
+// ---------------------------------------------------------------- partitioned large tables
+
+constexpr int kPartSteps = kPartRows / kRowsPerStep;  // workgroup steps of a partitioning item
+static_assert(kPartRows % kRowsPerStep == 0, "whole steps per partitioning item");
+static_assert(3 * kPartStage + 1 <= kSmallCells && kPartStage <= 128, "a slice item's staged segments");
+
+// Slice of a key, key / sc, and the key within it: an FP32 reciprocal corrected by the
+// remainder.  Keys stay below 2^26 (at most kPartMaxSlices slices of at most kSliceCells
+// cells) and slices hold more than kSliceCells / 2 cells, so the FP32 quotient is within
+// one of the true one.
+__device__ __forceinline__ uint32_t part_slice_of(uint32_t key, uint32_t sc, float inv, uint32_t &rel)
+{
+    uint32_t q = (uint32_t)((float)key * inv);
+    int32_t r = (int32_t)(key - q * sc);
+    if (r < 0) {
+        --q;
+        r += (int32_t)sc;
+    } else if (r >= (int32_t)sc) {
+        ++q;
+        r -= (int32_t)sc;
+    }
+    rel = (uint32_t)r;
+    return q;
+}
+
+// Keys of the thread's rows of one partitioning item (rows [r0, r1), row r0 + s * 4096 +
+// 16 * thread + b in key[16 s + b]; ~0u for rows at or past r1), |Z| = D at compile time;
+// the next step's columns load while the current one is keyed.
+template <int D, int NW>
+__device__ __forceinline__ void part_keys_d(const DevInst &I, const DevTest &T, const Shared &sh, int64_t r0,
+                                            int64_t r1, uint32_t (&key)[kPartSteps * kRowsPerThread])
+{
+    const ColSet<D, NW> cs(I, T, sh);
+    uint32_t zc[D > 0 ? D : 1];
+#pragma unroll
+    for (int j = 0; j < D; ++j) zc[j] = (uint32_t)sh.zcard[j];
+    const uint32_t cx = (uint32_t)T.cx, cy = (uint32_t)T.cy;
+    constexpr int X = (D > 0 ? D : 1);
+    const int64_t r = r0 + (int64_t)threadIdx.x * kRowsPerThread;
+    Cols<D, NW> raw[2];
+    cs.load((r < r1) ? r : r0, raw[0]);
+#pragma unroll
+    for (int st = 0; st < kPartSteps; ++st) {
+        const int64_t rs = r + (int64_t)st * kRowsPerStep;
+        if (st + 1 < kPartSteps) {
+            const int64_t rn = rs + kRowsPerStep;
+            cs.load((rn < r1) ? rn : r0, raw[(st + 1) & 1]);
+        }
+        Cols<D> cur;
+        cs.unpack(raw[st & 1], cur);
+#pragma unroll
+        for (int b = 0; b < kRowsPerThread; ++b) {
+            uint32_t k = 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) k = k * zc[j] + byte_of(cur.v[j], b);
+            k = (k * cx + byte_of(cur.v[X], b)) * cy + byte_of(cur.v[X + 1], b);
+            key[st * kRowsPerThread + b] = (rs + b < r1) ? k : ~0u;
+        }
+    }
+}
+
+// Exclusive scan of the item's per-slice row counts cnt[0, S) (LDS): each slice's
+// (offset, count) segment goes to seg, and cnt[s] becomes the slice's fill position.
+__device__ void part_scan(uint32_t *cnt, int S, uint32_t *seg, Shared &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (S + kWG - 1) / kWG;
+    const int i0 = tid * per;
+    uint32_t loc = 0;
+    for (int j = 0; j < per; ++j)
+        if (i0 + j < S) loc += cnt[i0 + j];
+    uint32_t inc = loc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) sh.red_i[wave] = (int32_t)inc;
+    __syncthreads();
+    uint32_t off = inc - loc;
+    for (int w = 0; w < wave; ++w) off += (uint32_t)sh.red_i[w];
+    for (int j = 0; j < per; ++j) {
+        const int i = i0 + j;
+        if (i < S) {
+            const uint32_t c = cnt[i];
+            seg[2 * i] = off;
+            seg[2 * i + 1] = c;
+            cnt[i] = off;
+            off += c;
+        }
+    }
+}
+
+// One partitioning item: row chunk `chunk` of test T.  Every row's key once; its slice's
+// row count (LDS cnt, zero on entry); the scan; then each row's slice-relative key stored
+// at its slice's next position within the chunk's kPartRows keys.
+template <bool kPacked>
+__device__ void part_item(const LaunchArgs &a, const DevInst &I, const DevTest &T, Shared &sh, int64_t chunk,
+                          uint32_t *cnt)
+{
+    const int64_t r0 = chunk * kPartRows;
+    const int64_t r1 = (r0 + kPartRows < I.n_rows) ? r0 + kPartRows : I.n_rows;
+    constexpr int NW = kPacked ? 2 : 8;
+    constexpr int K = kPartSteps * kRowsPerThread;
+    uint32_t key[K];
+    switch (T.d) {
+    case 0: part_keys_d<0, NW>(I, T, sh, r0, r1, key); break;
+    case 1: part_keys_d<1, NW>(I, T, sh, r0, r1, key); break;
+    case 2: part_keys_d<2, NW>(I, T, sh, r0, r1, key); break;
+    case 3: part_keys_d<3, NW>(I, T, sh, r0, r1, key); break;
+    case 4: part_keys_d<4, NW>(I, T, sh, r0, r1, key); break;
+    case 5: part_keys_d<5, NW>(I, T, sh, r0, r1, key); break;
+    case 6: part_keys_d<6, NW>(I, T, sh, r0, r1, key); break;
+    case 7: part_keys_d<7, NW>(I, T, sh, r0, r1, key); break;
+    default: part_keys_d<8, NW>(I, T, sh, r0, r1, key); break;  // the host routes |Z| <= 8 only
+    }
+    const uint32_t sc = (uint32_t)T.slice_cells;
+    const float inv = 1.0f / (float)sc;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (key[i] != ~0u) {
+            uint32_t rel;
+            atomicAdd(&cnt[part_slice_of(key[i], sc, inv, rel)], 1u);
+        }
+    __syncthreads();
+    int32_t *base = a.part + T.ws_off;
+    const PartLayout L = part_layout(T.dimz, T.cx, T.cy, T.n_slices, I.n_rows);
+    part_scan(cnt, T.n_slices, reinterpret_cast<uint32_t *>(base + L.seg) + chunk * T.n_slices * 2, sh);
+    __syncthreads();
+    uint16_t *out = reinterpret_cast<uint16_t *>(base + L.keys) + chunk * kPartRows;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (key[i] != ~0u) {
+            uint32_t rel;
+            const uint32_t s = part_slice_of(key[i], sc, inv, rel);
+            out[atomicAdd(&cnt[s], 1u)] = (uint16_t)rel;
+        }
+}
+
 // ---------------------------------------------------------------- bit-plane counting
 
 // One-hot masks of the P values of a variable over words [w0, w0 + kBitsWords) from its
@@ -1134,6 +1272,177 @@ __device__ void finalize(const LaunchArgs &a, const DevTest &T, int t, const int
     __syncthreads();
 }
 
+// Partial statistic of the z slices [z0, z1) of a table (slice z at tab + z c(X) c(Y), one
+// per thread): marginals into ni + z c(X) and nj + z c(Y), df contribution, compensated
+// FP32 terms and FP64 terms (the reference's order within a slice, IndependenceTest.cpp:
+// 112-137), reduced over the workgroup; the result is thread 0's.  clear: the slices'
+// cells and marginals are zeroed once read.  Every thread of the workgroup calls it.
+__device__ WidePart range_partial(int32_t *tab, int32_t z0, int32_t z1, int cx, int cy, int32_t *ni, int32_t *nj,
+                                  bool clear, Shared &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cxy = cx * cy;
+    int df = 0;
+    float s = 0.f, c = 0.f, as = 0.f;
+    double g = 0.0;
+    for (int32_t z = z0 + tid; z < z1; z += kWG) {
+        int32_t *tz = tab + (int64_t)z * cxy;
+        int32_t *zi = ni + (int64_t)z * cx, *zj = nj + (int64_t)z * cy;
+        zslice32(tz, cx, cy, zi, zj, df, s, c, as);
+        int32_t nkz = 0;
+        for (int i = 0; i < cx; ++i) nkz += zi[i];
+        if (nkz != 0) {
+            for (int i = 0; i < cx; ++i) {
+                if (zi[i] == 0) continue;
+                for (int j = 0; j < cy; ++j) {
+                    const int32_t n = tz[i * cy + j];
+                    if (n) g += term64(n, zi[i], zj[j], nkz);
+                }
+            }
+        }
+        if (clear) {
+            for (int k = 0; k < cxy; ++k) tz[k] = 0;
+            for (int i = 0; i < cx; ++i) zi[i] = 0;
+            for (int j = 0; j < cy; ++j) zj[j] = 0;
+        }
+    }
+    df = wave_sum_i(df);
+    wave_nsum(s, c);
+    as = wave_sum_f(as);
+    g = wave_sum_d(g);
+    if (lane == 0) {
+        sh.red_i[wave] = df;
+        sh.red_f[3 * wave] = s;
+        sh.red_f[3 * wave + 1] = c;
+        sh.red_f[3 * wave + 2] = as;
+        sh.red_d[wave] = g;
+    }
+    __syncthreads();
+    WidePart P{0.0, 0.f, 0.f, 0.f, 0};
+    if (tid == 0)
+        for (int w = 0; w < kWG / 64; ++w) {
+            P.df += sh.red_i[w];
+            nadd(P.s, P.c, sh.red_f[3 * w]);
+            P.c += sh.red_f[3 * w + 1];
+            P.as += sh.red_f[3 * w + 2];
+            P.g64 += sh.red_d[w];
+        }
+    __syncthreads();  // the reduction scratch is free again
+    return P;
+}
+
+// Slice `slice` of a partitioned test (keys [lo, lo + len)) from its rows' stored keys.
+// Its (offset, count) segment in each row chunk is staged, kPartStage chunks at a time, in
+// seg_lds (3 kPartStage + 1 words: offset, count, and the prefix of the segments' 8-key
+// groups); the workgroup's threads take the groups of all staged segments in turn, each
+// group one 16-byte load of 8 keys (aligned: a segment's groups start at its offset
+// rounded down to 8 keys; keys outside the segment are masked), four loads in flight per
+// thread, and count them into tab (LDS).  Then the slice's strata give their partial
+// statistic, and the last slice of the test combines the partials in slice order and
+// decides as finalize does (FP64 G^2 at hand).
+__device__ void part_slice_stat(const LaunchArgs &a, const DevInst &I, const DevTest &T, int t, int slice, int32_t lo,
+                                int32_t len, int32_t *tab, int32_t *seg_lds, Shared &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < len; i += kWG) tab[i] = 0;
+    int32_t *base = a.part + T.ws_off;
+    const PartLayout L = part_layout(T.dimz, T.cx, T.cy, T.n_slices, I.n_rows);
+    const int64_t chunks = (I.n_rows + kPartRows - 1) / kPartRows;
+    const uint32_t *seg = reinterpret_cast<const uint32_t *>(base + L.seg);
+    const uint16_t *keys = reinterpret_cast<const uint16_t *>(base + L.keys);
+    int32_t *s_off = seg_lds, *s_cnt = seg_lds + kPartStage, *s_pre = seg_lds + 2 * kPartStage;
+    for (int64_t cb = 0; cb < chunks; cb += kPartStage) {
+        const int nc = (int)((chunks - cb < kPartStage) ? chunks - cb : kPartStage);
+        uint32_t ng = 0;
+        if (tid < nc) {
+            const int64_t e = ((cb + tid) * T.n_slices + slice) * 2;
+            const int32_t off = (int32_t)seg[e], cnt = (int32_t)seg[e + 1];
+            s_off[tid] = off;
+            s_cnt[tid] = cnt;
+            ng = cnt ? (uint32_t)((off + cnt - (off & ~7) + 7) >> 3) : 0u;
+        }
+        // exclusive prefix of the groups over the staged chunks (kPartStage <= 2 waves)
+        uint32_t inc = ng;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += v;
+        }
+        if (lane == 63) sh.red_i[wave] = (int32_t)inc;
+        __syncthreads();  // (the first pass: tab cleared as well)
+        if (tid < nc) s_pre[tid] = (int32_t)(inc - ng) + (wave ? sh.red_i[0] : 0);
+        if (tid == 0) s_pre[nc] = sh.red_i[0] + (nc > 64 ? sh.red_i[1] : 0);
+        __syncthreads();
+        const int32_t total = s_pre[nc];
+        int j = 0;  // the staged chunk of this thread's current group (groups ascend)
+        for (int32_t f0 = tid; f0 < total; f0 += 4 * kWG) {
+            uint4 q[4];
+            int32_t k0[4], o[4], e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int32_t f = f0 + u * kWG;
+                k0[u] = -1;
+                if (f < total) {
+                    while (s_pre[j + 1] <= f) ++j;
+                    o[u] = s_off[j];
+                    e[u] = o[u] + s_cnt[j];
+                    k0[u] = (o[u] & ~7) + 8 * (f - s_pre[j]);
+                    q[u] = *reinterpret_cast<const uint4 *>(keys + (cb + j) * kPartRows + k0[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k0[u] < 0) continue;
+                const uint32_t w[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    const int32_t idx = k0[u] + h;
+                    if (idx >= o[u] && idx < e[u]) atomicAdd(&tab[(w[h >> 1] >> (16 * (h & 1))) & 0xffffu], 1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int cxy = T.cx * T.cy;
+    const int32_t zbase = lo / cxy;
+    int32_t *ni = base + (int64_t)zbase * T.cx;
+    int32_t *nj = base + (int64_t)T.dimz * T.cx + (int64_t)zbase * T.cy;
+    const WidePart P = range_partial(tab, 0, len / cxy, T.cx, T.cy, ni, nj, false, sh);
+    WidePart *parts = reinterpret_cast<WidePart *>(base + L.parts);
+    if (tid == 0) parts[slice] = P;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t prev = atomicAdd(&a.done[t], 1u);
+        sh.flag = (prev + 1 == (uint32_t)T.n_slices) ? 1 : 0;
+    }
+    __syncthreads();
+    if (sh.flag && tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        int df = 0;
+        float S = 0.f, C = 0.f, AS = 0.f;
+        double G = 0.0;
+        for (int i = 0; i < T.n_slices; ++i) {
+            const WidePart Q = parts[i];
+            df += Q.df;
+            nadd(S, C, Q.s);
+            C += Q.c;
+            AS += Q.as;
+            G += Q.g64;
+        }
+        DevResult R;
+        const int need64 = decide32(a, T.flags, df, S + C, AS, R);
+        if (need64) {
+            R.g2_64 = G;
+            R.exact = 1;
+            R.need_p = (df != 0) ? 1 : 0;  // k_pvalue decides from the FP64 p-value
+        }
+        a.res[t] = R;
+        mark_edge(a, T, R.indep, R.need_p);
+        a.done[t] = 0;
+    }
+    __syncthreads();
+}
+
 template <int kMode>
 struct CritLds {
     static constexpr int value = (kMode == kSinkShared) ? kCritLdsShared : kCritLds;
@@ -1374,6 +1683,11 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(
         } else if (kMode == kSinkSlice) {
             const int32_t lo = (int32_t)slice * T.slice_cells;
             const int32_t len = (int32_t)((cells - lo < T.slice_cells) ? cells - lo : T.slice_cells);
+            if (T.flags & kTFPart) {  // the slice's rows, partitioned beforehand
+                if (tid == kQ) pk_next = resolve();
+                part_slice_stat(a, I, T, t, (int)slice, lo, len, lds, small, sh);
+                continue;
+            }
             if (tid == 0) {
                 sh.slice_lo = lo;
                 sh.slice_len = len;
@@ -1478,59 +1792,15 @@ __global__ __launch_bounds__(kWG) void k_compact(CompactArgs c)
 __global__ __launch_bounds__(kWG) void k_fin_wide_part(LaunchArgs a, const WideItem *items, WidePart *parts)
 {
     __shared__ Shared sh;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const WideItem it = items[blockIdx.x];
     const DevTest T = a.tests[it.t];
     if (skipped(a, T)) return;  // the table was never counted: it is still zero
-    const int cx = T.cx, cy = T.cy, cxy = cx * cy;
+    const int cx = T.cx, cy = T.cy;
     int32_t *tab = a.ws + T.ws_off;
-    int32_t *ni = tab + (int64_t)T.dimz * cxy;
+    int32_t *ni = tab + (int64_t)T.dimz * cx * cy;
     int32_t *nj = ni + (int64_t)T.dimz * cx;
-    int df = 0;
-    float s = 0.f, c = 0.f, as = 0.f;
-    double g = 0.0;
-    for (int32_t z = it.z0 + tid; z < it.z1; z += kWG) {
-        int32_t *tz = tab + (int64_t)z * cxy;
-        int32_t *zi = ni + (int64_t)z * cx, *zj = nj + (int64_t)z * cy;
-        zslice32(tz, cx, cy, zi, zj, df, s, c, as);
-        int32_t nkz = 0;
-        for (int i = 0; i < cx; ++i) nkz += zi[i];
-        if (nkz != 0) {
-            for (int i = 0; i < cx; ++i) {
-                if (zi[i] == 0) continue;
-                for (int j = 0; j < cy; ++j) {
-                    const int32_t n = tz[i * cy + j];
-                    if (n) g += term64(n, zi[i], zj[j], nkz);
-                }
-            }
-        }
-        for (int k = 0; k < cxy; ++k) tz[k] = 0;
-        for (int i = 0; i < cx; ++i) zi[i] = 0;
-        for (int j = 0; j < cy; ++j) zj[j] = 0;
-    }
-    df = wave_sum_i(df);
-    wave_nsum(s, c);
-    as = wave_sum_f(as);
-    g = wave_sum_d(g);
-    if (lane == 0) {
-        sh.red_i[wave] = df;
-        sh.red_f[3 * wave] = s;
-        sh.red_f[3 * wave + 1] = c;
-        sh.red_f[3 * wave + 2] = as;
-        sh.red_d[wave] = g;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        WidePart P{0.0, 0.f, 0.f, 0.f, 0};
-        for (int w = 0; w < kWG / 64; ++w) {
-            P.df += sh.red_i[w];
-            nadd(P.s, P.c, sh.red_f[3 * w]);
-            P.c += sh.red_f[3 * w + 1];
-            P.as += sh.red_f[3 * w + 2];
-            P.g64 += sh.red_d[w];
-        }
-        parts[blockIdx.x] = P;
-    }
+    const WidePart P = range_partial(tab, it.z0, it.z1, cx, cy, ni, nj, true, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = P;
 }
 
 // Grid-wide finalize, part 2: one thread per test sums its partials in order and decides
@@ -1913,7 +2183,54 @@ __global__ __launch_bounds__(kWG) void k_build_planes(const uint8_t *cols, int64
     }
 }
 
+// The partitioning launch: a fixed grid, each workgroup taking items blockIdx.x,
+// blockIdx.x + gridDim.x, ... (items are alike: kPartItemChunks row chunks of one test
+// each, the test's Z columns staged once for them).
+template <bool kPacked>
+__global__ __launch_bounds__(kWG) void k_part_keys(PartArgs p)
+{
+    __shared__ uint32_t s_cnt[kPartMaxSlices];
+    __shared__ Shared sh;
+    const LaunchArgs &a = p.a;
+    const int tid = threadIdx.x;
+    for (int64_t it = blockIdx.x; it < p.n_items; it += gridDim.x) {
+        const uint64_t pk = p.items[it];
+        const int t = (int)(pk >> 32);
+        const int64_t c0 = (int64_t)(uint32_t)pk;
+        const DevTest T = a.tests[t];
+        const DevInst I = a.inst[T.inst];
+        const int64_t chunks = (I.n_rows + kPartRows - 1) / kPartRows;
+        const int64_t c1 = (c0 + kPartItemChunks < chunks) ? c0 + kPartItemChunks : chunks;
+        if (tid < T.d) {
+            const int32_t zc = a.zpool[T.z_off + tid];
+            sh.zcol[tid] = zc;
+            sh.zcard[tid] = a.card[zc];
+            sh.zw[tid] = a.colw[zc];
+        }
+        if (tid == 0) {
+            sh.wx = a.colw[T.x];
+            sh.wy = a.colw[T.y];
+        }
+        const bool skip = skipped(a, T);  // uniform: the flag cannot drop below round meanwhile
+        for (int64_t chunk = c0; chunk < c1 && !skip; ++chunk) {
+            for (int i = tid; i < T.n_slices; i += kWG) s_cnt[i] = 0u;
+            __syncthreads();  // (the first chunk: the Z columns staged as well)
+            part_item<kPacked>(a, I, T, sh, chunk, s_cnt);
+            __syncthreads();  // the next chunk clears the counters
+        }
+        __syncthreads();  // the next item restages the Z columns
+    }
+}
+
 }  // namespace
+
+hipError_t launch_part_keys(const PartArgs &p, bool packed, int grid, hipStream_t s)
+{
+    if (p.n_items <= 0) return hipSuccess;
+    if (packed) hipLaunchKernelGGL(k_part_keys<true>, dim3((unsigned)grid), dim3(kWG), 0, s, p);
+    else hipLaunchKernelGGL(k_part_keys<false>, dim3((unsigned)grid), dim3(kWG), 0, s, p);
+    return hipGetLastError();
+}
 
 hipError_t launch_group_count(const GroupArgs &g, int n_a, hipStream_t s)
 {

@@ -146,6 +146,8 @@ struct Slot {
     DevBuf<GroupChunk> d_gchunks;  // grouped level-1 chunks, their tests and work items
     DevBuf<int32_t> d_gtest;
     DevBuf<uint64_t> d_gitems;
+    DevBuf<uint64_t> d_pitems;  // partitioning work items (test << 32 | row chunk)
+    HostBuf<uint64_t> h_pitems;
     HostBuf<GroupChunk> h_gchunks;
     HostBuf<int32_t> h_gtest;
     HostBuf<uint64_t> h_gitems;
@@ -170,6 +172,7 @@ struct Slot {
     HostBuf<WideTest> h_wide_tests;
     std::vector<int32_t> mode_of;
     std::vector<int64_t> need;  // workspace int32 entries of each test of the batch being built
+    std::vector<int64_t> pneed; // partition-buffer int32 entries of each (kTFPart) test
     std::vector<uint8_t> fit;   // table and marginals fit the workgroup's LDS scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
     hipEvent_t ev_grp[kGroups + 1] = {};  // per-sink timing when tracing: start of each group
@@ -219,6 +222,8 @@ struct Slot {
         d_gchunks.release();
         d_gtest.release();
         d_gitems.release();
+        d_pitems.release();
+        h_pitems.release();
         h_gchunks.release();
         h_gtest.release();
         h_gitems.release();
@@ -339,6 +344,16 @@ struct fpgm_ctx {
     // per stage; 37 variables x 5 000 rows 0.90 -> 0.93 ms, x 200 000 rows 1.50 -> 1.74 ms,
     // 300 variables x 500 000 rows 8.83 -> 9.64 ms (few tests per round: the host's rounds
     // cost less than the extra launches)
+    // tables beyond LDS counted from rows partitioned by z slice first (k_part_keys, then
+    // the z-slice launch over each slice's rows only), for instances of at least
+    // part_min_rows rows: FPGM_PART=1 (default) the tables of more z slices than max_slices,
+    // which would otherwise go to device memory (1000 variables x 10^6 rows: stage
+    // 151.9 -> 149.7 ms, two runs each; the device-memory sink's 10.7 ms of atomics per
+    // stage gone), 2 every table beyond LDS (166.7 ms: for tables of a few slices the
+    // partitioning pass costs more than reading the rows once per slice), 0 none
+    int part = 1;
+    int64_t part_min_rows = 0;
+    DevBuf<int32_t> d_part;  // partition buffers of a launch sequence's partitioned tests
     int devrounds = -1;
     int64_t devround_max_rows = 65536;
     int64_t devround_min_tests = (int64_t)1 << 20;  // ... and levels whose walks hold at least this many tests
@@ -497,9 +512,11 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     DevTest *dt = S.h_tests.p;
     std::vector<int32_t> &gidx = S.mode_of;
     std::vector<int64_t> &need = S.need;
+    std::vector<int64_t> &pneed = S.pneed;
     std::vector<uint8_t> &fit = S.fit;
     gidx.resize((size_t)m);
     need.resize((size_t)m);
+    pneed.resize((size_t)m);
     fit.resize((size_t)m);
     int too_big = 0;
     // grouped level-1 counting: instances long enough, no table kept for a parity read, and
@@ -553,7 +570,23 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             // finalize kernels (the counted table of a parity read stays in place)
             if (!(tflags & kTFKeep)) T.flags |= kTFWide;
             const int64_t cxy = (int64_t)T.cx * T.cy;
-            if (cxy <= kSliceCells && !(ctx->sink_mask & 2)) {
+            // partitioned by z slice: the compile-time row loops (|Z| <= 8, every column of a
+            // packed context packed narrow), no table kept for a parity read
+            const bool part_ok = ctx->part && !(tflags & kTFKeep) && h.d <= 8 &&
+                                 (ctx->d_pcols == nullptr || narrow) &&
+                                 ctx->inst[h.inst].n_rows >= std::max<int64_t>(ctx->part_min_rows, 1);
+            if (part_ok && cxy <= kSliceCells) {
+                const int64_t sc = (kSliceCells / cxy) * cxy;
+                const int64_t ns = (cells + sc - 1) / sc;
+                // mode 1: the tables beyond the slice limit (else device memory); 2: all
+                if (ns <= kPartMaxSlices && (ctx->part == 2 || ns > ctx->max_slices || (ctx->sink_mask & 2))) {
+                    gi = 3;  // the z-slice launch, over partitioned rows
+                    T.n_slices = (int32_t)ns;
+                    T.slice_cells = (int32_t)sc;
+                    T.flags = (T.flags & ~kTFWide) | kTFPart;
+                }
+            }
+            if (!(T.flags & kTFPart) && cxy <= kSliceCells && !(ctx->sink_mask & 2)) {
                 const int64_t sc = (kSliceCells / cxy) * cxy;
                 const int64_t ns = (cells + sc - 1) / sc;
                 if (ns <= ctx->max_slices) {
@@ -617,9 +650,16 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
 #pragma omp parallel for if (nth > 1) num_threads(nth) schedule(static)
     for (int64_t k = 0; k < m; ++k) {
         DevTest &T = dt[k];
+        pneed[k] = 0;
         if (gidx[k] == kGroupMfma) {  // counted by the grouped kernel: no work items of its own
             T.n_chunks = 0;
             need[k] = 0;
+            continue;
+        }
+        if (T.flags & kTFPart) {  // one work item per slice over the partitioned rows; no workspace
+            T.n_chunks = 1;
+            need[k] = 0;
+            pneed[k] = part_layout(T.dimz, T.cx, T.cy, T.n_slices, ctx->inst[T.inst].n_rows).words;
             continue;
         }
         const int64_t r = rpc[gidx[k]];
@@ -631,7 +671,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         if (!need_ws) T.flags |= kTFLdsMarg;
         need[k] = need_ws ? (cells + marg + 63) / 64 * 64 : 0;
     }
-    int64_t ws_used = 0, n_items_total = 0, nb = 0;
+    int64_t ws_used = 0, part_used = 0, n_items_total = 0, nb = 0, n_pitems = 0;
     int64_t grp_begin[kGroups] = {0, 0, 0, 0, 0}, grp_len[kGroups] = {0, 0, 0, 0, 0};
     S.bytes = 0.0;
     S.pbytes = 0.0;
@@ -646,6 +686,14 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             if (need[nb] > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test workspace exceeds 4 GiB");
             T.ws_off = ws_used;
             ws_used += need[nb];
+        }
+        if (pneed[nb] > 0) {  // the partition buffer: the test's offset in ws_off
+            if (part_used + pneed[nb] > kWsMax && nb > 0) break;
+            if (pneed[nb] > kWsMax) return fail(ctx, FPGM_ETOOBIG, "test partition buffer exceeds 4 GiB");
+            T.ws_off = part_used;
+            part_used += pneed[nb];
+            const int64_t ch = (ctx->inst[T.inst].n_rows + kPartRows - 1) / kPartRows;
+            n_pitems += (ch + kPartItemChunks - 1) / kPartItemChunks;
         }
         const int64_t cnt = (int64_t)T.n_chunks * T.n_slices;
         T.item0 = (int32_t)grp_len[gidx[nb]];  // within the group; shifted by the group start below
@@ -686,6 +734,18 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         for (int64_t j = 0; j < cnt; ++j) S.h_items.p[dt[k].item0 + j] = ((uint64_t)k << 32) | (uint64_t)j;
     }
     const int64_t ni = n_items_total;
+    // partitioning work items: every row chunk of every partitioned test
+    if (n_pitems > 0) {
+        HIP_TRY(S.h_pitems.ensure(n_pitems, ctx->stream), "alloc pinned partition items");
+        int64_t q = 0;
+        for (int64_t k = 0; k < nb; ++k) {
+            if (!(dt[k].flags & kTFPart)) continue;
+            const int64_t ch = (ctx->inst[dt[k].inst].n_rows + kPartRows - 1) / kPartRows;
+            for (int64_t c = 0; c < ch; c += kPartItemChunks) S.h_pitems.p[q++] = ((uint64_t)k << 32) | (uint64_t)c;
+        }
+        HIP_TRY(S.d_pitems.ensure(n_pitems, false, ctx->stream), "alloc partition items");
+        HIP_TRY(ctx->d_part.ensure(part_used, false, ctx->stream), "alloc partition buffer");
+    }
     // device-memory tables: z ranges for the grid-wide finalize
     int64_t n_wi = 0, n_wt = 0;
     for (int64_t k = 0; k < nb; ++k)
@@ -877,6 +937,10 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
             "upload tests");
     HIP_TRY(hipMemcpyAsync(S.d_items.p, S.h_items.p, sizeof(uint64_t) * ni, hipMemcpyHostToDevice, ctx->stream),
             "upload items");
+    if (n_pitems > 0)
+        HIP_TRY(hipMemcpyAsync(S.d_pitems.p, S.h_pitems.p, sizeof(uint64_t) * n_pitems, hipMemcpyHostToDevice,
+                               ctx->stream),
+                "upload partition items");
     LaunchArgs a{};
     a.inst = ctx->d_inst;
     a.tests = S.d_tests.p;
@@ -895,6 +959,7 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
     a.res = S.d_res.p;
     a.edge_done = ctx->skip_round > 0 ? ctx->d_edge_done.p : nullptr;
     a.round = ctx->skip_round;
+    a.part = ctx->d_part.p;
     ctx->t_prep += now_s() - tp0;
     if (!graph) HIP_TRY(hipEventRecord(S.ev0, ctx->stream), "event");
     uint32_t *live_count = S.d_queue.p + kGroups * kXcds;
@@ -965,6 +1030,16 @@ fpgm_status batch_submit(fpgm_ctx *ctx, Slot &S, const HostTest *tests, int64_t 
         a.rows_per_chunk = rpc[gi];
         a.queue = S.d_queue.p + gi * kXcds;
         a.n_items_dev = a.edge_done ? live_count + gi : nullptr;
+        if (kGroupSink[gi] == kSinkSlice && n_pitems > 0) {
+            // the partitioned tests' rows grouped by z slice, before the slice launch counts them
+            PartArgs pa{};
+            pa.a = a;
+            pa.items = S.d_pitems.p;
+            pa.n_items = n_pitems;
+            const int pgrid = (int)std::min<int64_t>(n_pitems, (int64_t)ctx->n_cu * 3);
+            HIP_TRY(launch_part_keys(pa, ctx->d_pcols != nullptr, pgrid, gs), "launch partition");
+            ctx->k_launches += 1;
+        }
         const int sink = (gi == 1 && lane_w128) ? kSinkLaneW128 : kGroupSink[gi];
         const int64_t wg_per_cu = wpc_of(ctx, gi);
         int64_t grid = std::min<int64_t>(a.n_items, (int64_t)ctx->n_cu * wg_per_cu);
@@ -1622,6 +1697,8 @@ static fpgm_status ctx_init(fpgm_ctx *ctx, int32_t n_vars, const int32_t *card)
     if (const char *e = std::getenv("FPGM_GROUP")) ctx->group = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_GRAPH")) ctx->graph = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_DEVROUNDS")) ctx->devrounds = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("FPGM_PART")) ctx->part = std::min(2, std::max(0, std::atoi(e)));
+    if (const char *e = std::getenv("FPGM_PART_MIN_ROWS")) ctx->part_min_rows = std::max<int64_t>(0, std::atoll(e));
     if (const char *e = std::getenv("FPGM_BITS")) ctx->bits = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("FPGM_BITS_CELLS")) ctx->bits_cells = std::min(kLaneCells, std::max(0, std::atoi(e)));
     if (const char *e = std::getenv("FPGM_UPLOAD_PIPELINE")) ctx->upload_pipeline = std::atoi(e) != 0 ? 1 : 0;
@@ -1939,6 +2016,7 @@ void fpgm_ctx_destroy(fpgm_ctx *ctx)
     if (ctx->d_crit) (void)hipFree(ctx->d_crit);
     for (Slot &sl : ctx->slot) sl.release();
     ctx->d_ws.release();
+    ctx->d_part.release();
     ctx->d_done.release();
     ctx->d_flag.release();
     ctx->d_C.release();

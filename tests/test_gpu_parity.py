@@ -751,6 +751,35 @@ def test_sliced_and_device_memory_sinks():
     np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("n_rows", [1, 8192, 50_001, 300_007])
+def test_partitioned_large_tables(monkeypatch, n_rows, mode):
+    # tables beyond LDS counted from rows partitioned by z slice (whole row chunks of 8192
+    # rows, a partial one, a single row); card 8 throughout: |Z| = 3 -> 32768 cells (2
+    # slices: partitioned in mode 2, z slices in mode 1), 4 -> 262144 (16 slices), 5 -> 2^21
+    # (128); |Z| = 2 (4096 cells) stays in LDS
+    monkeypatch.setenv("FPGM_PART", mode)
+    s = datagen.random_sample(23, 9, n_rows, 8, 8)
+    D = O.Data.of(s)
+    rng = np.random.default_rng(7)
+    tests = []
+    for d in (2, 3, 4, 5):
+        for _ in range(4):
+            vs = rng.choice(9, size=d + 2, replace=False)
+            tests.append((int(min(vs[0], vs[1])), int(max(vs[0], vs[1])), tuple(sorted(int(v) for v in vs[2:]))))
+    arr = np.array([[x, y, len(z)] + list(z) + [0] * (5 - len(z)) for (x, y, z) in tests], dtype=np.int32)
+    o = O.ci_batch(D, arr, 5, 0.05, n_threads=8)
+    with _ctx(s) as ctx:
+        for flags in (0, _lib.FPGM_F_EXACT):
+            r = ctx.ci_batch(tests, alpha=0.05, flags=flags)
+            assert np.array_equal(r["df"], o["df"]) and np.array_equal(r["indep"], o["indep"])
+            if flags:
+                np.testing.assert_allclose(r["g2"], o["g2"], rtol=1e-11, atol=1e-9)
+        # a parity read keeps the counted table: those tests take the unpartitioned sinks
+        for (x, y, z) in tests[4:6]:
+            assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
+
+
 @pytest.mark.parametrize("min_rows", ["0", "1048576"], ids=["chunked", "whole-rows"])
 def test_slice_sink_whole_row_items(monkeypatch, min_rows):
     # z-slice tables (8193 .. 131072 cells) with the rows cut into chunks, whose slices are
