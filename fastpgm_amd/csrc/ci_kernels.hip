@@ -794,11 +794,11 @@ __device__ void part_scan(uint32_t *cnt, int S, uint32_t *seg, Shared &sh)
 }
 
 // One partitioning item: row chunk `chunk` of test T.  Every row's key once; its slice's
-// row count (LDS cnt, zero on entry); the scan; then each row's slice-relative key stored
+// row count (LDS cnt, zero on entry); the scan; then each row's slice-relative key placed
 // at its slice's next position within the chunk's kPartRows keys.
 template <bool kPacked>
 __device__ void part_item(const LaunchArgs &a, const DevInst &I, const DevTest &T, Shared &sh, int64_t chunk,
-                          uint32_t *cnt)
+                          uint32_t *cnt, uint16_t *stage)
 {
     const int64_t r0 = chunk * kPartRows;
     const int64_t r1 = (r0 + kPartRows < I.n_rows) ? r0 + kPartRows : I.n_rows;
@@ -829,14 +829,18 @@ __device__ void part_item(const LaunchArgs &a, const DevInst &I, const DevTest &
     const PartLayout L = part_layout(T.dimz, T.cx, T.cy, T.n_slices, I.n_rows);
     part_scan(cnt, T.n_slices, reinterpret_cast<uint32_t *>(base + L.seg) + chunk * T.n_slices * 2, sh);
     __syncthreads();
-    uint16_t *out = reinterpret_cast<uint16_t *>(base + L.keys) + chunk * kPartRows;
+    // placed in LDS first (stage: kPartRows keys), then written out in 16-byte stores
 #pragma unroll
     for (int i = 0; i < K; ++i)
         if (key[i] != ~0u) {
             uint32_t rel;
             const uint32_t s = part_slice_of(key[i], sc, inv, rel);
-            out[atomicAdd(&cnt[s], 1u)] = (uint16_t)rel;
+            stage[atomicAdd(&cnt[s], 1u)] = (uint16_t)rel;
         }
+    __syncthreads();
+    uint4 *out = reinterpret_cast<uint4 *>(reinterpret_cast<uint16_t *>(base + L.keys) + chunk * kPartRows);
+    const int n16 = (int)((r1 - r0 + 7) / 8);  // 8 keys per store; keys past the rows are never read
+    for (int i = threadIdx.x; i < n16; i += kWG) out[i] = reinterpret_cast<const uint4 *>(stage)[i];
 }
 
 // ---------------------------------------------------------------- bit-plane counting
@@ -2190,6 +2194,7 @@ template <bool kPacked>
 __global__ __launch_bounds__(kWG) void k_part_keys(PartArgs p)
 {
     __shared__ uint32_t s_cnt[kPartMaxSlices];
+    __shared__ __align__(16) uint16_t s_stage[kPartRows];
     __shared__ Shared sh;
     const LaunchArgs &a = p.a;
     const int tid = threadIdx.x;
@@ -2215,7 +2220,7 @@ __global__ __launch_bounds__(kWG) void k_part_keys(PartArgs p)
         for (int64_t chunk = c0; chunk < c1 && !skip; ++chunk) {
             for (int i = tid; i < T.n_slices; i += kWG) s_cnt[i] = 0u;
             __syncthreads();  // (the first chunk: the Z columns staged as well)
-            part_item<kPacked>(a, I, T, sh, chunk, s_cnt);
+            part_item<kPacked>(a, I, T, sh, chunk, s_cnt, s_stage);
             __syncthreads();  // the next chunk clears the counters
         }
         __syncthreads();  // the next item restages the Z columns
