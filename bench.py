@@ -253,6 +253,8 @@ def kernel_family(name):
         return SINK_OF_KIND.get(k[len("k_count_stat<"):].split(",")[0].strip(), "other")
     if k.startswith("k_fin_wide"):
         return "device"
+    if k.startswith("k_part_keys"):  # the partitioning pass of the z-slice launch's partitioned tables
+        return "slice"
     if "copyBuffer" in name:
         return "copy"
     return k.split("<")[0]
@@ -515,8 +517,9 @@ def summarize_trace(trace_dir, out_path):
         "  lane       k_count_stat<2>  per-lane 16-bit LDS counters, tables <= 64 cells",
         "  lane-wide  k_count_stat<3>, <5>  the same with 48 / 64 KiB of counters, <= 96 / 128 cells",
         "  shared     k_count_stat<0>  shared LDS histogram, tables <= 8192 cells",
-        "  slice      k_count_stat<4>  z-sliced LDS histogram (16384-cell slices)",
-        "  device     k_count_stat<1> + k_fin_wide_*  device-memory table",
+        "  slice      k_count_stat<4>  z-sliced LDS histogram (16384-cell slices); tables of more slices",
+        "             than the slice limit partitioned by slice first (k_part_keys, counted here)",
+        "  device     k_count_stat<1> + k_fin_wide_*  device-memory table (beyond 4096 slices)",
         "  bits       k_count_stat<6>  |Z| = 1 tables <= 48 cells from bit planes (AND + population count)",
         "  k_gram_l0  level 0 as the int8 one-hot Gram matrix (v_mfma_i32_32x32x32_i8)",
         "",
