@@ -507,10 +507,10 @@ def summarize_trace(trace_dir, out_path):
     lines = [
         f"k_count_stat / k_gram_l0 -- kernel trace, workload {run['config']['workload']}",
         "Command (GPU box, repository root, TMPDIR=/tmp):",
-        "  rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_kt -o w3 -- python bench.py --steps 1 \\",
+        f"  rocprofv3 --kernel-trace --stats -f csv -d {trace_dir} -o w3 -- python bench.py --steps 1 \\",
         "      --warmup 0 --no-cpu --no-plain --no-traffic --no-g2 --tune-file profiles/tune_dag-1000v-1500e-1m.json",
-        "  (its JSON line saved as gpurun_out/prof_kt/run.json), then, on any host:",
-        "  python bench.py --summarize-trace gpurun_out/prof_kt",
+        f"  (its JSON line saved as {trace_dir}/run.json), then, on any host:",
+        f"  python bench.py --summarize-trace {trace_dir}",
         f"The run executes the CI stage {stages} times (timed step + read-back run); per-stage figures",
         "below are the trace totals divided by the stage count.",
         "Sinks (the counting kernels read the packed columns, 2, 4 or 8 bits per value, or the bit planes):",
@@ -611,12 +611,12 @@ def summarize_sq(sq_dir, out_path, trace_dir=None):
     lines = ["kernel_stats of SQ counters (totals over all dispatches of each kernel)",
              "Command (GPU box, repository root, TMPDIR=/tmp):",
              "  rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES \\",
-             "      SQ_INSTS_VMEM_RD SQ_LDS_BANK_CONFLICT SQ_INSTS_SALU -f csv -d gpurun_out/pmc_sq -o sq -- \\",
+             f"      SQ_INSTS_VMEM_RD SQ_LDS_BANK_CONFLICT SQ_INSTS_SALU -f csv -d {sq_dir} -o sq -- \\",
              "      python bench.py --steps 1 --warmup 0 --no-cpu --no-plain --no-traffic --no-g2 \\",
              "      --tune-file profiles/tune_dag-1000v-1500e-1m.json",
-             "  (its JSON line saved as gpurun_out/pmc_sq/run.json), then, with the kernel-trace run of the",
-             "  same command in gpurun_out/prof_kt:",
-             "  python bench.py --summarize-sq gpurun_out/pmc_sq --sq-times gpurun_out/prof_kt",
+             f"  (its JSON line saved as {sq_dir}/run.json), then, with the kernel-trace run of the",
+             f"  same command in {trace_dir or 'DIR'}:",
+             f"  python bench.py --summarize-sq {sq_dir} --sq-times {trace_dir or 'DIR'}",
              "kernel                 disp " + " ".join(f"{n[3:]:>17s}" for n in names)]
     for k in sorted(agg, key=lambda k: -agg[k]["SQ_BUSY_CYCLES"]):
         d = agg[k]
