@@ -780,6 +780,35 @@ def test_partitioned_large_tables(monkeypatch, n_rows, mode):
             assert np.array_equal(ctx.counts(x, y, z), O.counts(D, x, y, z))
 
 
+def _latent_sample(seed, n_vars, n_rows, card=8):
+    # every column = (shared latent + its own noise in 0..2) mod card: no conditioning set
+    # screens a pair off, so every edge survives to the deepest level and the tables grow to
+    # card^(|Z| + 2) cells (|Z| = 4: 262 144 cells, 16 z slices)
+    rng = np.random.default_rng(seed)
+    cols = np.zeros((n_vars, datagen.padded_stride(n_rows)), dtype=np.uint8)
+    w = rng.integers(0, card, n_rows)
+    for v in range(n_vars):
+        cols[v, :n_rows] = (w + rng.integers(0, 3, n_rows)) % card
+    return datagen.Sample(cols=cols, n_rows=n_rows, card=np.full(n_vars, card, dtype=np.int32))
+
+
+@pytest.mark.parametrize("mode", ["0", "1", "2"])
+def test_partitioned_tables_in_a_multi_instance_stage(monkeypatch, mode):
+    # three instances of different lengths (one row chunk; several; a partial last one) in
+    # one level-synchronous stage to depth 5, whose levels 3 and 4 hold 2- and 16-slice
+    # tables: z slices and device memory (mode 0), partitioned beyond the slice limit (1),
+    # partitioned throughout (2); each instance's skeleton and test count against the oracle
+    monkeypatch.setenv("FPGM_PART", mode)
+    reps = [_latent_sample(40 + i, 8, n) for i, n in enumerate((6_000, 20_011, 50_001))]
+    with _lib.Context.of_many(reps) as ctx:
+        g = ctx.pc_skeleton_multi(0.05, 5)
+    for i, r in enumerate(reps):
+        o = O.pc_skeleton(O.Data.of(r), 0.05, 5, n_threads=8)
+        assert o["tests_per_level"][4] > 0
+        assert np.array_equal(g["adj"][i], o["adj"]), i
+        assert g["num_ci_test_per_instance"][i] == o["num_ci_test"], i
+
+
 @pytest.mark.parametrize("min_rows", ["0", "1048576"], ids=["chunked", "whole-rows"])
 def test_slice_sink_whole_row_items(monkeypatch, min_rows):
     # z-slice tables (8193 .. 131072 cells) with the rows cut into chunks, whose slices are
