@@ -98,6 +98,7 @@ struct WideTest {
 constexpr int kPartRows = 8192;       // rows per partitioning work item (2 workgroup steps)
 constexpr int kPartStage = 128;       // row chunks whose segments a slice item stages at a time
 constexpr int kPartItemChunks = 8;    // row chunks per partitioning work item
+constexpr int kPartPrivSlices = 16;   // at most this many slices: a private counter per slice and thread
 constexpr int kPartMaxSlices = 4096;  // z slices of a partitioned table at most
 struct PartLayout {
     int64_t parts, seg, keys, words;

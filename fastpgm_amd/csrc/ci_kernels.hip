@@ -793,6 +793,33 @@ __device__ void part_scan(uint32_t *cnt, int S, uint32_t *seg, Shared &sh)
     }
 }
 
+// Exclusive scan, in place, of the S * kWG private slice counters cnt[s * kWG + thread]
+// in that order; thread t sums words [t S, t S + S).  Leaves the total in
+// sh.red_i[kWG / 64 - 1] (as the last wave's inclusive sum plus the waves before).
+__device__ void part_scan_flat(uint32_t *cnt, int S, Shared &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = tid * S;
+    uint32_t loc = 0;
+    for (int j = 0; j < S; ++j) loc += cnt[i0 + j];
+    uint32_t inc = loc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) sh.red_i[wave] = (int32_t)inc;
+    __syncthreads();
+    uint32_t off = inc - loc;
+    for (int w = 0; w < wave; ++w) off += (uint32_t)sh.red_i[w];
+    for (int j = 0; j < S; ++j) {
+        const uint32_t c = cnt[i0 + j];
+        cnt[i0 + j] = off;
+        off += c;
+    }
+    __syncthreads();
+    if (tid == kWG - 1) sh.red_i[kWG / 64 - 1] = (int32_t)off;  // the total
+}
+
 // One partitioning item: row chunk `chunk` of test T.  Every row's key once; its slice's
 // row count (LDS cnt, zero on entry); the scan; then each row's slice-relative key placed
 // at its slice's next position within the chunk's kPartRows keys.
@@ -818,25 +845,61 @@ __device__ void part_item(const LaunchArgs &a, const DevInst &I, const DevTest &
     }
     const uint32_t sc = (uint32_t)T.slice_cells;
     const float inv = 1.0f / (float)sc;
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-        if (key[i] != ~0u) {
-            uint32_t rel;
-            atomicAdd(&cnt[part_slice_of(key[i], sc, inv, rel)], 1u);
-        }
-    __syncthreads();
     int32_t *base = a.part + T.ws_off;
     const PartLayout L = part_layout(T.dimz, T.cx, T.cy, T.n_slices, I.n_rows);
-    part_scan(cnt, T.n_slices, reinterpret_cast<uint32_t *>(base + L.seg) + chunk * T.n_slices * 2, sh);
-    __syncthreads();
-    // placed in LDS first (stage: kPartRows keys), then written out in 16-byte stores
+    uint32_t *seg = reinterpret_cast<uint32_t *>(base + L.seg) + chunk * T.n_slices * 2;
+    const int S = T.n_slices;
+    if (S <= kPartPrivSlices) {
+        // few slices: a private counter per slice and thread, cnt[s * kWG + thread] (a wave's
+        // 64 adds to one slice would otherwise serialise on one word); the scan over them in
+        // that order places each thread's keys of a slice together, threads in order
+        const int tid = threadIdx.x;
+        for (int s2 = 0; s2 < S; ++s2) cnt[s2 * kWG + tid] = 0u;  // the thread's own words
 #pragma unroll
-    for (int i = 0; i < K; ++i)
-        if (key[i] != ~0u) {
-            uint32_t rel;
-            const uint32_t s = part_slice_of(key[i], sc, inv, rel);
-            stage[atomicAdd(&cnt[s], 1u)] = (uint16_t)rel;
+        for (int i = 0; i < K; ++i)
+            if (key[i] != ~0u) {
+                uint32_t rel;
+                ++cnt[part_slice_of(key[i], sc, inv, rel) * kWG + tid];
+            }
+        __syncthreads();
+        part_scan_flat(cnt, S, sh);
+        __syncthreads();
+        uint32_t so = 0, sn = 0;  // slice tid's segment (read before the counters move on)
+        if (tid < S) {
+            so = cnt[tid * kWG];
+            sn = ((tid + 1 < S) ? cnt[(tid + 1) * kWG] : (uint32_t)sh.red_i[kWG / 64 - 1]) - so;
         }
+        __syncthreads();
+        if (tid < S) {
+            seg[2 * tid] = so;
+            seg[2 * tid + 1] = sn;
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (key[i] != ~0u) {
+                uint32_t rel;
+                const uint32_t s2 = part_slice_of(key[i], sc, inv, rel);
+                stage[cnt[s2 * kWG + tid]++] = (uint16_t)rel;
+            }
+    } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (key[i] != ~0u) {
+                uint32_t rel;
+                atomicAdd(&cnt[part_slice_of(key[i], sc, inv, rel)], 1u);
+            }
+        __syncthreads();
+        part_scan(cnt, S, seg, sh);
+        __syncthreads();
+        // placed in LDS first (stage: kPartRows keys), then written out in 16-byte stores
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (key[i] != ~0u) {
+                uint32_t rel;
+                const uint32_t s2 = part_slice_of(key[i], sc, inv, rel);
+                stage[atomicAdd(&cnt[s2], 1u)] = (uint16_t)rel;
+            }
+    }
     __syncthreads();
     uint4 *out = reinterpret_cast<uint4 *>(reinterpret_cast<uint16_t *>(base + L.keys) + chunk * kPartRows);
     const int n16 = (int)((r1 - r0 + 7) / 8);  // 8 keys per store; keys past the rows are never read
